@@ -1,0 +1,142 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 SGH zone-cycles/s on one MI355X, with HBM roofline and a CPU baseline.
+
+Workload: the 3D point-energy blast (Sedov-type) on the [0,1.2]^3 uniform
+hex mesh at 256^3 elements (16,777,216 zones, 16,974,593 nodes), Morton
+renumbered, ideal gas gamma=5/3, q1=1, q2=1.333, RK2, CFL 0.5.  A "step" is
+one full RK2 cycle of the fused, device-resident cycle loop (dt reduction,
+2 x {corner force, velocity+BC+position, energy+volume+EOS+divergence}).
+Inputs are resident in HBM before the timed region starts.
+
+One JSON line on stdout (rank 0).  --gpus N>1 is not supported: this build
+runs one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, cycles, threads):
+    """The oracle (a C/OpenMP restatement of the reference operators) timed
+    on the host cores over a bounded sample of the same blast problem."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "threads")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # noqa: E402  (test/baseline infrastructure)
+    from fierro_amd import problems as P
+
+    spec = P.sedov(n, cycles=cycles)
+    pr = O.OracleProblem(spec, fast=True)
+    r = pr.solve()
+    zc = pr.E * r["cycles"] / r["seconds"]
+    return {"value": zc, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "sample": f"sedov {n}^3 ({pr.E} zones), {r['cycles']} RK2 cycles, "
+                      f"{r['seconds']:.2f} s, oracle/liboracle_sgh_fast.so (gcc -O3, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256, help="elements per edge")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-cycles", type=int, default=100)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from fierro_amd import bytemodel, problems as P
+    from fierro_amd.sgh import SGHSolver
+
+    n = args.n
+    spec = P.sedov(n, cycles=10 ** 9)
+    t0 = time.perf_counter()
+    g = SGHSolver(spec, renumber=True)
+    log(f"setup {n}^3: E={g.E} N={g.N} in {time.perf_counter() - t0:.1f} s")
+    g.run_begin()
+    g.run(args.warmup, use_graph=True)
+    log(f"warmup {args.warmup} cycles done")
+
+    # graph-replayed loop, no per-kernel events
+    t = time.perf_counter()
+    g.run(args.steps, use_graph=True)
+    graph_s = time.perf_counter() - t
+    log(f"graph loop: {args.steps} cycles in {graph_s:.3f} s")
+
+    # timed region: per-kernel HIP events on the context stream
+    g.profile(True)
+    t = time.perf_counter()
+    g.run(args.steps, use_graph=False)
+    wall = time.perf_counter() - t
+    prof = g.profile_read()
+    g.profile(False)
+    st = g.status(0)
+    log(f"timed loop: {args.steps} cycles in {wall:.3f} s; status {st}")
+    ie, ke = g.energy(0)
+
+    E, N = g.E, g.N
+    kernels = {}
+    for name, fn in bytemodel.KERNELS.items():
+        ms, calls = prof.get(name, (0.0, 0))
+        avg = ms / max(calls, 1)
+        b = fn(E, N)
+        kernels[name] = {"avg_ms": avg, "calls": calls, "alg_bytes": b,
+                         "GBs": b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["calls"])
+    d = kernels[dom]
+    cyc_bytes = bytemodel.cycle(E, N)
+    value = E * args.steps / wall
+    out = {
+        "metric": "FP64 SGH zone-cycles/s (Sedov 256^3 blast, plain mesh)",
+        "value": value,
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated Sedov blast mesh and initial state)",
+        "config": {"workload": f"sedov{n}", "elements": E, "nodes": N, "rk_stages": 2,
+                   "renumber": "morton", "cfl": 0.5, "q2": 1.333},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": None},
+        "kernels": kernels,
+        "cycle_alg_GBs": cyc_bytes * args.steps / wall / 1e9,
+        "value_graph": E * args.steps / graph_s,
+        "energy": {"ie": ie, "ke": ke, "time": st["time_value"]},
+    }
+    del g
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        try:
+            out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_cycles, threads)
+        except Exception as exc:  # the baseline is reported, never the product
+            out["cpu_baseline"] = {"value": None, "error": repr(exc)}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

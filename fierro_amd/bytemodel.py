@@ -1,0 +1,32 @@
+"""Algorithmic HBM bytes of the SGH kernels (unique bytes a launch must move).
+
+Counting rule: every array element a kernel must read or write is counted
+once (FP64 = 8 B, int32 index = 4 B); values recomputed on chip or re-read
+from cache count 0.  The ideal-gas, strength-free path reads no stress; with
+one material the material id is not read.  E elements, N nodes, C = 8E.
+"""
+from __future__ import annotations
+
+
+def force(E, N):
+    # conn 32E + x1,v1 48N + vol,div,den,sspd,pres 40E + corner CSR (4C + 4(N+1)) + F 24C
+    return 32 * E + 48 * N + 40 * E + (32 * E + 4 * (N + 1)) + 192 * E
+
+
+def vel_bc_pos(E, N):
+    # F 24C + corner CSR + m 8N + v0 24N + v1 24N (w) + x0 24N + x1 24N (w) + BC mask 1N
+    return 192 * E + (32 * E + 4 * (N + 1)) + 8 * N + 96 * N + N
+
+
+def energy_state_div(E, N):
+    # F 24C + conn 32E + v1,v0,x1 72N + M, e0 16E + e1,V,rho,p,c,div 48E (w) + gamma,csmin 16E
+    return 192 * E + 32 * E + 72 * N + 16 * E + 48 * E + 16 * E
+
+
+KERNELS = {"force": force, "vel_bc_pos": vel_bc_pos, "energy_state_div": energy_state_div}
+
+
+def cycle(E, N):
+    """Bytes of one fused RK2 cycle (2 stages; dt and control kernels are O(1)
+    per block)."""
+    return 2 * (force(E, N) + vel_bc_pos(E, N) + energy_state_div(E, N))

@@ -1,0 +1,103 @@
+// fsgh_internal.h -- shared declarations of the SGH HIP library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fsgh.h"
+
+namespace fsgh {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// One workgroup's work item: a contiguous range [start, end) of elements (or
+// nodes) inside one instance.  `lid` is the logical (instance-ordered) block id.
+struct BlockItem {
+    int32_t inst;
+    int32_t start;
+    int32_t end;
+    int32_t lid;
+};
+
+// Per-instance control block, device resident (dt and time never leave the
+// device inside the cycle loop).
+struct InstCtl {
+    double dt;               // dt of the current cycle
+    double time;             // time_value
+    double graphics_time;
+    double time_final, dt_max, dt_min, dt_cfl, graphics_dt_ival;
+    double fuzz, small;
+    long long cycle;         // cycles completed
+    long long cycle_stop, graphics_cyc_ival;
+    int32_t stopped;
+    int32_t final_bin;       // physical bin holding the latest state once stopped
+    int32_t eblk_begin, eblk_end;   // logical element-block range of the instance
+};
+
+struct Material {
+    double q1, q1ex, q2, q2ex;
+};
+
+// Device view of the mesh (new numbering, int32 indices).
+struct DMesh {
+    int32_t N, E;
+    const int32_t* conn;       // E x 8
+    const int32_t* cin_start;  // N+1
+    const int32_t* cin;        // 8E corner ids, each node's list in ascending original-element order
+    const BlockItem* eblk;     // element work items, XCD-ordered
+    const BlockItem* nblk;     // node work items, XCD-ordered
+    int32_t n_eblk, n_nblk;
+};
+
+// Device view of the state (structure of arrays); bin 0 = t_n, bin 1 = current.
+struct DState {
+    double *x0, *y0, *z0, *x1, *y1, *z1;
+    double *u0, *v0, *w0, *u1, *v1, *w1;
+    double* nmass;
+    const uint8_t* bcmask;     // per node: bit d set -> velocity component d forced to 0
+    double *den, *pres, *sspd, *sie0, *sie1, *vol, *div, *emass, *gamma, *csmin;
+    const int32_t* mat;        // material id per element
+    double* F;                 // 8E x 3 corner forces
+    const Material* mats;
+    int32_t nmat;
+};
+
+void set_error(fsgh_ctx* ctx, const std::string& msg);
+
+}  // namespace fsgh
+
+// ---- kernel launchers (fsgh_kernels.hip) ----
+namespace fsgh {
+void launch_get_vol(const DMesh&, const DState&, hipStream_t);
+void launch_get_divergence(const DMesh&, const DState&, hipStream_t);
+void launch_force(const DMesh&, const DState&, const InstCtl*, double fuzz, double small,
+                  hipStream_t);
+void launch_update_velocity(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
+void launch_boundary_velocity(const int32_t* nodes, const uint8_t* masks, int32_t count,
+                              const DState&, hipStream_t);
+void launch_update_energy(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
+void launch_update_position(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
+void launch_update_state(const DMesh&, const DState&, hipStream_t);
+void launch_rk_init(const DMesh&, const DState&, hipStream_t);
+// dt partial minima per logical element block (dt_lcl of K/time_integration.cpp:74-140)
+void launch_dt_partial(const DMesh&, const DState&, const InstCtl*, double* partial,
+                       double dt_cfl, double dt_max, double dt_min, double fuzz, int use_ctl,
+                       hipStream_t);
+void launch_reduce_min(const double* partial, int32_t n, double* out, hipStream_t);
+// fused cycle kernels
+void launch_cycle_dt(InstCtl* ctl, int32_t ninst, const double* partial, hipStream_t);
+void launch_vel_bc_pos(const DMesh&, const DState&, const InstCtl*, double alpha, hipStream_t);
+void launch_energy_state_div(const DMesh&, const DState&, const InstCtl*, double alpha,
+                             double* dt_partial, hipStream_t);
+void launch_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin, hipStream_t);
+void launch_energy_sums(const DMesh&, const DState&, const int32_t* eoff, const int32_t* noff,
+                        int32_t inst, double* out2, double* scratch, int32_t nscratch,
+                        hipStream_t);
+void launch_swap_bins_range(const DState&, int32_t n0, int32_t n1, int32_t e0, int32_t e1,
+                            hipStream_t);
+void launch_check(const DMesh&, const DState&, int32_t* flag, hipStream_t);
+}  // namespace fsgh

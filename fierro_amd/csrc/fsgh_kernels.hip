@@ -1,0 +1,748 @@
+// fsgh_kernels.hip -- CDNA4 (gfx950) FP64 kernels of the SGH cycle.
+//
+// Every kernel walks a table of BlockItems: one workgroup = one contiguous
+// element (or node) range inside one instance, dealt so that consecutive
+// ranges of the Morton-ordered mesh land on the same XCD (shared L2).
+//
+// Floating point: the library is compiled with -ffp-contract=off and every
+// expression keeps the reference's operation order (file:line cited at each
+// kernel), so results match the reference arithmetic bit for bit except for
+// pow() (ocml vs. the host libm) and reductions whose order the reference
+// leaves to Kokkos.
+#include <cfloat>
+#include <cmath>
+
+#define FSGH_HD __host__ __device__
+#include "fsgh_internal.h"
+#include "hex_geometry.inc"
+
+namespace fsgh {
+namespace {
+
+__device__ __forceinline__ void load_conn(const int32_t* conn, int e, int32_t nd[8])
+{
+    const int4* p = reinterpret_cast<const int4*>(conn + 8 * static_cast<size_t>(e));
+    const int4 a = p[0];
+    const int4 b = p[1];
+    nd[0] = a.x; nd[1] = a.y; nd[2] = a.z; nd[3] = a.w;
+    nd[4] = b.x; nd[5] = b.y; nd[6] = b.z; nd[7] = b.w;
+}
+
+__device__ __forceinline__ void gather3(const double* __restrict__ a, const double* __restrict__ b,
+                                        const double* __restrict__ c, const int32_t nd[8],
+                                        double x[8], double y[8], double z[8])
+{
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        x[i] = a[nd[i]];
+        y[i] = b[nd[i]];
+        z[i] = c[nd[i]];
+    }
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block-wide min over kBlock threads; result valid in thread 0
+__device__ __forceinline__ double block_min(double v)
+{
+    __shared__ double red[kBlock / 64];
+    v = wave_min(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int i = 1; i < kBlock / 64; ++i) r = fmin(r, red[i]);
+    return r;
+}
+
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double red[kBlock / 64];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int i = 1; i < kBlock / 64; ++i) r = r + red[i];
+    __syncthreads();
+    return r;
+}
+
+// velocity divergence from B, v and V (K/momentum.cpp:265-286)
+__device__ __forceinline__ double divergence(const double B[24], const double u[8],
+                                             const double v[8], const double w[8], double vol)
+{
+    const double inverse_vol = 1.0 / vol;
+    double sx = u[0] * B[0], sy = v[0] * B[1], sz = w[0] * B[2];
+#pragma unroll
+    for (int a = 1; a < 8; ++a) {
+        sx = sx + u[a] * B[3 * a + 0];
+        sy = sy + v[a] * B[3 * a + 1];
+        sz = sz + w[a] * B[3 * a + 2];
+    }
+    double div = 0.0;
+    div += sx * inverse_vol;
+    div += sy * inverse_vol;
+    div += sz * inverse_vol;
+    return div;
+}
+
+// shortest of the 28 node-pair distances (K/time_integration.cpp:101-128).
+// sqrt is correctly rounded and monotone, so min(sqrt(d2)) == sqrt(min(d2)).
+__device__ __forceinline__ double min_pair_distance(const double x[8], const double y[8],
+                                                    const double z[8])
+{
+    double m = DBL_MAX;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = a + 1; b < 8; ++b) {
+            const double dx = x[b] - x[a], dy = y[b] - y[a], dz = z[b] - z[a];
+            m = fmin(m, dx * dx + dy * dy + dz * dz);
+        }
+    return sqrt(m);
+}
+
+// local CFL dt of one element, clamped (K/time_integration.cpp:131-135)
+__device__ __forceinline__ double elem_dt(double dist_min, double sspd, double dt_cfl,
+                                          double dt_max, double dt_min, double fuzz)
+{
+    double d = dt_cfl * dist_min / (sspd + fuzz);
+    d = fmin(d, dt_max);
+    d = fmax(d, dt_min);
+    return d;
+}
+
+// ideal gas (K/eos.cpp:30-43)
+__device__ __forceinline__ void ideal_gas(double gamma, double csmin, double den, double sie,
+                                          double& pres, double& sspd)
+{
+    pres = (gamma - 1.0) * sie * den;
+    double c = sqrt(gamma * (gamma - 1.0) * sie);
+    if (c < csmin) c = csmin;
+    sspd = c;
+}
+
+// --------------------------------------------------------------------------
+// corner force, 3D, strength-free (K/force_sgh.cpp:35-359)
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_force(DMesh m, DState s, const InstCtl* ctl,
+                                                  double fuzz, double small)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    if (ctl != nullptr) {
+        if (ctl[bi.inst].stopped) return;
+        fuzz = ctl[bi.inst].fuzz;
+        small = ctl[bi.inst].small;
+    }
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+
+    int32_t nd[8];
+    load_conn(m.conn, e, nd);
+    double x[8], y[8], z[8];
+    gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+    double B[24];
+    hex_bmatrix(x, y, z, B);
+
+    double u[8], v[8], w[8];
+    gather3(s.u1, s.v1, s.w1, nd, u, v, w);
+    const double vol = s.vol[e];
+    const double div = s.div[e];
+    const double den = s.den[e];
+    const double sspd = s.sspd[e];
+    const double pres = s.pres[e];
+    const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
+
+    // MARS: cell-average velocity estimate (:147-164)
+    double vsx = 0.0, vsy = 0.0, vsz = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        vsx += 0.125 * u[a];
+        vsy += 0.125 * v[a];
+        vsz += 0.125 * w[a];
+    }
+    // impedance x |corner area| per node (:177-255); compression vs expansion (:217-224)
+    const double qa = div < 0 ? mt.q1 : mt.q1ex;
+    const double qb = div < 0 ? mt.q2 : mt.q2ex;
+    double sum0 = 0.0, sum1 = 0.0, sum2 = 0.0, sum3 = 0.0;
+    double mu[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const double dx = u[a] - vsx, dy = v[a] - vsy, dz = w[a] - vsz;
+        const double mag_vel = sqrt(dx * dx + dy * dy + dz * dz);
+        const double muc = den * (qa * sspd + qb * mag_vel);
+        const double bx = B[3 * a], by = B[3 * a + 1], bz = B[3 * a + 2];
+        const double mu_term = muc * sqrt(bx * bx + by * by + bz * bz);
+        sum0 += mu_term * u[a];
+        sum1 += mu_term * v[a];
+        sum2 += mu_term * w[a];
+        sum3 += mu_term;
+        mu[a] = mu_term;
+    }
+    // Riemann velocity (:261-270)
+    if (sum3 > fuzz) {
+        vsx = sum0 / sum3;
+        vsy = sum1 / sum3;
+        vsz = sum2 / sum3;
+    } else {
+        vsx = 0.0;
+        vsy = 0.0;
+        vsz = 0.0;
+    }
+
+    // Shock detector (:287-328).  r_face = 0.9 (d_nb + small)/(div + small) is
+    // monotone in d_nb (every rounding step is monotone), so the minimum over
+    // neighbours is reached at min d_nb when div+small >= 0 and at max d_nb
+    // otherwise.  Neighbours = every other element sharing a node (the
+    // reference's elems_in_elem), walked through the corner CSR.
+    const double b = div + small;
+    const bool lo = b >= 0.0;
+    double ext = lo ? HUGE_VAL : -HUGE_VAL;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int q0 = m.cin_start[nd[a]], q1 = m.cin_start[nd[a] + 1];
+        for (int q = q0; q < q1; ++q) {
+            const int e2 = m.cin[q] >> 3;
+            if (e2 != e) {
+                const double d2 = s.div[e2];
+                ext = lo ? fmin(ext, d2) : fmax(ext, d2);
+            }
+        }
+    }
+    const double r_face = 0.9 * (ext + small) / b;
+    const double r_min = fmin(r_face, 1.0);
+    double phi = 1.0 - fmax(0.0, r_min);  // pow(phi, 1.0) == phi
+    const double c_length = pow(vol, 1.0 / 3.0);
+    const double alpha = fmin(1.0, 20.0 * (c_length * fabs(div)) / (sspd + fuzz));
+    phi = alpha * phi;
+
+    // corner forces (:338-359): a = -B, tau = -p I, so a_i . tau_d = B_id p
+    double2* out = reinterpret_cast<double2*>(s.F + 24 * static_cast<size_t>(e));
+    double f[24];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const double pm = phi * mu[a];
+        f[3 * a + 0] = B[3 * a + 0] * pres + pm * (vsx - u[a]);
+        f[3 * a + 1] = B[3 * a + 1] * pres + pm * (vsy - v[a]);
+        f[3 * a + 2] = B[3 * a + 2] * pres + pm * (vsz - w[a]);
+    }
+#pragma unroll
+    for (int i = 0; i < 12; ++i) out[i] = make_double2(f[2 * i], f[2 * i + 1]);
+}
+
+// --------------------------------------------------------------------------
+// operator kernels with the reference's per-operator semantics
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_get_vol(DMesh m, DState s)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    int32_t nd[8];
+    load_conn(m.conn, e, nd);
+    double x[8], y[8], z[8];
+    gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+    s.vol[e] = hex_volume(x, y, z);
+}
+
+__global__ __launch_bounds__(kBlock) void k_get_divergence(DMesh m, DState s)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    int32_t nd[8];
+    load_conn(m.conn, e, nd);
+    double x[8], y[8], z[8], u[8], v[8], w[8], B[24];
+    gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+    hex_bmatrix(x, y, z, B);
+    gather3(s.u1, s.v1, s.w1, nd, u, v, w);
+    s.div[e] = divergence(B, u, v, w, s.vol[e]);
+}
+
+// K/momentum.cpp:23-49: CSR gather in corners_in_node order, no atomics
+__device__ __forceinline__ void gather_node_force(const DMesh& m, const double* __restrict__ F,
+                                                  int n, double& f0, double& f1, double& f2)
+{
+    f0 = 0.0;
+    f1 = 0.0;
+    f2 = 0.0;
+    const int q0 = m.cin_start[n], q1 = m.cin_start[n + 1];
+    for (int q = q0; q < q1; ++q) {
+        const size_t c = static_cast<size_t>(m.cin[q]);
+        f0 += F[3 * c + 0];
+        f1 += F[3 * c + 1];
+        f2 += F[3 * c + 2];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_velocity(DMesh m, DState s, double alpha,
+                                                            double dt)
+{
+    const BlockItem bi = m.nblk[blockIdx.x];
+    const int n = bi.start + static_cast<int>(threadIdx.x);
+    if (n >= bi.end) return;
+    double f0, f1, f2;
+    gather_node_force(m, s.F, n, f0, f1, f2);
+    const double mass = s.nmass[n];
+    s.u1[n] = s.u0[n] + alpha * dt * f0 / mass;
+    s.v1[n] = s.v0[n] + alpha * dt * f1 / mass;
+    s.w1[n] = s.w0[n] + alpha * dt * f2 / mass;
+}
+
+// K/boundary.cpp:16-55, all sets merged into one mask per boundary node
+__global__ __launch_bounds__(kBlock) void k_boundary_velocity(const int32_t* nodes,
+                                                              const uint8_t* masks, int32_t count,
+                                                              DState s)
+{
+    const int i = blockIdx.x * kBlock + static_cast<int>(threadIdx.x);
+    if (i >= count) return;
+    const int n = nodes[i];
+    const uint8_t mk = masks[i];
+    if (mk & 1) s.u1[n] = 0.0;
+    if (mk & 2) s.v1[n] = 0.0;
+    if (mk & 4) s.w1[n] = 0.0;
+}
+
+// element power F . (v1+v0)/2 summed corner by corner (K/energy_sgh.cpp:18-46)
+__device__ __forceinline__ double elem_power(const DState& s, int e, const int32_t nd[8])
+{
+    const double2* Fp = reinterpret_cast<const double2*>(s.F + 24 * static_cast<size_t>(e));
+    double f[24];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        const double2 t = Fp[i];
+        f[2 * i] = t.x;
+        f[2 * i + 1] = t.y;
+    }
+    double power = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int n = nd[a];
+        power += f[3 * a + 0] * ((s.u1[n] + s.u0[n]) * 0.5);
+        power += f[3 * a + 1] * ((s.v1[n] + s.v0[n]) * 0.5);
+        power += f[3 * a + 2] * ((s.w1[n] + s.w0[n]) * 0.5);
+    }
+    return power;
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_energy(DMesh m, DState s, double alpha,
+                                                          double dt)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    int32_t nd[8];
+    load_conn(m.conn, e, nd);
+    const double power = elem_power(s, e, nd);
+    s.sie1[e] = s.sie0[e] - alpha * dt / s.emass[e] * power;
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_position(DMesh m, DState s, double alpha,
+                                                            double dt)
+{
+    const BlockItem bi = m.nblk[blockIdx.x];
+    const int n = bi.start + static_cast<int>(threadIdx.x);
+    if (n >= bi.end) return;
+    s.x1[n] = s.x0[n] + alpha * dt * ((s.u1[n] + s.u0[n]) * 0.5);
+    s.y1[n] = s.y0[n] + alpha * dt * ((s.v1[n] + s.v0[n]) * 0.5);
+    s.z1[n] = s.z0[n] + alpha * dt * ((s.w1[n] + s.w0[n]) * 0.5);
+}
+
+// K/properties.cpp:30-109 with the ideal gas
+__global__ __launch_bounds__(kBlock) void k_update_state(DMesh m, DState s)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    const double den = s.emass[e] / s.vol[e];
+    s.den[e] = den;
+    double p, c;
+    ideal_gas(s.gamma[e], s.csmin[e], den, s.sie1[e], p, c);
+    s.pres[e] = p;
+    s.sspd[e] = c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rk_init_nodes(DMesh m, DState s)
+{
+    const BlockItem bi = m.nblk[blockIdx.x];
+    const int n = bi.start + static_cast<int>(threadIdx.x);
+    if (n >= bi.end) return;
+    s.x0[n] = s.x1[n];
+    s.y0[n] = s.y1[n];
+    s.z0[n] = s.z1[n];
+    s.u0[n] = s.u1[n];
+    s.v0[n] = s.v1[n];
+    s.w0[n] = s.w1[n];
+}
+
+__global__ __launch_bounds__(kBlock) void k_rk_init_elems(DMesh m, DState s)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    s.sie0[e] = s.sie1[e];
+}
+
+// per-element CFL dt, block minimum into partial[logical block]
+__global__ __launch_bounds__(kBlock) void k_dt_partial(DMesh m, DState s, const InstCtl* ctl,
+                                                       double* partial, double dt_cfl,
+                                                       double dt_max, double dt_min, double fuzz,
+                                                       int use_ctl)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    double cfl = dt_cfl, dmax = dt_max, dmin = dt_min, fz = fuzz;
+    if (use_ctl) {
+        const InstCtl& c = ctl[bi.inst];
+        cfl = c.dt_cfl;
+        dmax = c.dt_max;
+        dmin = c.dt_min;
+        fz = c.fuzz;
+    }
+    double d = DBL_MAX;
+    if (e < bi.end) {
+        int32_t nd[8];
+        load_conn(m.conn, e, nd);
+        double x[8], y[8], z[8];
+        gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+        d = elem_dt(min_pair_distance(x, y, z), s.sspd[e], cfl, dmax, dmin, fz);
+    }
+    d = block_min(d);
+    if (threadIdx.x == 0) partial[bi.lid] = d;
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_min(const double* partial, int32_t n, double* out)
+{
+    double v = DBL_MAX;
+    for (int i = threadIdx.x; i < n; i += 1024) v = fmin(v, partial[i]);
+    __shared__ double red[16];
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int i = 1; i < 16; ++i) r = fmin(r, red[i]);
+        *out = r;
+    }
+}
+
+// --------------------------------------------------------------------------
+// fused cycle kernels (dt, time and stop state stay on the device)
+// --------------------------------------------------------------------------
+
+// start of a cycle: get_timestep's host part (K/time_integration.cpp:69-150)
+__global__ __launch_bounds__(kBlock) void k_cycle_dt(InstCtl* ctl, const double* partial)
+{
+    InstCtl& c = ctl[blockIdx.x];
+    if (c.stopped) return;
+    double v = DBL_MAX;
+    for (int i = c.eblk_begin + static_cast<int>(threadIdx.x); i < c.eblk_end; i += kBlock)
+        v = fmin(v, partial[i]);
+    v = block_min(v);
+    if (threadIdx.x == 0) {
+        double dt = c.dt * 1.1;
+        if (v < dt) dt = v;
+        dt = fmin(dt, (c.graphics_time - c.time) + c.fuzz);
+        dt = fmin(dt, c.time_final - c.time);
+        c.dt = dt;
+    }
+}
+
+// velocity + boundary + position, node-centric (K/momentum.cpp:23-49,
+// K/boundary.cpp:16-55, K/geometry.cpp:17-24)
+__global__ __launch_bounds__(kBlock) void k_vel_bc_pos(DMesh m, DState s, const InstCtl* ctl,
+                                                       double alpha)
+{
+    const BlockItem bi = m.nblk[blockIdx.x];
+    const InstCtl& c = ctl[bi.inst];
+    if (c.stopped) return;
+    const int n = bi.start + static_cast<int>(threadIdx.x);
+    if (n >= bi.end) return;
+    const double dt = c.dt;
+    double f0, f1, f2;
+    gather_node_force(m, s.F, n, f0, f1, f2);
+    const double mass = s.nmass[n];
+    const double u0 = s.u0[n], v0 = s.v0[n], w0 = s.w0[n];
+    double u1 = u0 + alpha * dt * f0 / mass;
+    double v1 = v0 + alpha * dt * f1 / mass;
+    double w1 = w0 + alpha * dt * f2 / mass;
+    const uint8_t mk = s.bcmask[n];
+    if (mk & 1) u1 = 0.0;
+    if (mk & 2) v1 = 0.0;
+    if (mk & 4) w1 = 0.0;
+    s.u1[n] = u1;
+    s.v1[n] = v1;
+    s.w1[n] = w1;
+    s.x1[n] = s.x0[n] + alpha * dt * ((u1 + u0) * 0.5);
+    s.y1[n] = s.y0[n] + alpha * dt * ((v1 + v0) * 0.5);
+    s.z1[n] = s.z0[n] + alpha * dt * ((w1 + w0) * 0.5);
+}
+
+// energy + volume + EOS + next divergence (+ next cycle's dt candidates),
+// element-centric (K/energy_sgh.cpp:16-52, K/geometry.cpp:255-306,
+// K/properties.cpp:30-106, K/momentum.cpp:229-287, K/time_integration.cpp:74-140)
+__global__ __launch_bounds__(kBlock) void k_energy_state_div(DMesh m, DState s,
+                                                             const InstCtl* ctl, double alpha,
+                                                             double* dt_partial)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const InstCtl& c = ctl[bi.inst];
+    if (c.stopped) return;
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    double dtl = DBL_MAX;
+    if (e < bi.end) {
+        int32_t nd[8];
+        load_conn(m.conn, e, nd);
+        const double power = elem_power(s, e, nd);
+        const double emass = s.emass[e];
+        const double sie = s.sie0[e] - alpha * c.dt / emass * power;
+        s.sie1[e] = sie;
+
+        double x[8], y[8], z[8];
+        gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+        const double vol = hex_volume(x, y, z);
+        s.vol[e] = vol;
+        const double den = emass / vol;
+        s.den[e] = den;
+        double p, cs;
+        ideal_gas(s.gamma[e], s.csmin[e], den, sie, p, cs);
+        s.pres[e] = p;
+        s.sspd[e] = cs;
+
+        double B[24], u[8], v[8], w[8];
+        hex_bmatrix(x, y, z, B);
+        gather3(s.u1, s.v1, s.w1, nd, u, v, w);
+        s.div[e] = divergence(B, u, v, w, vol);
+        if (dt_partial != nullptr)
+            dtl = elem_dt(min_pair_distance(x, y, z), cs, c.dt_cfl, c.dt_max, c.dt_min, c.fuzz);
+    }
+    if (dt_partial != nullptr) {
+        dtl = block_min(dtl);
+        if (threadIdx.x == 0) dt_partial[bi.lid] = dtl;
+    }
+}
+
+// end of a cycle (K/sgh_solve.cpp:590-632)
+__global__ void k_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin)
+{
+    const int i = blockIdx.x * blockDim.x + static_cast<int>(threadIdx.x);
+    if (i >= ninst) return;
+    InstCtl& c = ctl[i];
+    if (c.stopped) return;
+    const long long cycle = c.cycle;
+    c.time += c.dt;
+    int write = 0;
+    if ((cycle + 1) % c.graphics_cyc_ival == 0 && cycle > 0) write = 1;
+    else if (c.time >= c.time_final) write = 1;
+    else if (c.time >= c.graphics_time) write = 1;
+    if (write) c.graphics_time = c.time + c.graphics_dt_ival;
+    c.cycle = cycle + 1;
+    if (c.time >= c.time_final || c.cycle >= c.cycle_stop) {
+        c.stopped = 1;
+        c.final_bin = newest_bin;
+    }
+}
+
+// per-instance energy tallies, fixed-order two-level sums (K/sgh_solve.cpp:77-101)
+__global__ __launch_bounds__(kBlock) void k_energy_partial(DState s, int32_t e0, int32_t e1,
+                                                           int32_t n0, int32_t n1, double* scratch,
+                                                           int32_t nscratch)
+{
+    double ie = 0.0, ke = 0.0;
+    const int stride = nscratch * kBlock;
+    for (int e = e0 + blockIdx.x * kBlock + threadIdx.x; e < e1; e += stride)
+        ie += s.emass[e] * s.sie1[e];
+    for (int n = n0 + blockIdx.x * kBlock + threadIdx.x; n < n1; n += stride) {
+        double k = 0;
+        k += s.u1[n] * s.u1[n];
+        k += s.v1[n] * s.v1[n];
+        k += s.w1[n] * s.w1[n];
+        ke += s.nmass[n] * k;
+    }
+    ie = block_sum(ie);
+    ke = block_sum(ke);
+    if (threadIdx.x == 0) {
+        scratch[2 * blockIdx.x] = ie;
+        scratch[2 * blockIdx.x + 1] = ke;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy_final(const double* scratch, int32_t nscratch,
+                                                         double* out2)
+{
+    double ie = 0.0, ke = 0.0;
+    for (int i = threadIdx.x; i < nscratch; i += kBlock) {
+        ie += scratch[2 * i];
+        ke += scratch[2 * i + 1];
+    }
+    ie = block_sum(ie);
+    ke = block_sum(ke);
+    if (threadIdx.x == 0) {
+        out2[0] = ie;
+        out2[1] = 0.5 * ke;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_swap_bins(DState s, int32_t n0, int32_t n1, int32_t e0,
+                                                      int32_t e1)
+{
+    const int i = blockIdx.x * kBlock + static_cast<int>(threadIdx.x);
+    const int n = n0 + i;
+    if (n < n1) {
+        double t;
+        t = s.x0[n]; s.x0[n] = s.x1[n]; s.x1[n] = t;
+        t = s.y0[n]; s.y0[n] = s.y1[n]; s.y1[n] = t;
+        t = s.z0[n]; s.z0[n] = s.z1[n]; s.z1[n] = t;
+        t = s.u0[n]; s.u0[n] = s.u1[n]; s.u1[n] = t;
+        t = s.v0[n]; s.v0[n] = s.v1[n]; s.v1[n] = t;
+        t = s.w0[n]; s.w0[n] = s.w1[n]; s.w1[n] = t;
+    }
+    const int e = e0 + i;
+    if (e < e1) {
+        const double t = s.sie0[e];
+        s.sie0[e] = s.sie1[e];
+        s.sie1[e] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_check(DMesh m, DState s, int32_t* flag)
+{
+    const int i = blockIdx.x * kBlock + static_cast<int>(threadIdx.x);
+    int bad = 0;
+    if (i < m.E) {
+        const double vals[6] = {s.sie1[i], s.den[i], s.pres[i], s.sspd[i], s.vol[i], s.emass[i]};
+        for (double v : vals) bad |= !isfinite(v);
+        bad |= !(s.vol[i] > 0.0) || !(s.emass[i] > 0.0);
+    }
+    if (i < m.N) {
+        const double vals[7] = {s.x1[i], s.y1[i], s.z1[i], s.u1[i], s.v1[i], s.w1[i], s.nmass[i]};
+        for (double v : vals) bad |= !isfinite(v);
+        bad |= !(s.nmass[i] > 0.0);
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
+inline unsigned grid_for(int32_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers --
+void launch_get_vol(const DMesh& m, const DState& s, hipStream_t st)
+{
+    if (m.n_eblk) hipLaunchKernelGGL(k_get_vol, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s);
+}
+void launch_get_divergence(const DMesh& m, const DState& s, hipStream_t st)
+{
+    if (m.n_eblk) hipLaunchKernelGGL(k_get_divergence, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s);
+}
+void launch_force(const DMesh& m, const DState& s, const InstCtl* ctl, double fuzz, double small,
+                  hipStream_t st)
+{
+    if (m.n_eblk)
+        hipLaunchKernelGGL(k_force, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, ctl, fuzz, small);
+}
+void launch_update_velocity(const DMesh& m, const DState& s, double alpha, double dt,
+                            hipStream_t st)
+{
+    if (m.n_nblk)
+        hipLaunchKernelGGL(k_update_velocity, dim3(m.n_nblk), dim3(kBlock), 0, st, m, s, alpha, dt);
+}
+void launch_boundary_velocity(const int32_t* nodes, const uint8_t* masks, int32_t count,
+                              const DState& s, hipStream_t st)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_boundary_velocity, dim3(grid_for(count)), dim3(kBlock), 0, st, nodes,
+                           masks, count, s);
+}
+void launch_update_energy(const DMesh& m, const DState& s, double alpha, double dt, hipStream_t st)
+{
+    if (m.n_eblk)
+        hipLaunchKernelGGL(k_update_energy, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, alpha, dt);
+}
+void launch_update_position(const DMesh& m, const DState& s, double alpha, double dt,
+                            hipStream_t st)
+{
+    if (m.n_nblk)
+        hipLaunchKernelGGL(k_update_position, dim3(m.n_nblk), dim3(kBlock), 0, st, m, s, alpha, dt);
+}
+void launch_update_state(const DMesh& m, const DState& s, hipStream_t st)
+{
+    if (m.n_eblk) hipLaunchKernelGGL(k_update_state, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s);
+}
+void launch_rk_init(const DMesh& m, const DState& s, hipStream_t st)
+{
+    if (m.n_eblk) hipLaunchKernelGGL(k_rk_init_elems, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s);
+    if (m.n_nblk) hipLaunchKernelGGL(k_rk_init_nodes, dim3(m.n_nblk), dim3(kBlock), 0, st, m, s);
+}
+void launch_dt_partial(const DMesh& m, const DState& s, const InstCtl* ctl, double* partial,
+                       double dt_cfl, double dt_max, double dt_min, double fuzz, int use_ctl,
+                       hipStream_t st)
+{
+    if (m.n_eblk)
+        hipLaunchKernelGGL(k_dt_partial, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, ctl, partial,
+                           dt_cfl, dt_max, dt_min, fuzz, use_ctl);
+}
+void launch_reduce_min(const double* partial, int32_t n, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_min, dim3(1), dim3(1024), 0, st, partial, n, out);
+}
+void launch_cycle_dt(InstCtl* ctl, int32_t ninst, const double* partial, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cycle_dt, dim3(ninst), dim3(kBlock), 0, st, ctl, partial);
+}
+void launch_vel_bc_pos(const DMesh& m, const DState& s, const InstCtl* ctl, double alpha,
+                       hipStream_t st)
+{
+    if (m.n_nblk)
+        hipLaunchKernelGGL(k_vel_bc_pos, dim3(m.n_nblk), dim3(kBlock), 0, st, m, s, ctl, alpha);
+}
+void launch_energy_state_div(const DMesh& m, const DState& s, const InstCtl* ctl, double alpha,
+                             double* dt_partial, hipStream_t st)
+{
+    if (m.n_eblk)
+        hipLaunchKernelGGL(k_energy_state_div, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, ctl,
+                           alpha, dt_partial);
+}
+void launch_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_time_advance, dim3(grid_for(ninst)), dim3(kBlock), 0, st, ctl, ninst,
+                       newest_bin);
+}
+void launch_energy_sums(const DMesh& m, const DState& s, const int32_t* eoff, const int32_t* noff,
+                        int32_t inst, double* out2, double* scratch, int32_t nscratch,
+                        hipStream_t st)
+{
+    (void)m;
+    hipLaunchKernelGGL(k_energy_partial, dim3(nscratch), dim3(kBlock), 0, st, s, eoff[inst],
+                       eoff[inst + 1], noff[inst], noff[inst + 1], scratch, nscratch);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, scratch, nscratch, out2);
+}
+void launch_swap_bins_range(const DState& s, int32_t n0, int32_t n1, int32_t e0, int32_t e1,
+                            hipStream_t st)
+{
+    const int32_t cnt = (n1 - n0) > (e1 - e0) ? (n1 - n0) : (e1 - e0);
+    if (cnt > 0)
+        hipLaunchKernelGGL(k_swap_bins, dim3(grid_for(cnt)), dim3(kBlock), 0, st, s, n0, n1, e0, e1);
+}
+void launch_check(const DMesh& m, const DState& s, int32_t* flag, hipStream_t st)
+{
+    const int32_t cnt = m.N > m.E ? m.N : m.E;
+    if (cnt > 0) hipLaunchKernelGGL(k_check, dim3(grid_for(cnt)), dim3(kBlock), 0, st, m, s, flag);
+}
+
+}  // namespace fsgh

@@ -1,0 +1,40 @@
+// fsgh_mesh.h -- host-side mesh preparation for the SGH library.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "fsgh_internal.h"
+
+namespace fsgh {
+
+// Ragged instance layout: instance i owns [off[i], off[i+1]).
+struct Layout {
+    int32_t ninst = 1;
+    std::vector<int64_t> node_off, elem_off;
+};
+
+// Morton renumbering inside each instance.  new_of_old_elem / new_of_old_node
+// map caller ids to internal ids; old_of_new_* are their inverses.
+void morton_renumber(const Layout& L, const int64_t* nodes_in_elem, const double* coords,
+                     std::vector<int32_t>& new_of_old_elem, std::vector<int32_t>& new_of_old_node);
+
+// corners_in_node of the caller's mesh, each node's list in ascending element
+// order (K/mesh.h:218-241), as CSR over caller node ids.
+void corner_csr(int64_t N, int64_t E, const int64_t* nodes_in_elem, std::vector<int64_t>& start,
+                std::vector<int64_t>& corners);
+
+// Work items of kBlock entries per instance, dealt to XCDs so that logical
+// block L runs on XCD floor(L / ceil(nb/8))-ish: physical block b -> XCD b % 8.
+void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
+                 std::vector<int32_t>& inst_begin);
+
+// Boundary node sets of one instance (K/setup.cpp:431-665): nodes of the
+// boundary faces whose 4 nodes all satisfy the surface test.  Each set is
+// returned sorted by caller node id.
+int boundary_sets(const Layout& L, int32_t inst, const int64_t* nodes_in_elem,
+                  const double* coords, const std::vector<int64_t>& cstart,
+                  const std::vector<int64_t>& corners, int32_t nsets, const fsgh_boundary* bc,
+                  std::vector<std::vector<int64_t>>& out);
+
+}  // namespace fsgh

@@ -1,0 +1,202 @@
+"""Problem definitions: the compiled-in ``input()`` of the reference as data.
+
+The reference selects its problem by editing K/input.cpp:50-651 (materials,
+fills, boundary sets, run controls) and passing a mesh file (K/driver.cpp:138).
+Here a problem is a ``ProblemSpec``: plain data that both the HIP solver
+(``fierro_amd.sgh``) and the CPU oracle consume, so both run the same inputs.
+
+Mesh generator: ``box_mesh`` numbers nodes i-fastest, then j, then k, and
+elements with nodes (i,j,k),(i+1,j,k),(i+1,j+1,k),(i,j+1,k) followed by the
+k+1 layer -- the numbering of the reference's EnSight meshes (e.g.
+Explicit-Lagrange/meshes/mesh_2x2x2.geo:91-100).  Coordinates are the
+correctly rounded k*L/n, computed as (k*Lnum)/(n*Lden) in FP64, which is what
+reading a mesh written with exact decimal coordinates gives.
+"""
+from __future__ import annotations
+
+import dataclasses
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# enums of K/mesh.h:820-918
+GLOBAL, BOX, CYLINDER, SPHERE = 0, 1, 2, 3
+CARTESIAN, RADIAL, SPHERICAL = 0, 1, 2
+X_PLANE, Y_PLANE, Z_PLANE = 0, 1, 2
+FIXED, REFLECTED = 0, 1
+
+# K/driver.cpp:35-62 defaults
+DEFAULT_RUN = dict(time_final=1.0e16, dt_start=1.0e-8, dt_max=1.0e-2, dt_min=1.0e-8,
+                   dt_cfl=0.4, graphics_dt_ival=1.0e8, graphics_cyc_ival=50,
+                   cycle_stop=1000000000, fuzz=1.0e-16, tiny=1.0e-12, small=1.0e-8)
+
+
+def fill(volume=GLOBAL, mat_id=0, den=1.0, sie=0.0, velocity=CARTESIAN, u=0.0, v=0.0, w=0.0,
+         speed=0.0, x1=0.0, x2=0.0, y1=0.0, y2=0.0, z1=0.0, z2=0.0, radius1=0.0, radius2=0.0):
+    """One mat_fill_t (K/mesh.h:858-890)."""
+    return dict(volume=volume, mat_id=mat_id, x1=x1, x2=x2, y1=y1, y2=y2, z1=z1, z2=z2,
+                radius1=radius1, radius2=radius2, velocity=velocity, u=u, v=v, w=w,
+                speed=speed, sie=sie, den=den)
+
+
+@dataclass
+class ProblemSpec:
+    name: str
+    mesh: dict                      # {"kind": "box", nx, ny, nz, lx, ly, lz, lden} or {"kind": "geo", "path"}
+    materials: list                 # [(q1, q1ex, q2, q2ex)] per material
+    state_vars: np.ndarray          # num_materials x num_state_vars (gamma, csmin, ...)
+    mat_num_state_vars: list
+    fills: list
+    bcs: list                       # [(surface, value, hydro_bc)]
+    run: dict = field(default_factory=lambda: dict(DEFAULT_RUN))
+    num_state_vars: int = 6
+
+    def mesh_arrays(self, oracle_gen=None, oracle_read=None):
+        """(coords N x 3, nodes_in_elem E x 8).  The oracle passes its own
+        generator/reader so that its inputs come from its own code."""
+        m = self.mesh
+        if m["kind"] == "box":
+            gen = oracle_gen or box_mesh
+            return gen(m["nx"], m["ny"], m["nz"], m["lx"], m["ly"], m["lz"], m["lden"])
+        if m["kind"] == "geo":
+            rd = oracle_read or read_ensight
+            return rd(m["path"])
+        raise ValueError(m["kind"])
+
+    def with_run(self, **kw):
+        r = dict(self.run)
+        r.update(kw)
+        return dataclasses.replace(self, run=r)
+
+    @property
+    def num_elems(self):
+        m = self.mesh
+        return m["nx"] * m["ny"] * m["nz"] if m["kind"] == "box" else None
+
+
+def box_mesh(nx, ny, nz, lx_num=1.0, ly_num=1.0, lz_num=1.0, l_den=1.0):
+    """Structured hex box on [0,Lx]x[0,Ly]x[0,Lz], L = L_num/L_den."""
+    nx1, ny1, nz1 = nx + 1, ny + 1, nz + 1
+    k, j, i = np.meshgrid(np.arange(nz1), np.arange(ny1), np.arange(nx1), indexing="ij")
+    coords = np.empty((nx1 * ny1 * nz1, 3))
+    coords[:, 0] = (i.ravel().astype(np.float64) * lx_num) / (float(nx) * l_den)
+    coords[:, 1] = (j.ravel().astype(np.float64) * ly_num) / (float(ny) * l_den)
+    coords[:, 2] = (k.ravel().astype(np.float64) * lz_num) / (float(nz) * l_den)
+    ke, je, ie = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    n0 = (ie + nx1 * (je + ny1 * ke)).ravel().astype(np.int64)
+    conn = np.empty((nx * ny * nz, 8), np.int64)
+    conn[:, 0] = n0
+    conn[:, 1] = n0 + 1
+    conn[:, 2] = n0 + 1 + nx1
+    conn[:, 3] = n0 + nx1
+    conn[:, 4:] = conn[:, :4] + nx1 * ny1
+    return coords, conn
+
+
+def read_ensight(path):
+    """EnSight Gold ASCII hexa8 reader following K/read_mesh.cpp:12-147."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    pos = 8
+    toks = " ".join(lines[pos:]).split()
+    n = int(toks[0])
+    xyz = np.array(toks[1:1 + 3 * n], dtype=np.float64).reshape(3, n).T.copy()
+    rest = toks[1 + 3 * n:]
+    assert rest[0] == "hexa8", rest[0]
+    ne = int(rest[1])
+    conn = np.array(rest[2:2 + 8 * ne], dtype=np.int64).reshape(ne, 8) - 1
+    return xyz, conn
+
+
+def _gas(gamma, csmin=1.0e-14, nmat=1):
+    sv = np.zeros((nmat, 6))
+    sv[:, 0] = gamma
+    sv[:, 1] = csmin
+    sv[:, 2] = 1.0
+    return sv
+
+
+def sedov(n=32, q2=1.333, dt_cfl=0.5, cycles=200, gamma=5.0 / 3.0):
+    """Point-energy blast on [0,1.2]^3 with n^3 hexes: rho=1, u=0, e=1e-10
+    except the origin element, e=0.25/V (IE0 = 0.25); symmetry (reflected)
+    planes x=y=z=0; q1=1, q2 as given, q1ex=1, q2ex=0 (K/input.cpp:112-165);
+    RK2, CFL dt_cfl, dt_start 1e-5, fixed number of cycles."""
+    h = 1.2 / n
+    vol0 = h * h * h
+    spec = ProblemSpec(
+        name=f"sedov{n}",
+        mesh=dict(kind="box", nx=n, ny=n, nz=n, lx=12.0, ly=12.0, lz=12.0, lden=10.0),
+        materials=[(1.0, 1.0, q2, 0.0)],
+        state_vars=_gas(gamma),
+        mat_num_state_vars=[3],
+        fills=[fill(GLOBAL, den=1.0, sie=1.0e-10),
+               fill(SPHERE, den=1.0, sie=0.25 / vol0, radius1=0.0, radius2=h)],
+        bcs=[(X_PLANE, 0.0, REFLECTED), (Y_PLANE, 0.0, REFLECTED), (Z_PLANE, 0.0, REFLECTED)],
+    )
+    spec.run.update(time_final=1.0e16, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
+                    dt_cfl=dt_cfl, graphics_dt_ival=1.0e16, graphics_cyc_ival=1000000000,
+                    cycle_stop=cycles)
+    return spec
+
+
+def reference_sedov(mesh_path):
+    """The reference's default compiled-in problem (K/input.cpp:69-202 with
+    K/driver.cpp defaults) on a given .geo mesh: 6 reflected planes of the
+    [0,1.2]^3 box, sphere fill of radius 1.2/8, q2 = 1.3333, CFL 0.4."""
+    r2 = 1.2 / 8.0
+    sie = (963.652344 * (1.2 / 30.0) ** 3) / (r2 ** 3)
+    spec = ProblemSpec(
+        name="reference_sedov3d",
+        mesh=dict(kind="geo", path=mesh_path),
+        materials=[(1.0, 1.0, 1.3333, 0.0)],
+        state_vars=_gas(5.0 / 3.0),
+        mat_num_state_vars=[3],
+        fills=[fill(GLOBAL, den=1.0, sie=1.0e-10),
+               fill(SPHERE, den=1.0, sie=sie, radius1=0.0, radius2=r2)],
+        bcs=[(X_PLANE, 0.0, REFLECTED), (Y_PLANE, 0.0, REFLECTED), (Z_PLANE, 0.0, REFLECTED),
+             (X_PLANE, 1.2, REFLECTED), (Y_PLANE, 1.2, REFLECTED), (Z_PLANE, 1.2, REFLECTED)],
+    )
+    spec.run.update(time_final=1.0, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
+                    graphics_dt_ival=0.25, graphics_cyc_ival=1000000, cycle_stop=100000)
+    return spec
+
+
+def sod_tube(nx, rho_r, p_r, gamma, cycles=500, ny=4, nz=4, dt_cfl=0.5):
+    """One Sod-type shock tube: nx x ny x nz cubic hexes, h = 1/nx, on
+    [0,1]x[0,ny h]x[0,nz h]; left rho=1, p=1, right (rho_r, p_r); interface at
+    x = 0.5 by element centroid; reflected walls on all six faces."""
+    e_l = 1.0 / ((gamma - 1.0) * 1.0)
+    e_r = p_r / ((gamma - 1.0) * rho_r)
+    spec = ProblemSpec(
+        name=f"sod{nx}",
+        mesh=dict(kind="box", nx=nx, ny=ny, nz=nz, lx=float(nx), ly=float(ny), lz=float(nz),
+                  lden=float(nx)),
+        materials=[(1.0, 1.0, 1.333, 0.0)],
+        state_vars=_gas(gamma),
+        mat_num_state_vars=[3],
+        fills=[fill(GLOBAL, den=1.0, sie=e_l),
+               fill(BOX, den=rho_r, sie=e_r, x1=0.5, x2=2.0, y1=-1.0, y2=1.0e9, z1=-1.0,
+                    z2=1.0e9)],
+        bcs=[(X_PLANE, 0.0, REFLECTED), (X_PLANE, 1.0, REFLECTED),
+             (Y_PLANE, 0.0, REFLECTED), (Y_PLANE, ny / nx, REFLECTED),
+             (Z_PLANE, 0.0, REFLECTED), (Z_PLANE, nz / nx, REFLECTED)],
+    )
+    spec.run.update(time_final=1.0e16, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
+                    dt_cfl=dt_cfl, graphics_dt_ival=1.0e16, graphics_cyc_ival=1000000000,
+                    cycle_stop=cycles)
+    return spec
+
+
+def sod_ensemble(count=512, seed=0, cycles=500):
+    """The batched ensemble: `count` tubes with nx ~ U{64..1024},
+    rho_R, p_R ~ U(0.05, 0.5), gamma ~ U(1.3, 1.7), drawn per instance in that
+    order from numpy's PCG64(seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        nx = int(rng.integers(64, 1025))
+        rho_r = float(rng.uniform(0.05, 0.5))
+        p_r = float(rng.uniform(0.05, 0.5))
+        gamma = float(rng.uniform(1.3, 1.7))
+        out.append(sod_tube(nx, rho_r, p_r, gamma, cycles=cycles))
+    return out

@@ -1,0 +1,224 @@
+/*
+ * fsgh.h -- C ABI of the MI355X-native SGH (staggered-grid hydrodynamics) cycle.
+ *
+ * The library replaces the per-stage operators of Fierro's 3D explicit
+ * Lagrangian solver (reference: Explicit-Lagrange-Kokkos/SGH_solver/, cited
+ * below as K/) with hand-written HIP kernels for gfx950, behind plain C entry
+ * points: no C++ or torch types cross this boundary.
+ *
+ * Model of use (mirrors K/driver.cpp:94-313):
+ *   fsgh_create            mesh + coordinates in   (K/read_mesh.cpp, K/mesh.h:180-245)
+ *   fsgh_set_materials     q1,q1ex,q2,q2ex         (K/state.h:130-178)
+ *   fsgh_set_bcs           boundary sets           (K/setup.cpp:431-665)
+ *   fsgh_setup_fills       initial conditions      (K/setup.cpp:106-424)
+ *     or fsgh_upload of every state field
+ *   per-stage operators    exactly the reference's sequence, or
+ *   fsgh_run               the fused, graph-captured cycle loop (K/sgh_solve.cpp:125-634)
+ *   fsgh_download / fsgh_energy
+ *
+ * Conventions
+ *   - Every call returns FSGH_OK (0) or a negative FSGH_E_* code;
+ *     fsgh_last_error() gives the message.  No exception crosses the ABI.
+ *   - Field layouts at the ABI are the reference's (row-major MATAR):
+ *     coordinates/velocity [2 bins][N][3], sie [2 bins][E], corner force
+ *     [8E][3] with corner = 8e+i.  Indices are int64 (reference: size_t).
+ *     Internally the library keeps SoA FP64 fields and int32 connectivity,
+ *     optionally renumbered along a Morton curve; uploads and downloads
+ *     convert, so callers always see their own numbering.
+ *   - Operator calls are asynchronous on the context's HIP stream; download,
+ *     energy and any call that returns a host value synchronise.
+ *   - A context must not be used by two host threads at once.
+ *   - A batched context holds many independent instances packed as one ragged
+ *     mesh (instance i owns nodes [node_off[i], node_off[i+1]) and elements
+ *     [elem_off[i], elem_off[i+1])); every instance has its own dt, time,
+ *     boundary sets and stop state.  A plain mesh is a batch of one.
+ */
+#ifndef FSGH_H
+#define FSGH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSGH_OK 0
+#define FSGH_E_INVALID (-1)     /* bad argument, size mismatch, NULL handle */
+#define FSGH_E_HIP (-2)         /* HIP runtime error */
+#define FSGH_E_NOMEM (-3)       /* device or host allocation failed */
+#define FSGH_E_NONFINITE (-4)   /* non-finite or non-physical state found by fsgh_check */
+#define FSGH_E_STATE (-5)       /* call out of order (e.g. run before setup) */
+#define FSGH_E_UNSUPPORTED (-6) /* feature outside this build (strength models, 2D-RZ) */
+#define FSGH_E_NOTBUILT (-7)    /* library built without GPU code (never in a product build) */
+
+typedef struct fsgh_ctx fsgh_ctx;
+
+/* ---------------------------------------------------------------- mesh --- */
+typedef struct fsgh_mesh_desc {
+    int64_t num_nodes;                 /* N, total over instances */
+    int64_t num_elems;                 /* E, total over instances */
+    int32_t num_instances;             /* 1 for a plain mesh */
+    int32_t renumber;                  /* 0: keep caller numbering, 1: Morton order */
+    const int64_t *instance_node_off;  /* num_instances+1 entries, or NULL if 1 instance */
+    const int64_t *instance_elem_off;  /* num_instances+1 entries, or NULL if 1 instance */
+} fsgh_mesh_desc;
+
+/* Creates a context on HIP device `device`: builds the corner connectivity
+ * (corners_in_node in ascending element order, K/mesh.h:180-245), copies the
+ * coordinates into both RK bins (K/read_mesh.cpp:129-137), allocates device
+ * state.  nodes_in_elem: E x 8, 0-based, node ids global over the batch.
+ * Replaces: read_mesh_ensight + mesh.build_corner_connectivity
+ * (K/driver.cpp:138-139). */
+int fsgh_create(const fsgh_mesh_desc *desc, const int64_t *nodes_in_elem,
+                const double *coords, int device, fsgh_ctx **out);
+/* device < 0 creates a host-only context: connectivity, renumbering and
+ * boundary sets are built (and can be read back) but nothing touches a GPU;
+ * operators and field transfers return FSGH_E_STATE. */
+int fsgh_destroy(fsgh_ctx *ctx);
+const char *fsgh_last_error(const fsgh_ctx *ctx);
+/* 1 when the library holds gfx950 device code and a device is usable. */
+int fsgh_device_count(void);
+
+/* corners_in_node as CSR in the caller's numbering (start: N+1, data: 8E).
+ * Replaces the mesh_t ragged array read by K/momentum.cpp:31-34. */
+int fsgh_get_corners_in_node(fsgh_ctx *ctx, int64_t *start, int64_t *corners);
+/* Morton permutations: new_of_old_elem[E], new_of_old_node[N] (identity when
+ * renumber == 0). */
+int fsgh_get_renumbering(fsgh_ctx *ctx, int64_t *new_of_old_elem, int64_t *new_of_old_node);
+
+/* ------------------------------------------------------------ materials --- */
+typedef struct fsgh_material {
+    double q1, q1ex, q2, q2ex;         /* K/state.h:174-177 */
+} fsgh_material;
+/* EOS is the ideal gas (K/eos.cpp:12-46) with gamma = statev[e][0] and
+ * min sound speed = statev[e][1]. */
+int fsgh_set_materials(fsgh_ctx *ctx, int32_t num_materials, const fsgh_material *mats);
+
+/* ----------------------------------------------------- boundary sets --- */
+typedef struct fsgh_boundary {
+    int32_t surface;                   /* 0 x-plane, 1 y-plane, 2 z-plane, 3 cylinder, 4 sphere */
+    int32_t hydro_bc;                  /* 0 fixed, 1 reflected; other values: no-op */
+    double value;                      /* plane position or radius */
+} fsgh_boundary;
+/* Tags the boundary patches of `instance` whose 4 nodes lie within 1e-7 of
+ * the surface (K/setup.cpp:487-571) and stores each set's nodes.  Replaces
+ * mesh.init_bdy_sets + tag_bdys + build_boundry_node_sets. */
+int fsgh_set_bcs(fsgh_ctx *ctx, int32_t instance, int32_t num_sets,
+                 const fsgh_boundary *sets);
+/* number of nodes in boundary set `set` of `instance`, and the nodes (caller
+ * ids, ascending).  The reference stores the same set in first-seen order of
+ * its patch numbering (K/setup.cpp:575-665); the order has no effect on the
+ * velocity condition, which only zeroes components. */
+int fsgh_bc_set_size(fsgh_ctx *ctx, int32_t instance, int32_t set, int64_t *count);
+int fsgh_get_bc_set(fsgh_ctx *ctx, int32_t instance, int32_t set, int64_t *nodes);
+
+/* ------------------------------------------------- initial conditions --- */
+typedef struct fsgh_fill {
+    int32_t volume;                    /* 0 global, 1 box, 2 cylinder, 3 sphere */
+    int32_t mat_id;
+    double x1, x2, y1, y2, z1, z2;     /* box */
+    double radius1, radius2;           /* cylinder / sphere */
+    int32_t velocity;                  /* 0 cartesian, 1 radial, 2 spherical */
+    int32_t pad_;
+    double u, v, w, speed;
+    double sie, den;
+} fsgh_fill;
+/* Paints fills in order over `instance` by element centroid, sets M = rho V,
+ * e, statev, EOS, node velocity, applies the boundary velocity and the nodal
+ * mass sum_{elems of n} M_e/8 (K/setup.cpp:106-424).  state_vars is
+ * num_materials x num_state_vars; mat_num_state_vars[m] of them are copied. */
+int fsgh_setup_fills(fsgh_ctx *ctx, int32_t instance, int32_t num_fills,
+                     const fsgh_fill *fills, int32_t num_materials,
+                     int32_t num_state_vars, const double *state_vars,
+                     const int32_t *mat_num_state_vars);
+
+/* --------------------------------------------------------------- fields --- */
+typedef enum fsgh_field {
+    FSGH_NODE_COORDS = 0,  /* double [2][N][3] */
+    FSGH_NODE_VEL = 1,     /* double [2][N][3] */
+    FSGH_NODE_MASS = 2,    /* double [N] */
+    FSGH_ELEM_DEN = 3,     /* double [E] */
+    FSGH_ELEM_PRES = 4,    /* double [E] */
+    FSGH_ELEM_SSPD = 5,    /* double [E] */
+    FSGH_ELEM_SIE = 6,     /* double [2][E] */
+    FSGH_ELEM_VOL = 7,     /* double [E] */
+    FSGH_ELEM_DIV = 8,     /* double [E] */
+    FSGH_ELEM_MASS = 9,    /* double [E] */
+    FSGH_ELEM_MAT_ID = 10, /* int64  [E] */
+    FSGH_ELEM_GAMMA = 11,  /* double [E]  = statev(e,0) */
+    FSGH_ELEM_CSMIN = 12,  /* double [E]  = statev(e,1) */
+    FSGH_CORNER_FORCE = 13 /* double [8E][3] */
+} fsgh_field;
+int fsgh_upload(fsgh_ctx *ctx, int32_t field, const void *host, size_t bytes);
+int fsgh_download(fsgh_ctx *ctx, int32_t field, void *host, size_t bytes);
+
+/* ------------------------------------- per-stage operators (reference API) --- */
+/* Each applies to every instance of the context, with the reference's
+ * semantics and FP64 operation order.  dt and rk_alpha are scalars as in the
+ * reference (K/mesh.h:1211-1373). */
+int fsgh_rk_init(fsgh_ctx *ctx);                                    /* K/time_integration.cpp:9-44 */
+int fsgh_get_timestep(fsgh_ctx *ctx, double time_value, double graphics_time,
+                      double time_final, double dt_max, double dt_min, double dt_cfl,
+                      double fuzz, double *dt_inout);               /* K/time_integration.cpp:54-154 */
+int fsgh_get_divergence(fsgh_ctx *ctx);                             /* K/momentum.cpp:221-292 */
+int fsgh_get_force_sgh(fsgh_ctx *ctx, double fuzz, double small, double dt,
+                       double rk_alpha);                            /* K/force_sgh.cpp:12-399 */
+int fsgh_update_velocity_sgh(fsgh_ctx *ctx, double rk_alpha, double dt); /* K/momentum.cpp:11-53 */
+int fsgh_boundary_velocity(fsgh_ctx *ctx);                          /* K/boundary.cpp:10-55 */
+int fsgh_update_energy_sgh(fsgh_ctx *ctx, double rk_alpha, double dt);   /* K/energy_sgh.cpp:6-55 */
+int fsgh_update_position_sgh(fsgh_ctx *ctx, double rk_alpha, double dt); /* K/geometry.cpp:9-26 */
+int fsgh_get_vol(fsgh_ctx *ctx);                                    /* K/geometry.cpp:238-310 */
+int fsgh_update_state(fsgh_ctx *ctx, double dt, double rk_alpha);   /* K/properties.cpp:9-114 */
+/* extensive energies IE = sum M e, KE = 1/2 sum m |v|^2 of one instance
+ * (K/sgh_solve.cpp:66-104); synchronises. */
+int fsgh_energy(fsgh_ctx *ctx, int32_t instance, double *ie, double *ke);
+
+/* -------------------------------------------------- fused cycle loop --- */
+typedef struct fsgh_run_params {
+    double time_final;                 /* K/input.cpp:73 */
+    double dt_start;                   /* K/input.cpp:76 */
+    double dt_max, dt_min, dt_cfl;     /* K/driver.cpp:47-49 */
+    double graphics_dt_ival;           /* K/driver.cpp:39 */
+    int64_t graphics_cyc_ival;         /* K/driver.cpp:36 */
+    int64_t cycle_stop;                /* K/driver.cpp:56 */
+    double fuzz, small;                /* K/driver.cpp:60-62 */
+} fsgh_run_params;
+
+typedef struct fsgh_instance_status {
+    int64_t cycles;                    /* cycles executed */
+    double time_value;
+    double dt;                         /* last dt */
+    double graphics_time;
+    int32_t stopped;                   /* time >= time_final or cycle_stop reached */
+    int32_t pad_;
+} fsgh_instance_status;
+
+/* Starts the cycle loop of every instance at time 0 with dt = dt_start
+ * (K/driver.cpp:269-274); params has one entry per instance. */
+int fsgh_run_begin(fsgh_ctx *ctx, const fsgh_run_params *params);
+/* Advances up to `ncycles` cycles of the loop of K/sgh_solve.cpp:125-634
+ * with fused kernels: dt (device-resident, x1.1 growth, clamps, per instance),
+ * then per RK stage force -> velocity+BC+position -> energy+volume+EOS+div.
+ * use_graph != 0 replays one captured hipGraph per cycle.  Returns when all
+ * instances stopped or ncycles were issued; synchronises. */
+int fsgh_run(fsgh_ctx *ctx, int64_t ncycles, int32_t use_graph);
+int fsgh_run_status(fsgh_ctx *ctx, int32_t instance, fsgh_instance_status *out);
+
+/* ------------------------------------------------------- diagnostics --- */
+/* Per-kernel timing with HIP events on the context's stream.  enable != 0
+ * starts recording (graph replays are not split per kernel). */
+int fsgh_profile_enable(fsgh_ctx *ctx, int32_t enable);
+/* names: '\n'-separated kernel names; ms: accumulated milliseconds; calls:
+ * launch counts.  max entries in, count out. */
+int fsgh_profile_read(fsgh_ctx *ctx, int32_t max, char *names, size_t names_bytes,
+                      double *ms, int64_t *calls, int32_t *count);
+/* Scans the state for non-finite values or non-positive volume/mass. */
+int fsgh_check(fsgh_ctx *ctx);
+/* HIP stream handle of the context (hipStream_t, as void*). */
+void *fsgh_stream(fsgh_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSGH_H */
