@@ -513,9 +513,9 @@ __global__ __launch_bounds__(kBlock) void k_energy_state_div(DMesh m, DState s,
         const double sie = s.sie0[e] - alpha * c.dt / emass * power;
         s.sie1[e] = sie;
 
-        double x[8], y[8], z[8];
+        double x[8], y[8], z[8], B[24];
         gather3(s.x1, s.y1, s.z1, nd, x, y, z);
-        const double vol = hex_volume(x, y, z);
+        const double vol = hex_bmatrix_volume(x, y, z, B);
         s.vol[e] = vol;
         const double den = emass / vol;
         s.den[e] = den;
@@ -524,8 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_energy_state_div(DMesh m, DState s,
         s.pres[e] = p;
         s.sspd[e] = cs;
 
-        double B[24], u[8], v[8], w[8];
-        hex_bmatrix(x, y, z, B);
+        double u[8], v[8], w[8];
         gather3(s.u1, s.v1, s.w1, nd, u, v, w);
         s.div[e] = divergence(B, u, v, w, vol);
         if (dt_partial != nullptr)

@@ -95,24 +95,43 @@ def main():
     for (i, j, k), s in S.items():
         assert S.get((j, i, k), 0) == -s and S.get((i, k, j), 0) == -s
 
+    def ks(p, j):
+        return sorted(k for (a, b, k) in S if a == p and b == j)
+
     def inner(p, j, var):
-        ks = sorted(k for (a, b, k) in S if a == p and b == j)
-        terms = []
-        for q, k in enumerate(ks):
-            s = S[(p, j, k)]
+        terms = None
+        for q, k in enumerate(ks(p, j)):
+            sg = S[(p, j, k)]
             if q == 0:
-                terms.append(f"{var}[{k}]" if s > 0 else f"-{var}[{k}]")
+                terms = f"{var}[{k}]" if sg > 0 else f"-{var}[{k}]"
             else:
-                terms = [f"({terms[0]} {'+' if s > 0 else '-'} {var}[{k}])"]
-        return terms[0]
+                terms = f"({terms} {'+' if sg > 0 else '-'} {var}[{k}])"
+        return terms
 
     def js(p):
         return sorted({b for (a, b, k) in S if a == p})
 
-    def brow(p, A, Bv):
+    # Inner sums are antisymmetric: inner(j, p) evaluates the same terms with
+    # every sign flipped, and IEEE rounding commutes with negation, so
+    # inner(j, p) == -inner(p, j) bit for bit.  Each unordered pair is summed
+    # once per component.
+    COMP = [("y", "z"), ("z", "x"), ("x", "y")]
+
+    def shared_inners(w, d):
+        A, Bv = COMP[d]
+        for p in range(8):
+            for j in js(p):
+                if p < j:
+                    w(f"    const double i{d}_{p}{j} = {inner(p, j, Bv)};")
+
+    def iref(d, p, j):
+        return f"i{d}_{p}{j}" if p < j else f"(-i{d}_{j}{p})"
+
+    def brow_shared(d, p):
+        A, _ = COMP[d]
         acc = None
         for j in js(p):
-            t = f"{A}[{j}] * {inner(p, j, Bv)}"
+            t = f"{A}[{j}] * {iref(d, p, j)}"
             acc = t if acc is None else f"({acc}) + {t}"
         return acc
 
@@ -127,33 +146,53 @@ def main():
     w("#define FSGH_HD")
     w("#endif")
     w("")
-    for p in range(8):
-        w(f"FSGH_HD inline void hex_brow_{p}(const double* x, const double* y, const double* z,")
-        w("                                 double& b0, double& b1, double& b2) {")
-        w("    const double twelfth = 1. / 12.;")
-        w(f"    b0 = ({brow(p, 'y', 'z')}) * twelfth;")
-        w(f"    b1 = ({brow(p, 'z', 'x')}) * twelfth;")
-        w(f"    b2 = ({brow(p, 'x', 'y')}) * twelfth;")
-        w("}")
-        w("")
     w("FSGH_HD inline void hex_bmatrix(const double* x, const double* y, const double* z,")
     w("                                double* B) {")
-    for p in range(8):
-        w(f"    hex_brow_{p}(x, y, z, B[{3 * p}], B[{3 * p + 1}], B[{3 * p + 2}]);")
+    w("    const double twelfth = 1. / 12.;")
+    for d in range(3):
+        shared_inners(w, d)
+        for p in range(8):
+            w(f"    B[{3 * p + d}] = ({brow_shared(d, p)}) * twelfth;")
     w("}")
     w("")
+    # volume: x_i * (sum_j y_j * inner(i, j, z)) in the reference's order; the
+    # products y_j * inner(i, j, z) are the ones B(i, 0) sums.
+    def vol_expr(prod):
+        outer = None
+        for i in VOL_OUTER:
+            acc = None
+            assert sorted(VOL_JORDER[i]) == js(i)
+            for j in VOL_JORDER[i]:
+                t = prod(i, j)
+                acc = t if acc is None else f"({acc}) + {t}"
+            t = f"x[{i}] * ({acc})"
+            outer = t if outer is None else f"({outer})\n        + {t}"
+        return outer
+
     w("FSGH_HD inline double hex_volume(const double* x, const double* y, const double* z) {")
     w("    const double twelfth = 1. / 12.;")
-    outer = None
-    for i in VOL_OUTER:
+    shared_inners(w, 0)
+    w(f"    return ({vol_expr(lambda i, j: f'y[{j}] * {iref(0, i, j)}')}) * twelfth;")
+    w("}")
+    w("")
+    w("// B and V together: V reuses the products of B's x-component rows.")
+    w("FSGH_HD inline double hex_bmatrix_volume(const double* x, const double* y, const double* z,")
+    w("                                         double* B) {")
+    w("    const double twelfth = 1. / 12.;")
+    shared_inners(w, 0)
+    for p in range(8):
+        for j in js(p):
+            w(f"    const double t_{p}{j} = y[{j}] * {iref(0, p, j)};")
+    for p in range(8):
         acc = None
-        assert sorted(VOL_JORDER[i]) == js(i)
-        for j in VOL_JORDER[i]:
-            t = f"y[{j}] * {inner(i, j, 'z')}"
-            acc = t if acc is None else f"({acc}) + {t}"
-        t = f"x[{i}] * ({acc})"
-        outer = t if outer is None else f"({outer})\n        + {t}"
-    w(f"    return ({outer}) * twelfth;")
+        for j in js(p):
+            acc = f"t_{p}{j}" if acc is None else f"({acc}) + t_{p}{j}"
+        w(f"    B[{3 * p}] = ({acc}) * twelfth;")
+    for d in (1, 2):
+        shared_inners(w, d)
+        for p in range(8):
+            w(f"    B[{3 * p + d}] = ({brow_shared(d, p)}) * twelfth;")
+    w(f"    return ({vol_expr(lambda i, j: f't_{i}{j}')}) * twelfth;")
     w("}")
     print("\n".join(out))
 

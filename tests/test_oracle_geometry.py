@@ -113,7 +113,8 @@ _SRC = r"""
 #include "hex_geometry.inc"
 extern "C" void geo(const double* x, const double* y, const double* z, double* B, double* V) {
     hex_bmatrix(x, y, z, B);
-    *V = hex_volume(x, y, z);
+    V[0] = hex_volume(x, y, z);
+    V[1] = hex_bmatrix_volume(x, y, z, B + 24);
 }
 """
 
@@ -140,8 +141,11 @@ def test_generated_geometry_bitwise_equals_oracle(oracle, generated):
     for k in range(200):
         X = random_hex(rng, h=10.0 ** rng.uniform(-3, 0), offset=rng.uniform(0, 2, 3))
         x, y, z = (np.ascontiguousarray(X[:, d]) for d in range(3))
-        B = np.zeros(24)
-        V = np.zeros(1)
+        B = np.zeros(48)
+        V = np.zeros(2)
         generated.geo(x, y, z, B, V)
-        assert np.array_equal(B.reshape(8, 3), oracle.bmatrix(x, y, z)), k
-        assert V[0] == oracle.vol_hex(x, y, z), k
+        Bo = oracle.bmatrix(x, y, z)
+        assert np.array_equal(B[:24].reshape(8, 3), Bo), k
+        assert np.array_equal(B[24:].reshape(8, 3), Bo), k
+        Vo = oracle.vol_hex(x, y, z)
+        assert V[0] == Vo and V[1] == Vo, k
