@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfsgh.so")
 
 FSGH_OK = 0
+RUN_GRAPH, RUN_STOP_ON_OUTPUT = 1, 2
 ERRORS = {-1: "FSGH_E_INVALID", -2: "FSGH_E_HIP", -3: "FSGH_E_NOMEM", -4: "FSGH_E_NONFINITE",
           -5: "FSGH_E_STATE", -6: "FSGH_E_UNSUPPORTED", -7: "FSGH_E_NOTBUILT"}
 
@@ -67,7 +68,8 @@ class RunParams(C.Structure):
 
 class InstanceStatus(C.Structure):
     _fields_ = [("cycles", C.c_int64), ("time_value", C.c_double), ("dt", C.c_double),
-                ("graphics_time", C.c_double), ("stopped", C.c_int32), ("pad_", C.c_int32)]
+                ("graphics_time", C.c_double), ("stopped", C.c_int32),
+                ("write_pending", C.c_int32)]
 
 
 _lib = None

@@ -8,9 +8,15 @@ one material the material id is not read.  E elements, N nodes, C = 8E.
 from __future__ import annotations
 
 
+def div_extremes(E, N):
+    # corner CSR (4C + 4(N+1)) + div 8E + two (min, 2nd min)/(max, 2nd max) pairs 32N (w)
+    return (32 * E + 4 * (N + 1)) + 8 * E + 32 * N
+
+
 def force(E, N):
-    # conn 32E + x1,v1 48N + vol,div,den,sspd,pres 40E + corner CSR (4C + 4(N+1)) + F 24C
-    return 32 * E + 48 * N + 40 * E + (32 * E + 4 * (N + 1)) + 192 * E
+    # conn 32E + x1,v1 48N + vol,div,den,sspd,pres 40E + one divergence-extreme pair 16N
+    # + F 24C (w)
+    return 32 * E + 48 * N + 40 * E + 16 * N + 192 * E
 
 
 def vel_bc_pos(E, N):
@@ -23,10 +29,11 @@ def energy_state_div(E, N):
     return 192 * E + 32 * E + 72 * N + 16 * E + 48 * E + 16 * E
 
 
-KERNELS = {"force": force, "vel_bc_pos": vel_bc_pos, "energy_state_div": energy_state_div}
+KERNELS = {"div_extremes": div_extremes, "force": force, "vel_bc_pos": vel_bc_pos,
+           "energy_state_div": energy_state_div}
 
 
 def cycle(E, N):
     """Bytes of one fused RK2 cycle (2 stages; dt and control kernels are O(1)
     per block)."""
-    return 2 * (force(E, N) + vel_bc_pos(E, N) + energy_state_div(E, N))
+    return 2 * (div_extremes(E, N) + force(E, N) + vel_bc_pos(E, N) + energy_state_div(E, N))

@@ -36,7 +36,12 @@ struct fsgh_ctx {
     double* d_node[2][6] = {};
     double* d_sie[2] = {};
     double *d_nmass = nullptr, *d_den = nullptr, *d_pres = nullptr, *d_sspd = nullptr;
-    double *d_vol = nullptr, *d_div = nullptr, *d_emass = nullptr, *d_gamma = nullptr;
+    double *d_vol = nullptr, *d_emass = nullptr, *d_gamma = nullptr;
+    // two divergence buffers: stage s's force reads d_div[s] and the fused
+    // energy kernel of stage s writes d_div[1-s]; operators use d_div[divcur]
+    double* d_div[2] = {nullptr, nullptr};
+    int divcur = 0;
+    double2 *d_ext_lo = nullptr, *d_ext_hi = nullptr;
     double *d_csmin = nullptr, *d_F = nullptr;
     int32_t* d_mat = nullptr;
     uint8_t* d_bcmask = nullptr;
@@ -120,10 +125,11 @@ DState make_state(fsgh_ctx* c, int b0, int b1)
     s.bcmask = c->d_bcmask;
     s.den = c->d_den; s.pres = c->d_pres; s.sspd = c->d_sspd;
     s.sie0 = c->d_sie[b0]; s.sie1 = c->d_sie[b1];
-    s.vol = c->d_vol; s.div = c->d_div; s.emass = c->d_emass;
+    s.vol = c->d_vol; s.div = c->d_div[c->divcur]; s.div_out = s.div; s.emass = c->d_emass;
     s.gamma = c->d_gamma; s.csmin = c->d_csmin;
     s.mat = c->d_mat; s.F = c->d_F;
     s.mats = c->d_mats; s.nmat = c->nmat;
+    s.ext_lo = c->d_ext_lo; s.ext_hi = c->d_ext_hi;
     return s;
 }
 
@@ -415,7 +421,10 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
     rc |= dalloc(c, &c->d_pres, E);
     rc |= dalloc(c, &c->d_sspd, E);
     rc |= dalloc(c, &c->d_vol, E);
-    rc |= dalloc(c, &c->d_div, E);
+    rc |= dalloc(c, &c->d_div[0], E);
+    rc |= dalloc(c, &c->d_div[1], E);
+    rc |= dalloc(c, &c->d_ext_lo, N);
+    rc |= dalloc(c, &c->d_ext_hi, N);
     rc |= dalloc(c, &c->d_emass, E);
     rc |= dalloc(c, &c->d_gamma, E);
     rc |= dalloc(c, &c->d_csmin, E);
@@ -450,7 +459,7 @@ int fsgh_destroy(fsgh_ctx* c)
     resolve_profile(c);
     drop_graphs(c);
     void* ptrs[] = {c->d_conn, c->d_cin_start, c->d_cin, c->d_eblk, c->d_nblk, c->d_nmass,
-                    c->d_den, c->d_pres, c->d_sspd, c->d_vol, c->d_div, c->d_emass, c->d_gamma,
+                    c->d_den, c->d_pres, c->d_sspd, c->d_vol, c->d_div[0], c->d_div[1], c->d_ext_lo, c->d_ext_hi, c->d_emass, c->d_gamma,
                     c->d_csmin, c->d_F, c->d_mat, c->d_bcmask, c->d_mats, c->d_ctl, c->d_partial,
                     c->d_scalar, c->d_scratch, c->d_flag, c->d_bcnodes, c->d_bcmasks,
                     c->d_sie[0], c->d_sie[1]};
@@ -705,7 +714,7 @@ int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill*
     rc |= upe(c->d_sspd, sspd);
     rc |= upe(c->d_gamma, gam);
     rc |= upe(c->d_csmin, csm);
-    rc |= upe(c->d_div, zeroE);
+    rc |= upe(c->d_div[c->divcur], zeroE);
     for (int d = 0; d < 3; ++d) {
         rc |= upn(c->d_node[b1][3 + d], vel, 3, d);
         rc |= upn(c->d_node[b0][3 + d], zeroN, 3, d);
@@ -786,7 +795,7 @@ static int field_io(fsgh_ctx* c, int32_t field, void* host, size_t bytes, bool u
     case FSGH_ELEM_PRES: dst = c->d_pres; break;
     case FSGH_ELEM_SSPD: dst = c->d_sspd; break;
     case FSGH_ELEM_VOL: dst = c->d_vol; break;
-    case FSGH_ELEM_DIV: dst = c->d_div; break;
+    case FSGH_ELEM_DIV: dst = c->d_div[c->divcur]; break;
     case FSGH_ELEM_MASS: dst = c->d_emass; break;
     case FSGH_ELEM_GAMMA: dst = c->d_gamma; break;
     case FSGH_ELEM_CSMIN: dst = c->d_csmin; break;
@@ -872,6 +881,7 @@ int fsgh_get_force_sgh(fsgh_ctx* c, double fuzz, double small, double dt, double
     if (int rc = ready(c)) return rc;
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
+    timed(c, "div_extremes", [&] { launch_div_extremes(m, s, nullptr, c->stream); });
     timed(c, "force", [&] { launch_force(m, s, nullptr, fuzz, small, c->stream); });
     OP_EPILOGUE(c);
 }
@@ -982,8 +992,12 @@ int fsgh_run_begin(fsgh_ctx* c, const fsgh_run_params* p)
     }
     HIPCHK(c, hipMemcpyAsync(c->d_ctl, h.data(), h.size() * sizeof(InstCtl), hipMemcpyHostToDevice,
                              c->stream));
+    c->divcur = 0;
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
+    // the first stage's divergence comes from the current state
+    // (K/sgh_solve.cpp:364); later stages take it from the fused energy kernel
+    timed(c, "get_divergence", [&] { launch_get_divergence(m, s, c->stream); });
     timed(c, "dt_partial",
           [&] { launch_dt_partial(m, s, c->d_ctl, c->d_partial, 0, 0, 0, 0, 1, c->stream); });
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -998,23 +1012,31 @@ static void issue_cycle(fsgh_ctx* c, int old)
     DMesh m = make_mesh(c);
     DState st0 = make_state(c, old, old);   // stage 0 force reads the t_n state
     DState st = make_state(c, old, nw);
+    DState sd[2] = {st, st};
+    for (int k = 0; k < 2; ++k) {
+        sd[k].div = c->d_div[k];
+        sd[k].div_out = c->d_div[1 - k];
+    }
+    st0.div = c->d_div[0];
     timed(c, "cycle_dt", [&] { launch_cycle_dt(c->d_ctl, c->L.ninst, c->d_partial, c->stream); });
     for (int stage = 0; stage < 2; ++stage) {
         const double alpha = 1.0 / (2.0 - static_cast<double>(stage));   // K/sgh_solve.cpp:353
         const InstCtl* ctl = c->d_ctl;
-        const DState& sf = stage == 0 ? st0 : st;
+        const DState& sf = stage == 0 ? st0 : sd[1];
+        timed(c, "div_extremes", [&] { launch_div_extremes(m, sf, ctl, c->stream); });
         timed(c, "force", [&] {
             launch_force(m, sf, ctl, 0.0, 0.0, c->stream);
         });
         timed(c, "vel_bc_pos", [&] { launch_vel_bc_pos(m, st, ctl, alpha, c->stream); });
         timed(c, "energy_state_div", [&] {
-            launch_energy_state_div(m, st, ctl, alpha, stage == 1 ? c->d_partial : nullptr, c->stream);
+            launch_energy_state_div(m, sd[stage], ctl, alpha, stage == 1 ? c->d_partial : nullptr,
+                                    c->stream);
         });
     }
     timed(c, "time_advance", [&] { launch_time_advance(c->d_ctl, c->L.ninst, nw, c->stream); });
 }
 
-int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t use_graph)
+int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t flags)
 {
     OP_PROLOGUE(c);
     if (!c->run_active) {
@@ -1022,8 +1044,9 @@ int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t use_graph)
         return FSGH_E_STATE;
     }
     if (ncycles < 0) return FSGH_E_INVALID;
-    const bool graph = use_graph != 0 && !c->prof;
-    const int64_t poll = 32;
+    const bool graph = (flags & FSGH_RUN_GRAPH) != 0 && !c->prof;
+    const bool stop_on_output = (flags & FSGH_RUN_STOP_ON_OUTPUT) != 0;
+    const int64_t poll = stop_on_output ? 1 : 32;
     for (int64_t k = 0; k < ncycles; ++k) {
         const int old = c->cur;
         if (graph) {
@@ -1045,12 +1068,17 @@ int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t use_graph)
             HIPCHK(c, hipMemcpyAsync(h.data(), c->d_ctl, h.size() * sizeof(InstCtl),
                                      hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
-            bool all = true;
-            for (auto& q : h) all = all && q.stopped;
-            if (all) break;
+            bool all = true, out = false;
+            for (auto& q : h) {
+                all = all && q.stopped;
+                out = out || q.write_pending;
+            }
+            if (all || (stop_on_output && out)) break;
         }
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the reference's elem_div after a cycle is the last stage's input
+    c->divcur = 1;
     OP_EPILOGUE(c);
 }
 
@@ -1067,7 +1095,7 @@ int fsgh_run_status(fsgh_ctx* c, int32_t inst, fsgh_instance_status* out)
     out->dt = h.dt;
     out->graphics_time = h.graphics_time;
     out->stopped = h.stopped;
-    out->pad_ = 0;
+    out->write_pending = h.write_pending;
     return FSGH_OK;
 }
 

@@ -36,6 +36,8 @@ struct InstCtl {
     int32_t stopped;
     int32_t final_bin;       // physical bin holding the latest state once stopped
     int32_t eblk_begin, eblk_end;   // logical element-block range of the instance
+    int32_t write_pending;   // the last cycle met the output condition (K/sgh_solve.cpp:594-606)
+    int32_t pad_;
 };
 
 struct Material {
@@ -60,6 +62,9 @@ struct DState {
     double* nmass;
     const uint8_t* bcmask;     // per node: bit d set -> velocity component d forced to 0
     double *den, *pres, *sspd, *sie0, *sie1, *vol, *div, *emass, *gamma, *csmin;
+    double* div_out;           // where the fused energy kernel writes the next stage's div
+    double2* ext_lo;           // per node: two smallest element divergences (with multiplicity)
+    double2* ext_hi;           // per node: two largest element divergences
     const int32_t* mat;        // material id per element
     double* F;                 // 8E x 3 corner forces
     const Material* mats;
@@ -76,6 +81,7 @@ void launch_get_vol(const DMesh&, const DState&, hipStream_t);
 void launch_get_divergence(const DMesh&, const DState&, hipStream_t);
 void launch_force(const DMesh&, const DState&, const InstCtl*, double fuzz, double small,
                   hipStream_t);
+void launch_div_extremes(const DMesh&, const DState&, const InstCtl*, hipStream_t);
 void launch_update_velocity(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
 void launch_boundary_velocity(const int32_t* nodes, const uint8_t* masks, int32_t count,
                               const DState&, hipStream_t);

@@ -209,20 +209,19 @@ __global__ __launch_bounds__(kBlock) void k_force(DMesh m, DState s, const InstC
     // monotone in d_nb (every rounding step is monotone), so the minimum over
     // neighbours is reached at min d_nb when div+small >= 0 and at max d_nb
     // otherwise.  Neighbours = every other element sharing a node (the
-    // reference's elems_in_elem), walked through the corner CSR.
+    // reference's elems_in_elem).  k_div_extremes stores per node the two
+    // smallest and two largest divergences of its elements (with
+    // multiplicity); removing one copy of this element's own value leaves the
+    // extreme over the other elements of that node.
     const double b = div + small;
     const bool lo = b >= 0.0;
+    const double2* __restrict__ xs = lo ? s.ext_lo : s.ext_hi;
     double ext = lo ? HUGE_VAL : -HUGE_VAL;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        const int q0 = m.cin_start[nd[a]], q1 = m.cin_start[nd[a] + 1];
-        for (int q = q0; q < q1; ++q) {
-            const int e2 = m.cin[q] >> 3;
-            if (e2 != e) {
-                const double d2 = s.div[e2];
-                ext = lo ? fmin(ext, d2) : fmax(ext, d2);
-            }
-        }
+        const double2 t = xs[nd[a]];
+        const double other = (t.x == div) ? t.y : t.x;
+        ext = lo ? fmin(ext, other) : fmax(ext, other);
     }
     const double r_face = 0.9 * (ext + small) / b;
     const double r_min = fmin(r_face, 1.0);
@@ -274,20 +273,65 @@ __global__ __launch_bounds__(kBlock) void k_get_divergence(DMesh m, DState s)
     s.div[e] = divergence(B, u, v, w, s.vol[e]);
 }
 
-// K/momentum.cpp:23-49: CSR gather in corners_in_node order, no atomics
+// K/momentum.cpp:23-49: CSR gather in corners_in_node order, no atomics.
+// The first eight entries are fetched with independent loads, then summed in
+// list order; longer lists (irregular meshes) continue in a loop.
 __device__ __forceinline__ void gather_node_force(const DMesh& m, const double* __restrict__ F,
                                                   int n, double& f0, double& f1, double& f2)
 {
     f0 = 0.0;
     f1 = 0.0;
     f2 = 0.0;
-    const int q0 = m.cin_start[n], q1 = m.cin_start[n + 1];
-    for (int q = q0; q < q1; ++q) {
-        const size_t c = static_cast<size_t>(m.cin[q]);
-        f0 += F[3 * c + 0];
-        f1 += F[3 * c + 1];
-        f2 += F[3 * c + 2];
+    const int q0 = m.cin_start[n], cnt = m.cin_start[n + 1] - q0;
+    int c[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c[j] = j < cnt ? m.cin[q0 + j] : 0;
+    double g[8][3];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const size_t k = 3 * static_cast<size_t>(c[j]);
+        g[j][0] = j < cnt ? F[k] : 0.0;
+        g[j][1] = j < cnt ? F[k + 1] : 0.0;
+        g[j][2] = j < cnt ? F[k + 2] : 0.0;
     }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (j < cnt) {
+            f0 += g[j][0];
+            f1 += g[j][1];
+            f2 += g[j][2];
+        }
+    for (int q = q0 + 8; q < q0 + cnt; ++q) {
+        const size_t k = 3 * static_cast<size_t>(m.cin[q]);
+        f0 += F[k];
+        f1 += F[k + 1];
+        f2 += F[k + 2];
+    }
+}
+
+// Per node, the two smallest and two largest divergences of the elements
+// around it, counted with multiplicity (feeds the shock detector).
+__global__ __launch_bounds__(kBlock) void k_div_extremes(DMesh m, DState s, const InstCtl* ctl)
+{
+    const BlockItem bi = m.nblk[blockIdx.x];
+    if (ctl != nullptr && ctl[bi.inst].stopped) return;
+    const int n = bi.start + static_cast<int>(threadIdx.x);
+    if (n >= bi.end) return;
+    const int q0 = m.cin_start[n], cnt = m.cin_start[n + 1] - q0;
+    double d[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) d[j] = j < cnt ? s.div[m.cin[q0 + j] >> 3] : 0.0;
+    double mn1 = HUGE_VAL, mn2 = HUGE_VAL, mx1 = -HUGE_VAL, mx2 = -HUGE_VAL;
+    auto put = [&](double v) {
+        if (v < mn1) { mn2 = mn1; mn1 = v; } else if (v < mn2) { mn2 = v; }
+        if (v > mx1) { mx2 = mx1; mx1 = v; } else if (v > mx2) { mx2 = v; }
+    };
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (j < cnt) put(d[j]);
+    for (int q = q0 + 8; q < q0 + cnt; ++q) put(s.div[m.cin[q] >> 3]);
+    s.ext_lo[n] = make_double2(mn1, mn2);
+    s.ext_hi[n] = make_double2(mx1, mx2);
 }
 
 __global__ __launch_bounds__(kBlock) void k_update_velocity(DMesh m, DState s, double alpha,
@@ -446,17 +490,29 @@ __global__ __launch_bounds__(1024) void k_reduce_min(const double* partial, int3
 // --------------------------------------------------------------------------
 
 // start of a cycle: get_timestep's host part (K/time_integration.cpp:69-150)
-__global__ __launch_bounds__(kBlock) void k_cycle_dt(InstCtl* ctl, const double* partial)
+__global__ __launch_bounds__(1024) void k_cycle_dt(InstCtl* ctl, const double* partial)
 {
     InstCtl& c = ctl[blockIdx.x];
     if (c.stopped) return;
-    double v = DBL_MAX;
-    for (int i = c.eblk_begin + static_cast<int>(threadIdx.x); i < c.eblk_end; i += kBlock)
-        v = fmin(v, partial[i]);
-    v = block_min(v);
+    double v0 = DBL_MAX, v1 = DBL_MAX, v2 = DBL_MAX, v3 = DBL_MAX;
+    int i = c.eblk_begin + static_cast<int>(threadIdx.x);
+    for (; i + 3 * 1024 < c.eblk_end; i += 4 * 1024) {
+        v0 = fmin(v0, partial[i]);
+        v1 = fmin(v1, partial[i + 1024]);
+        v2 = fmin(v2, partial[i + 2048]);
+        v3 = fmin(v3, partial[i + 3072]);
+    }
+    for (; i < c.eblk_end; i += 1024) v0 = fmin(v0, partial[i]);
+    double v = fmin(fmin(v0, v1), fmin(v2, v3));
+    __shared__ double red[16];
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int k = 1; k < 16; ++k) r = fmin(r, red[k]);
         double dt = c.dt * 1.1;
-        if (v < dt) dt = v;
+        if (r < dt) dt = r;
         dt = fmin(dt, (c.graphics_time - c.time) + c.fuzz);
         dt = fmin(dt, c.time_final - c.time);
         c.dt = dt;
@@ -526,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_energy_state_div(DMesh m, DState s,
 
         double u[8], v[8], w[8];
         gather3(s.u1, s.v1, s.w1, nd, u, v, w);
-        s.div[e] = divergence(B, u, v, w, vol);
+        s.div_out[e] = divergence(B, u, v, w, vol);
         if (dt_partial != nullptr)
             dtl = elem_dt(min_pair_distance(x, y, z), cs, c.dt_cfl, c.dt_max, c.dt_min, c.fuzz);
     }
@@ -550,6 +606,7 @@ __global__ void k_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin)
     else if (c.time >= c.time_final) write = 1;
     else if (c.time >= c.graphics_time) write = 1;
     if (write) c.graphics_time = c.time + c.graphics_dt_ival;
+    c.write_pending = write;
     c.cycle = cycle + 1;
     if (c.time >= c.time_final || c.cycle >= c.cycle_stop) {
         c.stopped = 1;
@@ -655,6 +712,11 @@ void launch_force(const DMesh& m, const DState& s, const InstCtl* ctl, double fu
     if (m.n_eblk)
         hipLaunchKernelGGL(k_force, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, ctl, fuzz, small);
 }
+void launch_div_extremes(const DMesh& m, const DState& s, const InstCtl* ctl, hipStream_t st)
+{
+    if (m.n_nblk)
+        hipLaunchKernelGGL(k_div_extremes, dim3(m.n_nblk), dim3(kBlock), 0, st, m, s, ctl);
+}
 void launch_update_velocity(const DMesh& m, const DState& s, double alpha, double dt,
                             hipStream_t st)
 {
@@ -702,7 +764,7 @@ void launch_reduce_min(const double* partial, int32_t n, double* out, hipStream_
 }
 void launch_cycle_dt(InstCtl* ctl, int32_t ninst, const double* partial, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cycle_dt, dim3(ninst), dim3(kBlock), 0, st, ctl, partial);
+    hipLaunchKernelGGL(k_cycle_dt, dim3(ninst), dim3(1024), 0, st, ctl, partial);
 }
 void launch_vel_bc_pos(const DMesh& m, const DState& s, const InstCtl* ctl, double alpha,
                        hipStream_t st)

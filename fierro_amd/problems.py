@@ -200,3 +200,42 @@ def sod_ensemble(count=512, seed=0, cycles=500):
         gamma = float(rng.uniform(1.3, 1.7))
         out.append(sod_tube(nx, rho_r, p_r, gamma, cycles=cycles))
     return out
+
+
+def noh(n=16, cycles=100, dt_cfl=0.4):
+    """Noh implosion (K/input.cpp:297-340): rho=1, e=1e-9, spherical inward
+    velocity of speed 1 on [0,1]^3 with n^3 hexes, reflected x=y=z=0 planes,
+    q2 = 1.3333, gamma 5/3, csmin 1e-14."""
+    spec = ProblemSpec(
+        name=f"noh{n}",
+        mesh=dict(kind="box", nx=n, ny=n, nz=n, lx=1.0, ly=1.0, lz=1.0, lden=1.0),
+        materials=[(1.0, 1.0, 1.3333, 0.0)],
+        state_vars=_gas(5.0 / 3.0),
+        mat_num_state_vars=[6],
+        fills=[fill(GLOBAL, den=1.0, sie=1e-9, velocity=SPHERICAL, speed=-1.0)],
+        bcs=[(X_PLANE, 0.0, REFLECTED), (Y_PLANE, 0.0, REFLECTED), (Z_PLANE, 0.0, REFLECTED)],
+    )
+    spec.run.update(time_final=0.6, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
+                    dt_cfl=dt_cfl, graphics_dt_ival=1.0e16, graphics_cyc_ival=1000000000,
+                    cycle_stop=cycles)
+    return spec
+
+
+def anvil(nx=16, ny=16, nz=96, cycles=400, speed=0.2, dt_cfl=0.5):
+    """Taylor-anvil-style impact: a 0.32 x 0.32 x 2.0 bar of nx x ny x nz
+    hexes, ideal gas gamma=1.4, rho=1, e=1e-3, uniform velocity (0,0,-speed)
+    into a rigid reflected wall at z=0, with symmetry (reflected) planes x=0
+    and y=0 as in the reference's anvil input (K/input.cpp:627-641)."""
+    spec = ProblemSpec(
+        name=f"anvil{nx}x{ny}x{nz}",
+        mesh=dict(kind="box", nx=nx, ny=ny, nz=nz, lx=0.32, ly=0.32, lz=2.0, lden=1.0),
+        materials=[(1.0, 1.0, 1.333, 0.0)],
+        state_vars=_gas(1.4),
+        mat_num_state_vars=[3],
+        fills=[fill(GLOBAL, den=1.0, sie=1.0e-3, velocity=CARTESIAN, w=-speed)],
+        bcs=[(X_PLANE, 0.0, REFLECTED), (Y_PLANE, 0.0, REFLECTED), (Z_PLANE, 0.0, REFLECTED)],
+    )
+    spec.run.update(time_final=1.0e16, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
+                    dt_cfl=dt_cfl, graphics_dt_ival=1.0e16, graphics_cyc_ival=1000000000,
+                    cycle_stop=cycles)
+    return spec

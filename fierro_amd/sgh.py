@@ -231,20 +231,38 @@ class SGHSolver:
         arr = (_abi.RunParams * len(ps))(*ps)
         check(self.L.fsgh_run_begin(self.ctx, arr), self.ctx, "run_begin")
 
-    def run(self, ncycles, use_graph=True):
-        check(self.L.fsgh_run(self.ctx, int(ncycles), 1 if use_graph else 0), self.ctx, "run")
+    def run(self, ncycles, use_graph=True, stop_on_output=False):
+        flags = (_abi.RUN_GRAPH if use_graph else 0) | \
+            (_abi.RUN_STOP_ON_OUTPUT if stop_on_output else 0)
+        check(self.L.fsgh_run(self.ctx, int(ncycles), flags), self.ctx, "run")
 
     def status(self, inst=0):
         st = _abi.InstanceStatus()
         check(self.L.fsgh_run_status(self.ctx, inst, C.byref(st)), self.ctx, "status")
-        return {k: getattr(st, k) for k, _ in _abi.InstanceStatus._fields_ if k != "pad_"}
+        return {k: getattr(st, k) for k, _ in _abi.InstanceStatus._fields_}
 
-    def sgh_solve(self, cycles=None, use_graph=True):
-        """The cycle loop of K/sgh_solve.cpp:125-634 (fused kernels)."""
+    def sgh_solve(self, cycles=None, use_graph=True, output=None):
+        """The cycle loop of K/sgh_solve.cpp:125-634 (fused kernels).  With
+        ``output(solver, time)`` (single instance) the dumps are written at
+        t=0 and at every cycle meeting the output condition, as
+        write_outputs is called at K/sgh_solve.cpp:45,609-628."""
         ie0 = [self.energy(i) for i in range(self.ninst)]
         self.run_begin()
         n = cycles if cycles is not None else max(int(s.run["cycle_stop"]) for s in self.specs)
-        self.run(n, use_graph)
+        if output is None:
+            self.run(n, use_graph)
+        else:
+            output(self, 0.0)
+            done = 0
+            while done < n:
+                before = self.status(0)["cycles"]
+                self.run(n - done, use_graph, stop_on_output=True)
+                st = self.status(0)
+                done += st["cycles"] - before
+                if st["write_pending"]:
+                    output(self, st["time_value"])
+                if st["stopped"] or st["cycles"] == before:
+                    break
         ie1 = [self.energy(i) for i in range(self.ninst)]
         return [dict(ie0=a[0], ke0=a[1], te0=a[0] + a[1], ie1=b[0], ke1=b[1], te1=b[0] + b[1],
                      **self.status(i)) for i, (a, b) in enumerate(zip(ie0, ie1))]

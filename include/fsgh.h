@@ -191,7 +191,7 @@ typedef struct fsgh_instance_status {
     double dt;                         /* last dt */
     double graphics_time;
     int32_t stopped;                   /* time >= time_final or cycle_stop reached */
-    int32_t pad_;
+    int32_t write_pending;             /* the last cycle met the output condition */
 } fsgh_instance_status;
 
 /* Starts the cycle loop of every instance at time 0 with dt = dt_start
@@ -199,10 +199,16 @@ typedef struct fsgh_instance_status {
 int fsgh_run_begin(fsgh_ctx *ctx, const fsgh_run_params *params);
 /* Advances up to `ncycles` cycles of the loop of K/sgh_solve.cpp:125-634
  * with fused kernels: dt (device-resident, x1.1 growth, clamps, per instance),
- * then per RK stage force -> velocity+BC+position -> energy+volume+EOS+div.
- * use_graph != 0 replays one captured hipGraph per cycle.  Returns when all
- * instances stopped or ncycles were issued; synchronises. */
-int fsgh_run(fsgh_ctx *ctx, int64_t ncycles, int32_t use_graph);
+ * then per RK stage divergence extremes -> force -> velocity+BC+position ->
+ * energy+volume+EOS+div.  flags: FSGH_RUN_GRAPH replays one captured hipGraph
+ * per cycle; FSGH_RUN_STOP_ON_OUTPUT returns after the first cycle in which an
+ * instance met the output condition (K/sgh_solve.cpp:594-606; see
+ * fsgh_instance_status.write_pending) so the caller can write its dump.
+ * Returns when all instances stopped, on such an output cycle, or after
+ * ncycles; synchronises. */
+#define FSGH_RUN_GRAPH 1
+#define FSGH_RUN_STOP_ON_OUTPUT 2
+int fsgh_run(fsgh_ctx *ctx, int64_t ncycles, int32_t flags);
 int fsgh_run_status(fsgh_ctx *ctx, int32_t instance, fsgh_instance_status *out);
 
 /* ------------------------------------------------------- diagnostics --- */

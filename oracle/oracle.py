@@ -66,7 +66,7 @@ class OrcRunParams(C.Structure):
                 ("graphics_time", C.c_double), ("graphics_dt_ival", C.c_double),
                 ("graphics_cyc_ival", C.c_int64), ("cycle_stop", C.c_int64),
                 ("rk_num_stages", C.c_int64), ("fuzz", C.c_double),
-                ("small", C.c_double)]
+                ("small", C.c_double), ("dt_replay", C.c_void_p), ("dt_record", C.c_void_p)]
 
 
 class OrcRunResult(C.Structure):
@@ -282,18 +282,26 @@ class OracleProblem:
                                   C.byref(ke))
         return ie.value, ke.value
 
-    def solve(self, cycle_stop=None):
+    def solve(self, cycle_stop=None, dt_replay=None, record_dt=False):
         r = dict(self.spec.run)
         if cycle_stop is not None:
             r["cycle_stop"] = cycle_stop
+        ncyc = int(r["cycle_stop"])
+        rep = None if dt_replay is None else np.ascontiguousarray(dt_replay, np.float64)
+        if rep is not None and rep.size < ncyc:
+            raise ValueError("dt_replay shorter than cycle_stop")
+        rec = np.zeros(ncyc) if record_dt else None
         p = OrcRunParams(0.0, r["time_final"], r["dt_start"], r["dt_max"], r["dt_min"],
                          r["dt_cfl"], r["graphics_dt_ival"], r["graphics_dt_ival"],
-                         int(r["graphics_cyc_ival"]), int(r["cycle_stop"]), 2, r["fuzz"],
-                         r["small"])
+                         int(r["graphics_cyc_ival"]), ncyc, 2, r["fuzz"],
+                         r["small"], _ptr(rep), _ptr(rec))
         res = OrcRunResult()
         self.L.orc_sgh_solve(C.byref(self._mesh), C.byref(self._state), self._mat, self._bc,
                              C.byref(p), C.byref(res))
-        return {k: getattr(res, k) for k, _ in OrcRunResult._fields_}
+        out = {k: getattr(res, k) for k, _ in OrcRunResult._fields_}
+        if record_dt:
+            out["dt_history"] = rec[:res.cycles]
+        return out
 
     def stage(self, dt, rk_alpha):
         """One RK stage in the reference order (K/sgh_solve.cpp:355-486)."""

@@ -430,8 +430,10 @@ void orc_sgh_solve(const orc_mesh *m, orc_state *s, const orc_material *mat,
     int64_t cycle = 0, done = 0;
     double t0 = now_seconds();
     for (cycle = 0; cycle < p->cycle_stop; ++cycle) {
-        orc_get_timestep(m, s, time_value, graphics_time, p->time_final, p->dt_max, p->dt_min,
-                         p->dt_cfl, &dt, p->fuzz);
+        if (p->dt_replay) dt = p->dt_replay[cycle];
+        else orc_get_timestep(m, s, time_value, graphics_time, p->time_final, p->dt_max, p->dt_min,
+                              p->dt_cfl, &dt, p->fuzz);
+        if (p->dt_record) p->dt_record[cycle] = dt;
         orc_rk_init(m, s);
         for (int64_t stage = 0; stage < p->rk_num_stages; ++stage) {
             double rk_alpha = 1.0 / ((double)p->rk_num_stages - (double)stage);

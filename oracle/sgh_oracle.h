@@ -98,6 +98,9 @@ typedef struct orc_run_params {
     int64_t cycle_stop;
     int64_t rk_num_stages;
     double fuzz, small;
+    const double *dt_replay;  /* non-NULL: use dt_replay[cycle] instead of get_timestep
+                                 (frozen time steps, for derivative checks) */
+    double *dt_record;        /* non-NULL: store the dt of every cycle */
 } orc_run_params;
 
 typedef struct orc_run_result {
