@@ -14,7 +14,7 @@
 
 #define FSGH_HD __host__ __device__
 #include "fsgh_internal.h"
-#include "hex_geometry.inc"
+#include "fsgh_physics.h"
 
 namespace fsgh {
 namespace {
@@ -82,25 +82,6 @@ __device__ __forceinline__ double block_sum(double v)
     return r;
 }
 
-// velocity divergence from B, v and V (K/momentum.cpp:265-286)
-__device__ __forceinline__ double divergence(const double B[24], const double u[8],
-                                             const double v[8], const double w[8], double vol)
-{
-    const double inverse_vol = 1.0 / vol;
-    double sx = u[0] * B[0], sy = v[0] * B[1], sz = w[0] * B[2];
-#pragma unroll
-    for (int a = 1; a < 8; ++a) {
-        sx = sx + u[a] * B[3 * a + 0];
-        sy = sy + v[a] * B[3 * a + 1];
-        sz = sz + w[a] * B[3 * a + 2];
-    }
-    double div = 0.0;
-    div += sx * inverse_vol;
-    div += sy * inverse_vol;
-    div += sz * inverse_vol;
-    return div;
-}
-
 // shortest of the 28 node-pair distances (K/time_integration.cpp:101-128).
 // sqrt is correctly rounded and monotone, so min(sqrt(d2)) == sqrt(min(d2)).
 __device__ __forceinline__ double min_pair_distance(const double x[8], const double y[8],
@@ -127,16 +108,6 @@ __device__ __forceinline__ double elem_dt(double dist_min, double sspd, double d
     return d;
 }
 
-// ideal gas (K/eos.cpp:30-43)
-__device__ __forceinline__ void ideal_gas(double gamma, double csmin, double den, double sie,
-                                          double& pres, double& sspd)
-{
-    pres = (gamma - 1.0) * sie * den;
-    double c = sqrt(gamma * (gamma - 1.0) * sie);
-    if (c < csmin) c = csmin;
-    sspd = c;
-}
-
 // --------------------------------------------------------------------------
 // corner force, 3D, strength-free (K/force_sgh.cpp:35-359)
 // --------------------------------------------------------------------------
@@ -154,12 +125,8 @@ __global__ __launch_bounds__(kBlock) void k_force(DMesh m, DState s, const InstC
 
     int32_t nd[8];
     load_conn(m.conn, e, nd);
-    double x[8], y[8], z[8];
+    double x[8], y[8], z[8], u[8], v[8], w[8];
     gather3(s.x1, s.y1, s.z1, nd, x, y, z);
-    double B[24];
-    hex_bmatrix(x, y, z, B);
-
-    double u[8], v[8], w[8];
     gather3(s.u1, s.v1, s.w1, nd, u, v, w);
     const double vol = s.vol[e];
     const double div = s.div[e];
@@ -168,53 +135,15 @@ __global__ __launch_bounds__(kBlock) void k_force(DMesh m, DState s, const InstC
     const double pres = s.pres[e];
     const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
 
-    // MARS: cell-average velocity estimate (:147-164)
-    double vsx = 0.0, vsy = 0.0, vsz = 0.0;
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        vsx += 0.125 * u[a];
-        vsy += 0.125 * v[a];
-        vsz += 0.125 * w[a];
-    }
-    // impedance x |corner area| per node (:177-255); compression vs expansion (:217-224)
-    const double qa = div < 0 ? mt.q1 : mt.q1ex;
-    const double qb = div < 0 ? mt.q2 : mt.q2ex;
-    double sum0 = 0.0, sum1 = 0.0, sum2 = 0.0, sum3 = 0.0;
-    double mu[8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const double dx = u[a] - vsx, dy = v[a] - vsy, dz = w[a] - vsz;
-        const double mag_vel = sqrt(dx * dx + dy * dy + dz * dz);
-        const double muc = den * (qa * sspd + qb * mag_vel);
-        const double bx = B[3 * a], by = B[3 * a + 1], bz = B[3 * a + 2];
-        const double mu_term = muc * sqrt(bx * bx + by * by + bz * bz);
-        sum0 += mu_term * u[a];
-        sum1 += mu_term * v[a];
-        sum2 += mu_term * w[a];
-        sum3 += mu_term;
-        mu[a] = mu_term;
-    }
-    // Riemann velocity (:261-270)
-    if (sum3 > fuzz) {
-        vsx = sum0 / sum3;
-        vsy = sum1 / sum3;
-        vsz = sum2 / sum3;
-    } else {
-        vsx = 0.0;
-        vsy = 0.0;
-        vsz = 0.0;
-    }
-
-    // Shock detector (:287-328).  r_face = 0.9 (d_nb + small)/(div + small) is
-    // monotone in d_nb (every rounding step is monotone), so the minimum over
-    // neighbours is reached at min d_nb when div+small >= 0 and at max d_nb
-    // otherwise.  Neighbours = every other element sharing a node (the
-    // reference's elems_in_elem).  k_div_extremes stores per node the two
-    // smallest and two largest divergences of its elements (with
-    // multiplicity); removing one copy of this element's own value leaves the
-    // extreme over the other elements of that node.
-    const double b = div + small;
-    const bool lo = b >= 0.0;
+    // Shock detector neighbour extreme (K/force_sgh.cpp:297-310).  r_face =
+    // 0.9 (d_nb + small)/(div + small) is monotone in d_nb (every rounding
+    // step is monotone), so the minimum over neighbours is reached at min d_nb
+    // when div+small >= 0 and at max d_nb otherwise.  Neighbours = every other
+    // element sharing a node (the reference's elems_in_elem).  k_div_extremes
+    // stores per node the two smallest and two largest divergences of its
+    // elements (with multiplicity); removing one copy of this element's own
+    // value leaves the extreme over the other elements of that node.
+    const bool lo = div + small >= 0.0;
     const double2* __restrict__ xs = lo ? s.ext_lo : s.ext_hi;
     double ext = lo ? HUGE_VAL : -HUGE_VAL;
 #pragma unroll
@@ -223,23 +152,10 @@ __global__ __launch_bounds__(kBlock) void k_force(DMesh m, DState s, const InstC
         const double other = (t.x == div) ? t.y : t.x;
         ext = lo ? fmin(ext, other) : fmax(ext, other);
     }
-    const double r_face = 0.9 * (ext + small) / b;
-    const double r_min = fmin(r_face, 1.0);
-    double phi = 1.0 - fmax(0.0, r_min);  // pow(phi, 1.0) == phi
-    const double c_length = pow(vol, 1.0 / 3.0);
-    const double alpha = fmin(1.0, 20.0 * (c_length * fabs(div)) / (sspd + fuzz));
-    phi = alpha * phi;
-
-    // corner forces (:338-359): a = -B, tau = -p I, so a_i . tau_d = B_id p
-    double2* out = reinterpret_cast<double2*>(s.F + 24 * static_cast<size_t>(e));
     double f[24];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const double pm = phi * mu[a];
-        f[3 * a + 0] = B[3 * a + 0] * pres + pm * (vsx - u[a]);
-        f[3 * a + 1] = B[3 * a + 1] * pres + pm * (vsy - v[a]);
-        f[3 * a + 2] = B[3 * a + 2] * pres + pm * (vsz - w[a]);
-    }
+    force_elem(x, y, z, u, v, w, den, pres, sspd, vol, div, ext, mt.q1, mt.q1ex, mt.q2, mt.q2ex,
+               fuzz, small, f);
+    double2* out = reinterpret_cast<double2*>(s.F + 24 * static_cast<size_t>(e));
 #pragma unroll
     for (int i = 0; i < 12; ++i) out[i] = make_double2(f[2 * i], f[2 * i + 1]);
 }
@@ -274,8 +190,9 @@ __global__ __launch_bounds__(kBlock) void k_get_divergence(DMesh m, DState s)
 }
 
 // K/momentum.cpp:23-49: CSR gather in corners_in_node order, no atomics.
-// The first eight entries are fetched with independent loads, then summed in
-// list order; longer lists (irregular meshes) continue in a loop.
+// The first eight entries are fetched with independent loads (indices clamped
+// to the list, so every load is in bounds and none waits on a branch), then
+// summed in list order; longer lists (irregular meshes) continue in a loop.
 __device__ __forceinline__ void gather_node_force(const DMesh& m, const double* __restrict__ F,
                                                   int n, double& f0, double& f1, double& f2)
 {
@@ -283,24 +200,23 @@ __device__ __forceinline__ void gather_node_force(const DMesh& m, const double* 
     f1 = 0.0;
     f2 = 0.0;
     const int q0 = m.cin_start[n], cnt = m.cin_start[n + 1] - q0;
-    int c[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) c[j] = j < cnt ? m.cin[q0 + j] : 0;
-    double g[8][3];
+    if (cnt <= 0) return;
+    const int last = cnt - 1;
+    double g0[8], g1[8], g2[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        const size_t k = 3 * static_cast<size_t>(c[j]);
-        g[j][0] = j < cnt ? F[k] : 0.0;
-        g[j][1] = j < cnt ? F[k + 1] : 0.0;
-        g[j][2] = j < cnt ? F[k + 2] : 0.0;
+        const size_t k = 3 * static_cast<size_t>(m.cin[q0 + (j < cnt ? j : last)]);
+        g0[j] = F[k];
+        g1[j] = F[k + 1];
+        g2[j] = F[k + 2];
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-        if (j < cnt) {
-            f0 += g[j][0];
-            f1 += g[j][1];
-            f2 += g[j][2];
-        }
+    for (int j = 0; j < 8; ++j) {
+        const bool ok = j < cnt;
+        f0 = ok ? f0 + g0[j] : f0;
+        f1 = ok ? f1 + g1[j] : f1;
+        f2 = ok ? f2 + g2[j] : f2;
+    }
     for (int q = q0 + 8; q < q0 + cnt; ++q) {
         const size_t k = 3 * static_cast<size_t>(m.cin[q]);
         f0 += F[k];
@@ -309,8 +225,22 @@ __device__ __forceinline__ void gather_node_force(const DMesh& m, const double* 
     }
 }
 
+// insert v into the pair (smallest, second smallest), multiplicity counted
+__device__ __forceinline__ void insert_min2(double& a, double& b, double v)
+{
+    b = fmin(b, fmax(a, v));
+    a = fmin(a, v);
+}
+__device__ __forceinline__ void insert_max2(double& a, double& b, double v)
+{
+    b = fmax(b, fmin(a, v));
+    a = fmax(a, v);
+}
+
 // Per node, the two smallest and two largest divergences of the elements
-// around it, counted with multiplicity (feeds the shock detector).
+// around it, counted with multiplicity (feeds the shock detector).  A NaN
+// divergence is left out, as fmin/fmax leave it out of the reference's
+// neighbour minimum.
 __global__ __launch_bounds__(kBlock) void k_div_extremes(DMesh m, DState s, const InstCtl* ctl)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
@@ -318,18 +248,26 @@ __global__ __launch_bounds__(kBlock) void k_div_extremes(DMesh m, DState s, cons
     const int n = bi.start + static_cast<int>(threadIdx.x);
     if (n >= bi.end) return;
     const int q0 = m.cin_start[n], cnt = m.cin_start[n + 1] - q0;
-    double d[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) d[j] = j < cnt ? s.div[m.cin[q0 + j] >> 3] : 0.0;
     double mn1 = HUGE_VAL, mn2 = HUGE_VAL, mx1 = -HUGE_VAL, mx2 = -HUGE_VAL;
-    auto put = [&](double v) {
-        if (v < mn1) { mn2 = mn1; mn1 = v; } else if (v < mn2) { mn2 = v; }
-        if (v > mx1) { mx2 = mx1; mx1 = v; } else if (v > mx2) { mx2 = v; }
-    };
+    if (cnt > 0) {
+        const int last = cnt - 1;
+        double d[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-        if (j < cnt) put(d[j]);
-    for (int q = q0 + 8; q < q0 + cnt; ++q) put(s.div[m.cin[q] >> 3]);
+        for (int j = 0; j < 8; ++j) d[j] = s.div[m.cin[q0 + (j < cnt ? j : last)] >> 3];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool ok = j < cnt && !isnan(d[j]);
+            insert_min2(mn1, mn2, ok ? d[j] : HUGE_VAL);
+            insert_max2(mx1, mx2, ok ? d[j] : -HUGE_VAL);
+        }
+        for (int q = q0 + 8; q < q0 + cnt; ++q) {
+            const double v = s.div[m.cin[q] >> 3];
+            if (!isnan(v)) {
+                insert_min2(mn1, mn2, v);
+                insert_max2(mx1, mx2, v);
+            }
+        }
+    }
     s.ext_lo[n] = make_double2(mn1, mn2);
     s.ext_hi[n] = make_double2(mx1, mx2);
 }
@@ -362,26 +300,15 @@ __global__ __launch_bounds__(kBlock) void k_boundary_velocity(const int32_t* nod
     if (mk & 4) s.w1[n] = 0.0;
 }
 
-// element power F . (v1+v0)/2 summed corner by corner (K/energy_sgh.cpp:18-46)
-__device__ __forceinline__ double elem_power(const DState& s, int e, const int32_t nd[8])
+__device__ __forceinline__ void load_forces(const DState& s, int e, double f[24])
 {
     const double2* Fp = reinterpret_cast<const double2*>(s.F + 24 * static_cast<size_t>(e));
-    double f[24];
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
         const double2 t = Fp[i];
         f[2 * i] = t.x;
         f[2 * i + 1] = t.y;
     }
-    double power = 0.0;
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const int n = nd[a];
-        power += f[3 * a + 0] * ((s.u1[n] + s.u0[n]) * 0.5);
-        power += f[3 * a + 1] * ((s.v1[n] + s.v0[n]) * 0.5);
-        power += f[3 * a + 2] * ((s.w1[n] + s.w0[n]) * 0.5);
-    }
-    return power;
 }
 
 __global__ __launch_bounds__(kBlock) void k_update_energy(DMesh m, DState s, double alpha,
@@ -392,8 +319,11 @@ __global__ __launch_bounds__(kBlock) void k_update_energy(DMesh m, DState s, dou
     if (e >= bi.end) return;
     int32_t nd[8];
     load_conn(m.conn, e, nd);
-    const double power = elem_power(s, e, nd);
-    s.sie1[e] = s.sie0[e] - alpha * dt / s.emass[e] * power;
+    double f[24], u1[8], v1[8], w1[8], u0[8], v0[8], w0[8];
+    load_forces(s, e, f);
+    gather3(s.u1, s.v1, s.w1, nd, u1, v1, w1);
+    gather3(s.u0, s.v0, s.w0, nd, u0, v0, w0);
+    s.sie1[e] = energy_update(f, u1, v1, w1, u0, v0, w0, s.sie0[e], s.emass[e], alpha, dt);
 }
 
 __global__ __launch_bounds__(kBlock) void k_update_position(DMesh m, DState s, double alpha,
@@ -530,23 +460,21 @@ __global__ __launch_bounds__(kBlock) void k_vel_bc_pos(DMesh m, DState s, const 
     const int n = bi.start + static_cast<int>(threadIdx.x);
     if (n >= bi.end) return;
     const double dt = c.dt;
-    double f0, f1, f2;
-    gather_node_force(m, s.F, n, f0, f1, f2);
-    const double mass = s.nmass[n];
-    const double u0 = s.u0[n], v0 = s.v0[n], w0 = s.w0[n];
-    double u1 = u0 + alpha * dt * f0 / mass;
-    double v1 = v0 + alpha * dt * f1 / mass;
-    double w1 = w0 + alpha * dt * f2 / mass;
-    const uint8_t mk = s.bcmask[n];
-    if (mk & 1) u1 = 0.0;
-    if (mk & 2) v1 = 0.0;
-    if (mk & 4) w1 = 0.0;
-    s.u1[n] = u1;
-    s.v1[n] = v1;
-    s.w1[n] = w1;
-    s.x1[n] = s.x0[n] + alpha * dt * ((u1 + u0) * 0.5);
-    s.y1[n] = s.y0[n] + alpha * dt * ((v1 + v0) * 0.5);
-    s.z1[n] = s.z0[n] + alpha * dt * ((w1 + w0) * 0.5);
+    double f[3], v0[3], x0[3], v1[3], x1[3];
+    gather_node_force(m, s.F, n, f[0], f[1], f[2]);
+    v0[0] = s.u0[n];
+    v0[1] = s.v0[n];
+    v0[2] = s.w0[n];
+    x0[0] = s.x0[n];
+    x0[1] = s.y0[n];
+    x0[2] = s.z0[n];
+    node_update(f, s.nmass[n], v0, x0, s.bcmask[n], alpha, dt, v1, x1);
+    s.u1[n] = v1[0];
+    s.v1[n] = v1[1];
+    s.w1[n] = v1[2];
+    s.x1[n] = x1[0];
+    s.y1[n] = x1[1];
+    s.z1[n] = x1[2];
 }
 
 // energy + volume + EOS + next divergence (+ next cycle's dt candidates),
@@ -564,25 +492,23 @@ __global__ __launch_bounds__(kBlock) void k_energy_state_div(DMesh m, DState s,
     if (e < bi.end) {
         int32_t nd[8];
         load_conn(m.conn, e, nd);
-        const double power = elem_power(s, e, nd);
+        double f[24], x[8], y[8], z[8], u1[8], v1[8], w1[8], u0[8], v0[8], w0[8];
+        load_forces(s, e, f);
+        gather3(s.u1, s.v1, s.w1, nd, u1, v1, w1);
+        gather3(s.u0, s.v0, s.w0, nd, u0, v0, w0);
         const double emass = s.emass[e];
-        const double sie = s.sie0[e] - alpha * c.dt / emass * power;
+        const double sie =
+            energy_update(f, u1, v1, w1, u0, v0, w0, s.sie0[e], emass, alpha, c.dt);
         s.sie1[e] = sie;
-
-        double x[8], y[8], z[8], B[24];
         gather3(s.x1, s.y1, s.z1, nd, x, y, z);
-        const double vol = hex_bmatrix_volume(x, y, z, B);
+        double vol, den, p, cs, div;
+        derived_state(x, y, z, u1, v1, w1, emass, sie, s.gamma[e], s.csmin[e], vol, den, p, cs,
+                      div);
         s.vol[e] = vol;
-        const double den = emass / vol;
         s.den[e] = den;
-        double p, cs;
-        ideal_gas(s.gamma[e], s.csmin[e], den, sie, p, cs);
         s.pres[e] = p;
         s.sspd[e] = cs;
-
-        double u[8], v[8], w[8];
-        gather3(s.u1, s.v1, s.w1, nd, u, v, w);
-        s.div_out[e] = divergence(B, u, v, w, vol);
+        s.div_out[e] = div;
         if (dt_partial != nullptr)
             dtl = elem_dt(min_pair_distance(x, y, z), cs, c.dt_cfl, c.dt_max, c.dt_min, c.fuzz);
     }

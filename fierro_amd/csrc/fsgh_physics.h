@@ -1,0 +1,160 @@
+// fsgh_physics.h -- the per-element / per-node physics of one SGH RK stage,
+// templated on the scalar type (double for the forward kernels, Dual for the
+// tangent-linear model).  Every expression keeps the reference's FP64
+// operation order (file:line at each function); with -ffp-contract=off the
+// double instantiation is bit-identical to the reference arithmetic.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#ifndef FSGH_HD
+#define FSGH_HD __host__ __device__
+#endif
+#include "fsgh_dual.h"
+#include "hex_geometry.inc"
+
+namespace fsgh {
+
+// velocity divergence from B, v and V (K/momentum.cpp:265-286)
+template <typename T>
+FSGH_HD inline T divergence(const T* B, const T* u, const T* v, const T* w, T vol)
+{
+    const T inverse_vol = 1.0 / vol;
+    T sx = u[0] * B[0], sy = v[0] * B[1], sz = w[0] * B[2];
+#pragma unroll
+    for (int a = 1; a < 8; ++a) {
+        sx = sx + u[a] * B[3 * a + 0];
+        sy = sy + v[a] * B[3 * a + 1];
+        sz = sz + w[a] * B[3 * a + 2];
+    }
+    T div = 0.0;
+    div += sx * inverse_vol;
+    div += sy * inverse_vol;
+    div += sz * inverse_vol;
+    return div;
+}
+
+// ideal gas (K/eos.cpp:30-43)
+template <typename T>
+FSGH_HD inline void ideal_gas(T gamma, double csmin, T den, T sie, T& pres, T& sspd)
+{
+    pres = (gamma - 1.0) * sie * den;
+    T c = sqrt(gamma * (gamma - 1.0) * sie);
+    if (c < csmin) c = csmin;
+    sspd = c;
+}
+
+// Corner forces of one element, 3D, strength-free (K/force_sgh.cpp:35-359).
+// `ext` is the extreme neighbour divergence chosen by the shock detector:
+// min over neighbours when div + small >= 0, max otherwise (the kernels
+// find it; r_face is monotone in the neighbour divergence).
+template <typename T>
+FSGH_HD inline void force_elem(const T* x, const T* y, const T* z, const T* u, const T* v,
+                               const T* w, T den, T pres, T sspd, T vol, T div, T ext,
+                               double q1, double q1ex, double q2, double q2ex, double fuzz,
+                               double small, T* f)
+{
+    T B[24];
+    hex_bmatrix(x, y, z, B);
+    // MARS: cell-average velocity estimate (:147-164)
+    T vsx = 0.0, vsy = 0.0, vsz = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        vsx += 0.125 * u[a];
+        vsy += 0.125 * v[a];
+        vsz += 0.125 * w[a];
+    }
+    // impedance x |corner area| per node (:177-255); compression vs expansion (:217-224)
+    const double qa = div < 0.0 ? q1 : q1ex;
+    const double qb = div < 0.0 ? q2 : q2ex;
+    T sum0 = 0.0, sum1 = 0.0, sum2 = 0.0, sum3 = 0.0;
+    T mu[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const T dx = u[a] - vsx, dy = v[a] - vsy, dz = w[a] - vsz;
+        const T mag_vel = sqrt(dx * dx + dy * dy + dz * dz);
+        const T muc = den * (qa * sspd + qb * mag_vel);
+        const T bx = B[3 * a], by = B[3 * a + 1], bz = B[3 * a + 2];
+        const T mu_term = muc * sqrt(bx * bx + by * by + bz * bz);
+        sum0 += mu_term * u[a];
+        sum1 += mu_term * v[a];
+        sum2 += mu_term * w[a];
+        sum3 += mu_term;
+        mu[a] = mu_term;
+    }
+    // Riemann velocity (:261-270)
+    if (sum3 > fuzz) {
+        vsx = sum0 / sum3;
+        vsy = sum1 / sum3;
+        vsz = sum2 / sum3;
+    } else {
+        vsx = 0.0;
+        vsy = 0.0;
+        vsz = 0.0;
+    }
+    // shock detector (:287-328); pow(phi, 1.0) == phi
+    const T b = div + small;
+    const T r_face = 0.9 * (ext + small) / b;
+    const T r_min = fmin(r_face, 1.0);
+    T phi = 1.0 - fmax(0.0, r_min);
+    const T c_length = pow(vol, 1.0 / 3.0);
+    const T alpha = fmin(1.0, 20.0 * (c_length * fabs(div)) / (sspd + fuzz));
+    phi = alpha * phi;
+    // corner forces (:338-359): a = -B, tau = -p I, so a_i . tau_d = B_id p
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const T pm = phi * mu[a];
+        f[3 * a + 0] = B[3 * a + 0] * pres + pm * (vsx - u[a]);
+        f[3 * a + 1] = B[3 * a + 1] * pres + pm * (vsy - v[a]);
+        f[3 * a + 2] = B[3 * a + 2] * pres + pm * (vsz - w[a]);
+    }
+}
+
+// Node velocity + boundary + position (K/momentum.cpp:44-47, K/boundary.cpp:16-55,
+// K/geometry.cpp:19-22); f = sum of the node's corner forces in CSR order.
+template <typename T>
+FSGH_HD inline void node_update(const T* f, T mass, const T* v0, const T* x0, uint8_t mask,
+                                double alpha, double dt, T* v1, T* x1)
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        T vn = v0[d] + alpha * dt * f[d] / mass;
+        if (mask & (1u << d)) vn = 0.0;
+        v1[d] = vn;
+        x1[d] = x0[d] + alpha * dt * ((vn + v0[d]) * 0.5);
+    }
+}
+
+// Element energy update (K/energy_sgh.cpp:18-50): F . (v1+v0)/2, corner by
+// corner, then e1 = e0 - alpha dt / M * power.
+template <typename T>
+FSGH_HD inline T energy_update(const T* f, const T* u1, const T* v1, const T* w1, const T* u0,
+                               const T* v0, const T* w0, T e0, T mass, double alpha, double dt)
+{
+    T power = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        power += f[3 * a + 0] * ((u1[a] + u0[a]) * 0.5);
+        power += f[3 * a + 1] * ((v1[a] + v0[a]) * 0.5);
+        power += f[3 * a + 2] * ((w1[a] + w0[a]) * 0.5);
+    }
+    return e0 - alpha * dt / mass * power;
+}
+
+// Element state from coordinates, velocities, mass and energy: volume
+// (K/geometry.cpp:298-306), density and ideal-gas EOS (K/properties.cpp:39,99),
+// divergence (K/momentum.cpp:229-287).
+template <typename T>
+FSGH_HD inline void derived_state(const T* x, const T* y, const T* z, const T* u, const T* v,
+                                  const T* w, T mass, T sie, T gamma, double csmin, T& vol,
+                                  T& den, T& pres, T& sspd, T& div)
+{
+    T B[24];
+    vol = hex_bmatrix_volume(x, y, z, B);
+    den = mass / vol;
+    ideal_gas(gamma, csmin, den, sie, pres, sspd);
+    div = divergence(B, u, v, w, vol);
+}
+
+}  // namespace fsgh

@@ -122,7 +122,7 @@ def main():
         for p in range(8):
             for j in js(p):
                 if p < j:
-                    w(f"    const double i{d}_{p}{j} = {inner(p, j, Bv)};")
+                    w(f"    const T i{d}_{p}{j} = {inner(p, j, Bv)};")
 
     def iref(d, p, j):
         return f"i{d}_{p}{j}" if p < j else f"(-i{d}_{j}{p})"
@@ -146,8 +146,8 @@ def main():
     w("#define FSGH_HD")
     w("#endif")
     w("")
-    w("FSGH_HD inline void hex_bmatrix(const double* x, const double* y, const double* z,")
-    w("                                double* B) {")
+    w("template <typename T>")
+    w("FSGH_HD inline void hex_bmatrix(const T* x, const T* y, const T* z, T* B) {")
     w("    const double twelfth = 1. / 12.;")
     for d in range(3):
         shared_inners(w, d)
@@ -169,20 +169,21 @@ def main():
             outer = t if outer is None else f"({outer})\n        + {t}"
         return outer
 
-    w("FSGH_HD inline double hex_volume(const double* x, const double* y, const double* z) {")
+    w("template <typename T>")
+    w("FSGH_HD inline T hex_volume(const T* x, const T* y, const T* z) {")
     w("    const double twelfth = 1. / 12.;")
     shared_inners(w, 0)
     w(f"    return ({vol_expr(lambda i, j: f'y[{j}] * {iref(0, i, j)}')}) * twelfth;")
     w("}")
     w("")
     w("// B and V together: V reuses the products of B's x-component rows.")
-    w("FSGH_HD inline double hex_bmatrix_volume(const double* x, const double* y, const double* z,")
-    w("                                         double* B) {")
+    w("template <typename T>")
+    w("FSGH_HD inline T hex_bmatrix_volume(const T* x, const T* y, const T* z, T* B) {")
     w("    const double twelfth = 1. / 12.;")
     shared_inners(w, 0)
     for p in range(8):
         for j in js(p):
-            w(f"    const double t_{p}{j} = y[{j}] * {iref(0, p, j)};")
+            w(f"    const T t_{p}{j} = y[{j}] * {iref(0, p, j)};")
     for p in range(8):
         acc = None
         for j in js(p):
@@ -193,6 +194,24 @@ def main():
         for p in range(8):
             w(f"    B[{3 * p + d}] = ({brow_shared(d, p)}) * twelfth;")
     w(f"    return ({vol_expr(lambda i, j: f't_{i}{j}')}) * twelfth;")
+    w("}")
+    w("")
+    # reverse mode: B is bilinear, B(p,d) = 1/12 sum_j A_j sum_k s_pjk C_k with
+    # (A, C) = (y, z), (z, x), (x, y) for d = 0, 1, 2.
+    w("// Reverse mode of hex_bmatrix: accumulates (dB . Bb)/dx into xb, yb, zb.")
+    w("FSGH_HD inline void hex_bmatrix_vjp(const double* x, const double* y, const double* z,")
+    w("                                    const double* Bb, double* xb, double* yb, double* zb) {")
+    w("    const double twelfth = 1. / 12.;")
+    for d in range(3):
+        A, Bv = COMP[d]
+        for p in range(8):
+            w(f"    {{ const double g = Bb[{3 * p + d}] * twelfth;")
+            for j in js(p):
+                w(f"      {A}b[{j}] += g * {inner(p, j, Bv)};")
+                for k in ks(p, j):
+                    sg = S[(p, j, k)]
+                    w(f"      {Bv}b[{k}] {'+' if sg > 0 else '-'}= g * {A}[{j}];")
+            w("    }")
     w("}")
     print("\n".join(out))
 

@@ -182,7 +182,7 @@ def test_output_cadence_and_dumps(oracle, Solver, tmp_path):
     cycles (K/sgh_solve.cpp:594-628) and hold the state of that cycle."""
     from fierro_amd import outputs
 
-    spec = P.noh(6, cycles=40).with_run(graphics_dt_ival=0.004)
+    spec = P.noh(6, cycles=40).with_run(graphics_dt_ival=0.0005)
     pr = oracle.OracleProblem(spec)
     ro = pr.solve(record_dt=True)
     # output times implied by the oracle's dt sequence and the write rule

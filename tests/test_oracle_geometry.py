@@ -116,6 +116,10 @@ extern "C" void geo(const double* x, const double* y, const double* z, double* B
     V[0] = hex_volume(x, y, z);
     V[1] = hex_bmatrix_volume(x, y, z, B + 24);
 }
+extern "C" void geo_vjp(const double* x, const double* y, const double* z, const double* Bb,
+                        double* xb, double* yb, double* zb) {
+    hex_bmatrix_vjp(x, y, z, Bb, xb, yb, zb);
+}
 """
 
 
@@ -149,3 +153,29 @@ def test_generated_geometry_bitwise_equals_oracle(oracle, generated):
         assert np.array_equal(B[24:].reshape(8, 3), Bo), k
         Vo = oracle.vol_hex(x, y, z)
         assert V[0] == Vo and V[1] == Vo, k
+
+
+def test_bmatrix_vjp_dot_product(generated):
+    """<Bb, dB(x)[dx]> == <vjp(Bb), dx>.  B is a quadratic polynomial of the
+    coordinates, so the central difference gives dB[dx] exactly up to rounding."""
+    rng = np.random.default_rng(13)
+    f64 = np.ctypeslib.ndpointer(np.float64)
+    generated.geo.argtypes = [f64, f64, f64, f64, f64]
+    generated.geo_vjp.argtypes = [f64] * 7
+
+    def B_of(X):
+        B = np.zeros(48)
+        generated.geo(*(np.ascontiguousarray(X[:, d]) for d in range(3)), B, np.zeros(2))
+        return B[:24]
+
+    for _ in range(10):
+        X = random_hex(rng)
+        dX = rng.standard_normal((8, 3))
+        Bb = rng.standard_normal(24)
+        t = 1e-3
+        dB = (B_of(X + t * dX) - B_of(X - t * dX)) / (2 * t)
+        xb, yb, zb = np.zeros(8), np.zeros(8), np.zeros(8)
+        generated.geo_vjp(*(np.ascontiguousarray(X[:, d]) for d in range(3)), Bb, xb, yb, zb)
+        lhs = float(Bb @ dB)
+        rhs = float(np.stack([xb, yb, zb], 1).ravel() @ dX.ravel())
+        assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1)
