@@ -57,7 +57,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-cycles", type=int, default=100)
-    ap.add_argument("--workload", choices=["sedov", "sod-ensemble"], default="sedov")
+    ap.add_argument("--workload", choices=["sedov", "sod-ensemble", "anvil-adjoint"],
+                    default="sedov")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--loop-sample", type=int, default=8,
                     help="time the per-instance loop on every k-th instance")
@@ -73,6 +74,8 @@ def main():
 
     if args.workload == "sod-ensemble":
         return ensemble(args)
+    if args.workload == "anvil-adjoint":
+        return adjoint(args)
 
     from fierro_amd import bytemodel, problems as P
     from fierro_amd.sgh import SGHSolver
@@ -231,6 +234,58 @@ def ensemble(args):
         "loop_plain": {"value": loop, "unit": "zone-cycles/s",
                        "sample": f"every {args.loop_sample}th instance, hipGraph per cycle"},
         "batched_over_loop": value / loop,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def adjoint(args):
+    """Adjoint cost: the Taylor-anvil bar (16 x 16 x 96 hexes, 400 RK2
+    cycles).  Gradient of bar length + base spread with respect to every
+    initial density, energy, gamma and velocity = one checkpointed forward
+    (every 20 cycles) + one reverse pass; compared with one plain forward
+    (fused loop, hipGraph per cycle) of the same cycles."""
+    from fierro_amd import problems as P
+    from fierro_amd.sgh import SGHSolver
+
+    cycles, every = 400, 20
+    spec = P.anvil(16, 16, 96, cycles=cycles)
+    plain = []
+    for _ in range(3):
+        g = SGHSolver(spec, renumber=True)
+        g.run_begin()
+        t = time.perf_counter()
+        g.run(cycles, use_graph=True)
+        plain.append(time.perf_counter() - t)
+        E = g.E
+        del g
+    t_plain = min(plain)
+    fw, bw = [], []
+    for _ in range(3):
+        g = SGHSolver(spec, renumber=True)
+        seed = P.anvil_objective(spec, g.coords0)
+        t = time.perf_counter()
+        g.forward(cycles, checkpoint_every=every)
+        t1 = time.perf_counter()
+        grad = g.backward(seed_x=seed)
+        t2 = time.perf_counter()
+        fw.append(t1 - t)
+        bw.append(t2 - t1)
+        del g
+    t_fw, t_bw = min(fw), min(bw)
+    ratio = (t_fw + t_bw) / t_plain
+    log(f"plain {t_plain:.3f} s, checkpointed forward {t_fw:.3f} s, reverse {t_bw:.3f} s")
+    out = {
+        "metric": "adjoint gradient time / plain forward time (anvil 16x16x96, 400 cycles)",
+        "value": ratio, "unit": "x", "n_gpus": 1, "steps": cycles, "warmup": 0,
+        "ms_per_step": 1e3 * (t_fw + t_bw) / cycles, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (generated anvil bar and initial state)",
+        "config": {"workload": "anvil16x16x96-adjoint", "elements": E, "cycles": cycles,
+                   "checkpoint_every": every},
+        "plain_forward_s": t_plain, "checkpointed_forward_s": t_fw, "reverse_s": t_bw,
+        "plain_zone_cycles_per_s": E * cycles / t_plain,
+        "grad_norm_den0": float((grad["den0"] ** 2).sum() ** 0.5),
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

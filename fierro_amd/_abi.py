@@ -31,6 +31,7 @@ EXPORTS = [
     "fsgh_update_position_sgh", "fsgh_get_vol", "fsgh_update_state", "fsgh_energy",
     "fsgh_run_begin", "fsgh_run", "fsgh_run_status", "fsgh_profile_enable",
     "fsgh_profile_read", "fsgh_check", "fsgh_stream",
+    "fsgh_forward", "fsgh_dt_history", "fsgh_backward", "fsgh_tangent",
 ]
 
 
@@ -114,6 +115,10 @@ def load() -> C.CDLL:
         "fsgh_profile_enable": [vp, i32],
         "fsgh_profile_read": [vp, i32, C.c_char_p, C.c_size_t, vp, vp, C.POINTER(i32)],
         "fsgh_check": [vp],
+        "fsgh_forward": [vp, vp, i64, i64, vp],
+        "fsgh_dt_history": [vp, vp, i64],
+        "fsgh_backward": [vp, vp, vp, vp, vp, vp, vp, vp],
+        "fsgh_tangent": [vp, vp, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -10,69 +10,10 @@
 #include <vector>
 
 #define FSGH_HD __host__ __device__
-#include "fsgh_internal.h"
-#include "fsgh_mesh.h"
+#include "fsgh_ctx.h"
 #include "hex_geometry.inc"
 
 using namespace fsgh;
-
-struct fsgh_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    Layout L;
-    int64_t N = 0, E = 0;
-    int renumber = 0;
-    std::vector<int64_t> conn;              // caller numbering, E x 8
-    std::vector<double> coords;             // caller numbering, N x 3 (initial)
-    std::vector<int32_t> ne_of_oe, nn_of_on, oe_of_ne, on_of_nn;
-    std::vector<int64_t> cstart, corners;   // caller-numbering corner CSR
-    std::vector<int32_t> eblk_begin, nblk_begin;
-    // device mesh
-    int32_t *d_conn = nullptr, *d_cin_start = nullptr, *d_cin = nullptr;
-    BlockItem *d_eblk = nullptr, *d_nblk = nullptr;
-    int32_t n_eblk = 0, n_nblk = 0;
-    // device state: node[phys bin][x,y,z,u,v,w], sie[phys bin]
-    double* d_node[2][6] = {};
-    double* d_sie[2] = {};
-    double *d_nmass = nullptr, *d_den = nullptr, *d_pres = nullptr, *d_sspd = nullptr;
-    double *d_vol = nullptr, *d_emass = nullptr, *d_gamma = nullptr;
-    // two divergence buffers: stage s's force reads d_div[s] and the fused
-    // energy kernel of stage s writes d_div[1-s]; operators use d_div[divcur]
-    double* d_div[2] = {nullptr, nullptr};
-    int divcur = 0;
-    double2 *d_ext_lo = nullptr, *d_ext_hi = nullptr;
-    double *d_csmin = nullptr, *d_F = nullptr;
-    int32_t* d_mat = nullptr;
-    uint8_t* d_bcmask = nullptr;
-    Material* d_mats = nullptr;
-    int32_t nmat = 0;
-    InstCtl* d_ctl = nullptr;
-    double* d_partial = nullptr;
-    double* d_scalar = nullptr;             // 2 doubles
-    double* d_scratch = nullptr;            // energy partial sums
-    int32_t nscratch = 1024;
-    int32_t* d_flag = nullptr;
-    int32_t* d_bcnodes = nullptr;
-    uint8_t* d_bcmasks = nullptr;
-    int32_t n_bcnodes = 0;
-    std::vector<uint8_t> bcmask;            // caller numbering
-    std::vector<std::vector<std::vector<int64_t>>> bcsets;  // [inst][set] caller node ids
-    std::vector<int64_t> eoff32, noff32;
-    int cur = 1;                            // physical bin holding reference bin 1
-    bool run_active = false;
-    bool host_only = false;                 // device < 0: host-side maps only
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    // profiling
-    bool prof = false;
-    struct Rec {
-        std::string name;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> recs;
-    std::map<std::string, std::pair<double, int64_t>> prof_acc;
-    std::vector<std::string> prof_order;
-};
 
 namespace fsgh {
 void set_error(fsgh_ctx* ctx, const std::string& msg)
@@ -81,40 +22,8 @@ void set_error(fsgh_ctx* ctx, const std::string& msg)
 }
 }  // namespace fsgh
 
-#define CHECK_CTX(c)                                   \
-    do {                                               \
-        if ((c) == nullptr) return FSGH_E_INVALID;     \
-    } while (0)
 
-#define HIPCHK(ctx, expr)                                                                 \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess) {                                                           \
-            set_error(ctx, std::string(#expr) + ": " + hipGetErrorString(e_));            \
-            return FSGH_E_HIP;                                                            \
-        }                                                                                 \
-    } while (0)
-
-namespace {
-
-template <typename T>
-int dalloc(fsgh_ctx* c, T** p, size_t n)
-{
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
-    if (e != hipSuccess) {
-        set_error(c, std::string("hipMalloc: ") + hipGetErrorString(e));
-        return FSGH_E_NOMEM;
-    }
-    e = hipMemsetAsync(*p, 0, n * sizeof(T), c->stream);
-    if (e != hipSuccess) {
-        set_error(c, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
-        return FSGH_E_HIP;
-    }
-    return FSGH_OK;
-}
-
-DState make_state(fsgh_ctx* c, int b0, int b1)
+DState fsgh::make_state(fsgh_ctx* c, int b0, int b1)
 {
     DState s;
     s.x0 = c->d_node[b0][0]; s.y0 = c->d_node[b0][1]; s.z0 = c->d_node[b0][2];
@@ -133,9 +42,8 @@ DState make_state(fsgh_ctx* c, int b0, int b1)
     return s;
 }
 
-DState ref_state(fsgh_ctx* c) { return make_state(c, 1 - c->cur, c->cur); }
 
-DMesh make_mesh(fsgh_ctx* c)
+DMesh fsgh::make_mesh(fsgh_ctx* c)
 {
     DMesh m;
     m.N = static_cast<int32_t>(c->N);
@@ -148,6 +56,29 @@ DMesh make_mesh(fsgh_ctx* c)
     m.n_eblk = c->n_eblk;
     m.n_nblk = c->n_nblk;
     return m;
+}
+
+
+namespace {
+
+DState ref_state(fsgh_ctx* c) { return make_state(c, 1 - c->cur, c->cur); }
+
+
+template <typename T>
+int dalloc(fsgh_ctx* c, T** p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(c, std::string("hipMalloc: ") + hipGetErrorString(e));
+        return FSGH_E_NOMEM;
+    }
+    e = hipMemsetAsync(*p, 0, n * sizeof(T), c->stream);
+    if (e != hipSuccess) {
+        set_error(c, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+        return FSGH_E_HIP;
+    }
+    return FSGH_OK;
 }
 
 // Launch `fn` with optional per-kernel event timing on the context stream.
@@ -458,6 +389,7 @@ int fsgh_destroy(fsgh_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     resolve_profile(c);
     drop_graphs(c);
+    free_adjoint(c);
     void* ptrs[] = {c->d_conn, c->d_cin_start, c->d_cin, c->d_eblk, c->d_nblk, c->d_nmass,
                     c->d_den, c->d_pres, c->d_sspd, c->d_vol, c->d_div[0], c->d_div[1], c->d_ext_lo, c->d_ext_hi, c->d_emass, c->d_gamma,
                     c->d_csmin, c->d_F, c->d_mat, c->d_bcmask, c->d_mats, c->d_ctl, c->d_partial,
@@ -1005,8 +937,10 @@ int fsgh_run_begin(fsgh_ctx* c, const fsgh_run_params* p)
     OP_EPILOGUE(c);
 }
 
+}  // extern "C"
+
 // one cycle: old = c->cur holds t_n; results go to the other physical bin.
-static void issue_cycle(fsgh_ctx* c, int old)
+void fsgh::issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record)
 {
     const int nw = 1 - old;
     DMesh m = make_mesh(c);
@@ -1018,7 +952,9 @@ static void issue_cycle(fsgh_ctx* c, int old)
         sd[k].div_out = c->d_div[1 - k];
     }
     st0.div = c->d_div[0];
-    timed(c, "cycle_dt", [&] { launch_cycle_dt(c->d_ctl, c->L.ninst, c->d_partial, c->stream); });
+    timed(c, "cycle_dt", [&] {
+        launch_cycle_dt(c->d_ctl, c->L.ninst, c->d_partial, dt_replay, dt_record, c->stream);
+    });
     for (int stage = 0; stage < 2; ++stage) {
         const double alpha = 1.0 / (2.0 - static_cast<double>(stage));   // K/sgh_solve.cpp:353
         const InstCtl* ctl = c->d_ctl;
@@ -1035,6 +971,8 @@ static void issue_cycle(fsgh_ctx* c, int old)
     }
     timed(c, "time_advance", [&] { launch_time_advance(c->d_ctl, c->L.ninst, nw, c->stream); });
 }
+
+extern "C" {
 
 int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t flags)
 {
@@ -1053,14 +991,14 @@ int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t flags)
             if (!c->graph[old]) {
                 hipGraph_t g;
                 HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                issue_cycle(c, old);
+                issue_cycle(c, old, nullptr, nullptr);
                 HIPCHK(c, hipStreamEndCapture(c->stream, &g));
                 HIPCHK(c, hipGraphInstantiate(&c->graph[old], g, nullptr, nullptr, 0));
                 HIPCHK(c, hipGraphDestroy(g));
             }
             HIPCHK(c, hipGraphLaunch(c->graph[old], c->stream));
         } else {
-            issue_cycle(c, old);
+            issue_cycle(c, old, nullptr, nullptr);
         }
         c->cur = 1 - old;
         if ((k + 1) % poll == 0 || k + 1 == ncycles) {

@@ -1,0 +1,95 @@
+// fsgh_ctx.h -- the context behind the opaque fsgh_ctx handle (library-internal).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fsgh_internal.h"
+#include "fsgh_mesh.h"
+
+struct AdjointData;  // fsgh_adjoint.hip
+
+struct fsgh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    fsgh::Layout L;
+    int64_t N = 0, E = 0;
+    int renumber = 0;
+    std::vector<int64_t> conn;              // caller numbering, E x 8
+    std::vector<double> coords;             // caller numbering, N x 3 (initial)
+    std::vector<int32_t> ne_of_oe, nn_of_on, oe_of_ne, on_of_nn;
+    std::vector<int64_t> cstart, corners;   // caller-numbering corner CSR
+    std::vector<int32_t> eblk_begin, nblk_begin;
+    // device mesh
+    int32_t *d_conn = nullptr, *d_cin_start = nullptr, *d_cin = nullptr;
+    fsgh::BlockItem *d_eblk = nullptr, *d_nblk = nullptr;
+    int32_t n_eblk = 0, n_nblk = 0;
+    // device state: node[phys bin][x,y,z,u,v,w], sie[phys bin]
+    double* d_node[2][6] = {};
+    double* d_sie[2] = {};
+    double *d_nmass = nullptr, *d_den = nullptr, *d_pres = nullptr, *d_sspd = nullptr;
+    double *d_vol = nullptr, *d_emass = nullptr, *d_gamma = nullptr;
+    // two divergence buffers: stage s's force reads d_div[s] and the fused
+    // energy kernel of stage s writes d_div[1-s]; operators use d_div[divcur]
+    double* d_div[2] = {nullptr, nullptr};
+    int divcur = 0;
+    double2 *d_ext_lo = nullptr, *d_ext_hi = nullptr;
+    double *d_csmin = nullptr, *d_F = nullptr;
+    int32_t* d_mat = nullptr;
+    uint8_t* d_bcmask = nullptr;
+    fsgh::Material* d_mats = nullptr;
+    int32_t nmat = 0;
+    fsgh::InstCtl* d_ctl = nullptr;
+    double* d_partial = nullptr;
+    double* d_scalar = nullptr;             // 2 doubles
+    double* d_scratch = nullptr;            // energy partial sums
+    int32_t nscratch = 1024;
+    int32_t* d_flag = nullptr;
+    int32_t* d_bcnodes = nullptr;
+    uint8_t* d_bcmasks = nullptr;
+    int32_t n_bcnodes = 0;
+    std::vector<uint8_t> bcmask;            // caller numbering
+    std::vector<std::vector<std::vector<int64_t>>> bcsets;  // [inst][set] caller node ids
+    std::vector<int64_t> eoff32, noff32;
+    int cur = 1;                            // physical bin holding reference bin 1
+    bool run_active = false;
+    bool host_only = false;                 // device < 0: host-side maps only
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // profiling
+    bool prof = false;
+    struct Rec {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::map<std::string, std::pair<double, int64_t>> prof_acc;
+    std::vector<std::string> prof_order;
+    // discrete adjoint / tangent state (allocated by fsgh_forward)
+    AdjointData* adj = nullptr;
+};
+
+
+namespace fsgh {
+// shared by the ABI translation units
+DState make_state(fsgh_ctx* c, int b0, int b1);
+DMesh make_mesh(fsgh_ctx* c);
+void issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record);
+void free_adjoint(fsgh_ctx* c);
+}  // namespace fsgh
+
+#define CHECK_CTX(c)                                   \
+    do {                                               \
+        if ((c) == nullptr) return FSGH_E_INVALID;     \
+    } while (0)
+
+#define HIPCHK(ctx, expr)                                                                 \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            set_error(ctx, std::string(#expr) + ": " + hipGetErrorString(e_));            \
+            return FSGH_E_HIP;                                                            \
+        }                                                                                 \
+    } while (0)

@@ -95,7 +95,9 @@ void launch_dt_partial(const DMesh&, const DState&, const InstCtl*, double* part
                        hipStream_t);
 void launch_reduce_min(const double* partial, int32_t n, double* out, hipStream_t);
 // fused cycle kernels
-void launch_cycle_dt(InstCtl* ctl, int32_t ninst, const double* partial, hipStream_t);
+// dt_replay / dt_record (layout [cycle][instance]) may be null
+void launch_cycle_dt(InstCtl* ctl, int32_t ninst, const double* partial, const double* dt_replay,
+                     double* dt_record, hipStream_t);
 void launch_vel_bc_pos(const DMesh&, const DState&, const InstCtl*, double alpha, hipStream_t);
 void launch_energy_state_div(const DMesh&, const DState&, const InstCtl*, double alpha,
                              double* dt_partial, hipStream_t);

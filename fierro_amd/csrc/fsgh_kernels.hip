@@ -420,10 +420,21 @@ __global__ __launch_bounds__(1024) void k_reduce_min(const double* partial, int3
 // --------------------------------------------------------------------------
 
 // start of a cycle: get_timestep's host part (K/time_integration.cpp:69-150)
-__global__ __launch_bounds__(1024) void k_cycle_dt(InstCtl* ctl, const double* partial)
+__global__ __launch_bounds__(1024) void k_cycle_dt(InstCtl* ctl, int32_t ninst,
+                                                    const double* partial,
+                                                    const double* dt_replay, double* dt_record)
 {
     InstCtl& c = ctl[blockIdx.x];
     if (c.stopped) return;
+    const size_t slot = static_cast<size_t>(c.cycle) * ninst + blockIdx.x;
+    if (dt_replay != nullptr) {
+        // frozen time steps (derivative checks): the recorded dt of this cycle
+        if (threadIdx.x == 0) {
+            c.dt = dt_replay[slot];
+            if (dt_record != nullptr) dt_record[slot] = c.dt;
+        }
+        return;
+    }
     double v0 = DBL_MAX, v1 = DBL_MAX, v2 = DBL_MAX, v3 = DBL_MAX;
     int i = c.eblk_begin + static_cast<int>(threadIdx.x);
     for (; i + 3 * 1024 < c.eblk_end; i += 4 * 1024) {
@@ -446,6 +457,7 @@ __global__ __launch_bounds__(1024) void k_cycle_dt(InstCtl* ctl, const double* p
         dt = fmin(dt, (c.graphics_time - c.time) + c.fuzz);
         dt = fmin(dt, c.time_final - c.time);
         c.dt = dt;
+        if (dt_record != nullptr) dt_record[slot] = dt;
     }
 }
 
@@ -688,9 +700,11 @@ void launch_reduce_min(const double* partial, int32_t n, double* out, hipStream_
 {
     hipLaunchKernelGGL(k_reduce_min, dim3(1), dim3(1024), 0, st, partial, n, out);
 }
-void launch_cycle_dt(InstCtl* ctl, int32_t ninst, const double* partial, hipStream_t st)
+void launch_cycle_dt(InstCtl* ctl, int32_t ninst, const double* partial, const double* dt_replay,
+                     double* dt_record, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cycle_dt, dim3(ninst), dim3(1024), 0, st, ctl, partial);
+    hipLaunchKernelGGL(k_cycle_dt, dim3(ninst), dim3(1024), 0, st, ctl, ninst, partial, dt_replay,
+                       dt_record);
 }
 void launch_vel_bc_pos(const DMesh& m, const DState& s, const InstCtl* ctl, double alpha,
                        hipStream_t st)

@@ -239,3 +239,23 @@ def anvil(nx=16, ny=16, nz=96, cycles=400, speed=0.2, dt_cfl=0.5):
                     dt_cfl=dt_cfl, graphics_dt_ival=1.0e16, graphics_cyc_ival=1000000000,
                     cycle_stop=cycles)
     return spec
+
+
+def anvil_objective(spec, coords0):
+    """Seed of the anvil objective J = final bar length + base lateral spread,
+    a linear functional of the final node coordinates: the mean z of the top
+    face (z0 = lz) plus the mean x of the base nodes on the outer face
+    x0 = lx and the mean y of the base nodes on y0 = ly (the bar is a
+    quarter, with symmetry planes x = 0 and y = 0).  Returns seed [N][3]."""
+    m = spec.mesh
+    x0 = np.asarray(coords0, np.float64)
+    tol = 1e-12
+    seed = np.zeros_like(x0)
+    top = np.abs(x0[:, 2] - m["lz"]) < tol
+    base = np.abs(x0[:, 2]) < tol
+    bx = base & (np.abs(x0[:, 0] - m["lx"]) < tol)
+    by = base & (np.abs(x0[:, 1] - m["ly"]) < tol)
+    seed[top, 2] += 1.0 / top.sum()
+    seed[bx, 0] += 1.0 / bx.sum()
+    seed[by, 1] += 1.0 / by.sum()
+    return seed

@@ -220,7 +220,7 @@ class SGHSolver:
         self.update_state(dt, rk_alpha)
 
     # -- fused cycle loop --
-    def run_begin(self):
+    def _run_params(self):
         ps = []
         for s in self.specs:
             r = s.run
@@ -228,8 +228,10 @@ class SGHSolver:
                                      r["dt_cfl"], r["graphics_dt_ival"],
                                      int(r["graphics_cyc_ival"]), int(r["cycle_stop"]),
                                      r["fuzz"], r["small"]))
-        arr = (_abi.RunParams * len(ps))(*ps)
-        check(self.L.fsgh_run_begin(self.ctx, arr), self.ctx, "run_begin")
+        return (_abi.RunParams * len(ps))(*ps)
+
+    def run_begin(self):
+        check(self.L.fsgh_run_begin(self.ctx, self._run_params()), self.ctx, "run_begin")
 
     def run(self, ncycles, use_graph=True, stop_on_output=False):
         flags = (_abi.RUN_GRAPH if use_graph else 0) | \
@@ -266,6 +268,52 @@ class SGHSolver:
         ie1 = [self.energy(i) for i in range(self.ninst)]
         return [dict(ie0=a[0], ke0=a[1], te0=a[0] + a[1], ie1=b[0], ke1=b[1], te1=b[0] + b[1],
                      **self.status(i)) for i, (a, b) in enumerate(zip(ie0, ie1))]
+
+    # -- discrete adjoint / tangent-linear model (frozen time steps) --
+    def forward(self, ncycles, checkpoint_every=20, dt_replay=None):
+        """Runs ``ncycles`` cycles of every instance from the set-up state,
+        recording dt and checkpointing the state every ``checkpoint_every``
+        cycles; ``dt_replay`` ([ncycles][instances]) replays given steps."""
+        rep = None
+        if dt_replay is not None:
+            rep = np.ascontiguousarray(dt_replay, np.float64).reshape(-1)
+            if rep.size != ncycles * self.ninst:
+                raise ValueError("dt_replay must hold ncycles x instances steps")
+        check(self.L.fsgh_forward(self.ctx, self._run_params(), int(ncycles),
+                                  int(checkpoint_every), None if rep is None else _p(rep)),
+              self.ctx, "forward")
+        self._adj_ncyc = int(ncycles)
+
+    def dt_history(self):
+        out = np.zeros((self._adj_ncyc, self.ninst))
+        check(self.L.fsgh_dt_history(self.ctx, _p(out), out.size), self.ctx, "dt_history")
+        return out
+
+    def backward(self, seed_x=None, seed_v=None, seed_e=None):
+        """Gradients of <seed, final state> with respect to the initial
+        density, specific internal energy, gamma and node velocity."""
+        def arr(a, shape):
+            return None if a is None else np.ascontiguousarray(a, np.float64).reshape(shape)
+        sx, sv = arr(seed_x, (self.N, 3)), arr(seed_v, (self.N, 3))
+        se = arr(seed_e, (self.E,))
+        g = dict(den0=np.zeros(self.E), sie0=np.zeros(self.E), gamma=np.zeros(self.E),
+                 vel0=np.zeros((self.N, 3)))
+        ptr = [None if a is None else _p(a) for a in (sx, sv, se)]
+        check(self.L.fsgh_backward(self.ctx, *ptr, _p(g["den0"]), _p(g["sie0"]),
+                                   _p(g["gamma"]), _p(g["vel0"])), self.ctx, "backward")
+        return g
+
+    def tangent(self, d_den0=None, d_sie0=None, d_gamma=None, d_vel0=None):
+        """Directional derivative of the final (coords, vel, sie) along the
+        given perturbation of the initial state."""
+        def arr(a, shape):
+            return None if a is None else np.ascontiguousarray(a, np.float64).reshape(shape)
+        ins = [arr(d_den0, (self.E,)), arr(d_sie0, (self.E,)), arr(d_gamma, (self.E,)),
+               arr(d_vel0, (self.N, 3))]
+        t = dict(x=np.zeros((self.N, 3)), v=np.zeros((self.N, 3)), e=np.zeros(self.E))
+        check(self.L.fsgh_tangent(self.ctx, *[None if a is None else _p(a) for a in ins],
+                                  _p(t["x"]), _p(t["v"]), _p(t["e"])), self.ctx, "tangent")
+        return t
 
     # -- profiling --
     def profile(self, enable=True):

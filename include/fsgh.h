@@ -211,6 +211,44 @@ int fsgh_run_begin(fsgh_ctx *ctx, const fsgh_run_params *params);
 int fsgh_run(fsgh_ctx *ctx, int64_t ncycles, int32_t flags);
 int fsgh_run_status(fsgh_ctx *ctx, int32_t instance, fsgh_instance_status *out);
 
+/* ------------------------------------ discrete adjoint / tangent model --- */
+/* No counterpart in the reference (which has no derivative of the SGH cycle).
+ * The model is the fused cycle of fsgh_run with the time steps frozen to the
+ * ones the forward pass used (the CFL minimum, its growth factor and the
+ * clamps are not differentiated).  Controls: the initial element density
+ * rho0 (element mass M = rho0 V0, node masses sum M/8, initial pressure and
+ * sound speed by the EOS), the initial specific internal energy, the element
+ * gamma (statev 0) and the initial node velocity.
+ *
+ * fsgh_forward: call right after fsgh_setup_fills (or the uploads of a full
+ * initial state); runs `ncycles` cycles of every instance from time 0,
+ * recording dt per cycle and instance and storing the state (x, v, e) every
+ * `checkpoint_every` cycles.  dt_replay (may be NULL; [ncycles][instances])
+ * replays given time steps instead of computing them.  Every instance must
+ * run all ncycles (time_final and cycle_stop not reached earlier), otherwise
+ * FSGH_E_INVALID.  The state afterwards is the forward result, as after
+ * fsgh_run. */
+int fsgh_forward(fsgh_ctx *ctx, const fsgh_run_params *params, int64_t ncycles,
+                 int64_t checkpoint_every, const double *dt_replay);
+/* The recorded time steps, n = ncycles * num_instances, [cycle][instance]. */
+int fsgh_dt_history(fsgh_ctx *ctx, double *out, int64_t n);
+/* Reverse pass: given the adjoint seeds of the final state (node
+ * coordinates and velocities [N][3], element energy [E]; NULL = zero),
+ * returns the gradients of <seed, final state> with respect to rho0 [E],
+ * e0 [E], gamma [E] and the initial velocity [N][3] (any output may be
+ * NULL).  Recomputes each checkpoint segment; leaves the forward's final
+ * state in place.  Caller numbering throughout. */
+int fsgh_backward(fsgh_ctx *ctx, const double *seed_x, const double *seed_v,
+                  const double *seed_e, double *grad_den0, double *grad_sie0,
+                  double *grad_gamma, double *grad_vel0);
+/* Tangent-linear model: the directional derivative of the final state
+ * (coordinates [N][3], velocities [N][3], energy [E]) along the control
+ * perturbation (d_den0 [E], d_sie0 [E], d_gamma [E], d_vel0 [N][3]; NULL =
+ * zero).  Uses the forward's first checkpoint and time steps. */
+int fsgh_tangent(fsgh_ctx *ctx, const double *d_den0, const double *d_sie0,
+                 const double *d_gamma, const double *d_vel0, double *t_x, double *t_v,
+                 double *t_e);
+
 /* ------------------------------------------------------- diagnostics --- */
 /* Per-kernel timing with HIP events on the context's stream.  enable != 0
  * starts recording (graph replays are not split per kernel). */
