@@ -48,6 +48,21 @@ def cpu_baseline(n, cycles, threads):
                       f"{r['seconds']:.2f} s, oracle/liboracle_sgh_fast.so (gcc -O3, OpenMP)"}
 
 
+def measured_traffic(workload, E, N, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (scaled by a plain copy's counters) for this workload, from
+    profiles/<workload>_traffic.json; None when there is no such record."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except OSError:
+        return None
+    if rec.get("elements") != E or rec.get("nodes") != N or kernel not in rec["kernels"]:
+        return None
+    return rec["kernels"][kernel]["traffic_bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,7 +149,8 @@ def main():
         "config": {"workload": f"sedov{n}", "elements": E, "nodes": N, "rk_stages": 2,
                    "renumber": "morton", "cfl": 0.5, "q2": 1.333},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": d["GBs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"sedov{n}", E, N, dom)},
         "kernels": kernels,
         "cycle_alg_GBs": cyc_bytes * args.steps / wall / 1e9,
         "value_graph": E * args.steps / graph_s,

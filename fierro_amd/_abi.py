@@ -117,7 +117,7 @@ def load() -> C.CDLL:
         "fsgh_check": [vp],
         "fsgh_forward": [vp, vp, i64, i64, vp],
         "fsgh_dt_history": [vp, vp, i64],
-        "fsgh_backward": [vp, vp, vp, vp, vp, vp, vp, vp],
+        "fsgh_backward": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "fsgh_tangent": [vp, vp, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():

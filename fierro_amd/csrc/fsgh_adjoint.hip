@@ -649,6 +649,36 @@ __global__ __launch_bounds__(kB) void k_adj_gather(DMesh m, const double* cb, do
     va2[n] += s[5];
 }
 
+// Final element density den = M / V(x) as an objective term: its seed sd
+// gives M-bar += sd / V and, per corner, x-bar = -sd den / V dV/dx_a with
+// dV/dx_a = B_a (velocity part zero).
+__global__ __launch_bounds__(kB) void k_adj_seed_den(DMesh m, DState s, const double* sd,
+                                                     double* Mb, double* cb)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    int32_t nd[8];
+    ld_conn(m.conn, e, nd);
+    double x[8], y[8], z[8], B[24];
+    g3(s.x1, s.y1, s.z1, nd, x, y, z);
+    const double vol = hex_bmatrix_volume(x, y, z, B);
+    const double M = s.emass[e];
+    const double g = sd[e];
+    Mb[e] += g / vol;
+    const double k = -g * (M / vol) / vol;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        double* p = cb + 6 * (8 * static_cast<size_t>(e) + a);
+        p[0] = k * B[3 * a];
+        p[1] = k * B[3 * a + 1];
+        p[2] = k * B[3 * a + 2];
+        p[3] = 0.0;
+        p[4] = 0.0;
+        p[5] = 0.0;
+    }
+}
+
 // d/d rho0 = rhob + V0 (Mb + sum_{nodes} mb / 8); initial-velocity adjoint masked
 __global__ __launch_bounds__(kB) void k_adj_finish_elem(DMesh m, AdjView A, const double* vol0,
                                                         double* grad_rho)
@@ -664,26 +694,34 @@ __global__ __launch_bounds__(kB) void k_adj_finish_elem(DMesh m, AdjView A, cons
     grad_rho[e] = A.rhob[e] + vol0[e] * (A.Mb[e] + acc);
 }
 
-__global__ void k_fill(double* p, size_t n, double v)
+
+// Up to kMaxSeg (dst, src, n) copies in one launch (src == nullptr: zero fill).
+constexpr int kMaxSeg = 16;
+struct SegList {
+    double* dst[kMaxSeg];
+    const double* src[kMaxSeg];
+    long long start[kMaxSeg + 1];   // prefix offsets of the segments
+    int count;
+};
+
+__global__ __launch_bounds__(kB) void k_segcopy(SegList L)
 {
-    const size_t i = static_cast<size_t>(blockIdx.x) * kB + threadIdx.x;
-    if (i < n) p[i] = v;
+    const long long i = static_cast<long long>(blockIdx.x) * kB + threadIdx.x;
+    if (i >= L.start[L.count]) return;
+    int k = 0;
+    while (k + 1 < L.count && i >= L.start[k + 1]) ++k;
+    const long long j = i - L.start[k];
+    L.dst[k][j] = L.src[k] != nullptr ? L.src[k][j] : 0.0;
 }
 
-__global__ void k_copy(double* dst, const double* src, size_t n)
-{
-    const size_t i = static_cast<size_t>(blockIdx.x) * kB + threadIdx.x;
-    if (i < n) dst[i] = src[i];
-}
-
-
-// cycle counter of every instance (selects the replayed dt); clears `stopped`
-__global__ void k_set_cycle(InstCtl* ctl, int32_t ninst, long long cycle)
+// cycle counter and replayed dt of every instance; clears `stopped`
+__global__ void k_set_cycle_dt(InstCtl* ctl, int32_t ninst, long long cycle, const double* dts)
 {
     const int i = blockIdx.x * kB + static_cast<int>(threadIdx.x);
     if (i >= ninst) return;
     ctl[i].cycle = cycle;
     ctl[i].stopped = 0;
+    ctl[i].dt = dts[static_cast<size_t>(cycle) * ninst + i];
 }
 
 // element state (vol, den, p, c, div) from (x, v, e) at a cycle start
@@ -729,14 +767,17 @@ struct AdjointData {
     std::vector<double*> chk;             // states every `every` cycles (x,y,z,u,v,w | e)
     std::vector<double*> seg;             // recomputed states of one segment (every + 1)
     double *den0 = nullptr, *pres0 = nullptr, *sspd0 = nullptr, *vol0 = nullptr;
-    double* fin = nullptr;                // final forward state, restored after a backward pass
+    double* fin = nullptr;                // final forward state, restored after a reverse pass
     std::vector<double*> bufs;            // everything allocated, released together
     bool done = false;
     // reverse pass
     bool rev_ready = false;
-    AdjView A{};
-    double *Fa = nullptr, *vol_in = nullptr, *div_in = nullptr, *esa = nullptr, *grho = nullptr;
-    double *xab[3] = {}, *vab[3] = {};
+    double* sa[5] = {};                   // stage-0 input element state: den, pres, sspd, vol, div
+    double* sb[5] = {};                   // stage-1 input element state
+    double* Fa = nullptr;                 // stage-0 corner forces
+    double* set[3] = {};                  // adjoint state sets (x,y,z,u,v,w | e), rotating roles
+    double* grho = nullptr;
+    AdjView A{};                          // shared per-run accumulators and scratch
     // tangent-linear model
     bool tlm_ready = false;
     double* tn[2][6][2] = {};             // node bins x (value, tangent)
@@ -749,6 +790,12 @@ struct AdjointData {
 };
 
 namespace {
+
+struct Seg {
+    double* dst;
+    const double* src;
+    size_t n;
+};
 
 int adj_alloc(fsgh_ctx* c, double** p, size_t n)
 {
@@ -770,27 +817,47 @@ int adj_alloc_t(fsgh_ctx* c, T** p, size_t n)
     return rc;
 }
 
-void fill(fsgh_ctx* c, double* p, size_t n, double v)
+// copies / zero fills, batched kMaxSeg per launch
+void segcopy(fsgh_ctx* c, const std::vector<Seg>& v)
 {
-    if (n) hipLaunchKernelGGL(k_fill, dim3(nb_for(n)), dim3(kB), 0, c->stream, p, n, v);
+    for (size_t b = 0; b < v.size(); b += kMaxSeg) {
+        SegList L{};
+        long long off = 0;
+        int k = 0;
+        for (size_t i = b; i < v.size() && k < kMaxSeg; ++i) {
+            if (v[i].n == 0) continue;
+            L.dst[k] = v[i].dst;
+            L.src[k] = v[i].src;
+            L.start[k] = off;
+            off += static_cast<long long>(v[i].n);
+            ++k;
+        }
+        if (k == 0) continue;
+        L.start[k] = off;
+        L.count = k;
+        hipLaunchKernelGGL(k_segcopy, dim3(nb_for(static_cast<size_t>(off))), dim3(kB), 0,
+                           c->stream, L);
+    }
 }
-void copy(fsgh_ctx* c, double* dst, const double* src, size_t n)
-{
-    if (n) hipLaunchKernelGGL(k_copy, dim3(nb_for(n)), dim3(kB), 0, c->stream, dst, src, n);
-}
+void fill0(fsgh_ctx* c, double* p, size_t n) { segcopy(c, {Seg{p, nullptr, n}}); }
+void copy(fsgh_ctx* c, double* dst, const double* src, size_t n) { segcopy(c, {Seg{dst, src, n}}); }
 
-// state layout of checkpoint/segment buffers: x, y, z, u, v, w (N each), e (E)
-void save_state(fsgh_ctx* c, double* dst, int bin)
+// state layout of checkpoint/segment buffers and adjoint sets: x, y, z, u, v, w (N each), e (E)
+std::vector<Seg> state_to_bin(fsgh_ctx* c, const double* src, int bin)
 {
     const size_t N = static_cast<size_t>(c->N), E = static_cast<size_t>(c->E);
-    for (int k = 0; k < 6; ++k) copy(c, dst + k * N, c->d_node[bin][k], N);
-    copy(c, dst + 6 * N, c->d_sie[bin], E);
+    std::vector<Seg> v;
+    for (int k = 0; k < 6; ++k) v.push_back(Seg{c->d_node[bin][k], src + k * N, N});
+    v.push_back(Seg{c->d_sie[bin], src + 6 * N, E});
+    return v;
 }
-void load_state(fsgh_ctx* c, const double* src, int bin)
+std::vector<Seg> bin_to_state(fsgh_ctx* c, double* dst, int bin)
 {
     const size_t N = static_cast<size_t>(c->N), E = static_cast<size_t>(c->E);
-    for (int k = 0; k < 6; ++k) copy(c, c->d_node[bin][k], src + k * N, N);
-    copy(c, c->d_sie[bin], src + 6 * N, E);
+    std::vector<Seg> v;
+    for (int k = 0; k < 6; ++k) v.push_back(Seg{dst + k * N, c->d_node[bin][k], N});
+    v.push_back(Seg{dst + 6 * N, c->d_sie[bin], E});
+    return v;
 }
 
 // every mutable state array of the context (both bins + element state)
@@ -807,12 +874,6 @@ std::vector<std::pair<double*, size_t>> state_arrays(fsgh_ctx* c)
     return v;
 }
 size_t state_words(fsgh_ctx* c) { return 12 * static_cast<size_t>(c->N) + 8 * static_cast<size_t>(c->E); }
-
-void set_cycle(fsgh_ctx* c, int64_t cycle)
-{
-    hipLaunchKernelGGL(k_set_cycle, dim3(nb_for(static_cast<size_t>(c->L.ninst))), dim3(kB), 0,
-                       c->stream, c->d_ctl, c->L.ninst, static_cast<long long>(cycle));
-}
 
 int init_ctl(fsgh_ctx* c, const fsgh_run_params* p)
 {
@@ -885,81 +946,122 @@ int down_node(fsgh_ctx* c, double* dst, const double* src, int comp, std::vector
     return FSGH_OK;
 }
 
-// Recompute cycle `cyc` from `state` (loaded into both bins) up to the
-// stage-1 forces.  Leaves: Fa = stage-0 forces, bin 0 = S_c, bin 1 = stage-0
-// output, element arrays = stage-1 input state, F = stage-1 forces,
-// vol_in/div_in[s*E] = stage-s input volume and divergence.
-void recompute_cycle(fsgh_ctx* c, int64_t cyc, const double* state)
+// state view with the element state taken from `set` (den, pres, sspd, vol, div)
+DState elem_view(fsgh_ctx* c, int b0, int b1, double* const* set)
+{
+    DState s = make_state(c, b0, b1);
+    s.den = set[0];
+    s.pres = set[1];
+    s.sspd = set[2];
+    s.vol = set[3];
+    s.div = set[4];
+    s.div_out = set[4];
+    return s;
+}
+
+// the element state the stage-0 force of cycle `cyc` reads: the set-up state
+// (rho0, EOS pressure and sound speed, V0) for cycle 0, else set A
+void stage0_set(fsgh_ctx* c, int64_t cyc, double* out[5])
 {
     AdjointData& D = *c->adj;
-    const size_t E = D.E;
-    DMesh m = make_mesh(c);
-    load_state(c, state, 0);
-    load_state(c, state, 1);
-    DState s = make_state(c, 0, 1);
-    s.div = c->d_div[0];
     if (cyc == 0) {
-        // the set-up element state of the run
-        copy(c, c->d_den, D.den0, E);
-        copy(c, c->d_pres, D.pres0, E);
-        copy(c, c->d_sspd, D.sspd0, E);
-        copy(c, c->d_vol, D.vol0, E);
-        launch_get_divergence(m, s, c->stream);
+        out[0] = D.den0;
+        out[1] = D.pres0;
+        out[2] = D.sspd0;
+        out[3] = D.vol0;
     } else {
-        hipLaunchKernelGGL(k_derived, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s);
+        for (int k = 0; k < 4; ++k) out[k] = D.sa[k];
     }
-    copy(c, D.vol_in, c->d_vol, E);
-    copy(c, D.div_in, c->d_div[0], E);
-    set_cycle(c, cyc);
-    launch_cycle_dt(c->d_ctl, c->L.ninst, c->d_partial, D.dt, nullptr, c->stream);
-    DState s0 = make_state(c, 0, 0);
-    s0.div = c->d_div[0];
+    out[4] = D.sa[4];
+}
+
+// Recompute cycle `cyc` from `state` (loaded into both bins) up to the
+// stage-1 forces.  Leaves: bin 0 = S_c, bin 1 = stage-0 output, Fa =
+// stage-0 forces, set A (or the set-up state) = stage-0 input element
+// state, set B = stage-1 input element state, d_F = stage-1 forces.
+// setA_ready: set A already holds the state derived from `state`.
+void recompute_cycle(fsgh_ctx* c, int64_t cyc, const double* state, bool setA_ready)
+{
+    AdjointData& D = *c->adj;
+    DMesh m = make_mesh(c);
+    std::vector<Seg> ld = state_to_bin(c, state, 0);
+    for (const Seg& q : state_to_bin(c, state, 1)) ld.push_back(q);
+    segcopy(c, ld);
+    double* S0[5];
+    stage0_set(c, cyc, S0);
+    if (cyc == 0) {
+        launch_get_divergence(m, elem_view(c, 0, 1, S0), c->stream);
+    } else if (!setA_ready) {
+        hipLaunchKernelGGL(k_derived, dim3(m.n_eblk), dim3(kB), 0, c->stream, m,
+                           elem_view(c, 0, 1, D.sa));
+    }
+    hipLaunchKernelGGL(k_set_cycle_dt, dim3(nb_for(static_cast<size_t>(c->L.ninst))), dim3(kB), 0,
+                       c->stream, c->d_ctl, c->L.ninst, static_cast<long long>(cyc),
+                       static_cast<const double*>(D.dt));
+    DState s0 = elem_view(c, 0, 0, S0);
+    s0.F = D.Fa;
     launch_div_extremes(m, s0, c->d_ctl, c->stream);
     launch_force(m, s0, c->d_ctl, 0.0, 0.0, c->stream);
-    copy(c, D.Fa, c->d_F, 24 * E);
-    DState st = make_state(c, 0, 1);
-    st.div = c->d_div[0];
-    st.div_out = c->d_div[1];
+    DState st = elem_view(c, 0, 1, D.sb);    // energy/EOS of stage 0 write set B
+    st.F = D.Fa;
     launch_vel_bc_pos(m, st, c->d_ctl, 0.5, c->stream);
     launch_energy_state_div(m, st, c->d_ctl, 0.5, nullptr, c->stream);
-    copy(c, D.vol_in + E, c->d_vol, E);
-    copy(c, D.div_in + E, c->d_div[1], E);
-    DState s1 = make_state(c, 0, 1);
-    s1.div = c->d_div[1];
+    DState s1 = elem_view(c, 0, 1, D.sb);
     launch_div_extremes(m, s1, c->d_ctl, c->stream);
     launch_force(m, s1, c->d_ctl, 0.0, 0.0, c->stream);
 }
 
-// Reverse one RK stage.  s_in: stage input (x1..w1, element arrays);
-// s_out: stage output velocities in u1..w1, cycle state in bin 0;
-// F: the stage's corner forces.  Output adjoints are in A.xo/vo/eo; the
-// adjoints of the cycle state accumulate into A.x0b/v0b/e0b, those of the
-// stage input into (xdst, vdst, ein_b).
+// adjoint set k: pointers to its x,y,z / u,v,w / e parts
+struct SetPtr {
+    double *x[3], *v[3], *e;
+};
+SetPtr set_ptr(const AdjointData& D, double* base)
+{
+    SetPtr p;
+    for (int d = 0; d < 3; ++d) {
+        p.x[d] = base + d * D.N;
+        p.v[d] = base + (3 + d) * D.N;
+    }
+    p.e = base + 6 * D.N;
+    return p;
+}
+
+// Reverse one RK stage.  out: adjoints of the stage outputs; acc: adjoints
+// of the cycle state (bin 0), accumulated; in: adjoints of the stage input
+// (x1..w1 of s_in and its element energy), accumulated.
 void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, const DState& s_out,
-                   const double* F, double alpha, const double* ein, double* ein_b,
-                   double* const* xdst, double* const* vdst)
+                   const double* F, double alpha, const double* ein, const SetPtr& out,
+                   const SetPtr& acc, const SetPtr& in)
 {
     AdjointData& D = *c->adj;
-    const size_t E = D.E;
     DMesh m = make_mesh(c);
     const double* dts = D.dt + static_cast<size_t>(cyc) * D.ninst;
-    const AdjView A = D.A;
-    const double* vol_a = D.vol_in + stage * E;
-    const double* div_a = D.div_in + stage * E;
+    AdjView A = D.A;
+    for (int d = 0; d < 3; ++d) {
+        A.xo[d] = out.x[d];
+        A.vo[d] = out.v[d];
+        A.x0b[d] = acc.x[d];
+        A.v0b[d] = acc.v[d];
+    }
+    A.eo = out.e;
+    A.e0b = acc.e;
     hipLaunchKernelGGL(k_adj_energy, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_out, F, A, dts,
                        alpha);
     hipLaunchKernelGGL(k_adj_node, dim3(m.n_nblk), dim3(kB), 0, c->stream, m, s_out, F, A, dts,
                        alpha);
-    hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), dim3(kB), 0, c->stream, m, div_a,
-                       static_cast<const double*>(nullptr), D.lo, D.hi, D.alo, D.ahi,
-                       static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
+    hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
+                       static_cast<const double*>(s_in.div), static_cast<const double*>(nullptr),
+                       D.lo, D.hi, D.alo, D.ahi, static_cast<double2*>(nullptr),
+                       static_cast<double2*>(nullptr));
     hipLaunchKernelGGL(k_adj_force, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_in, A,
-                       static_cast<const InstCtl*>(c->d_ctl), vol_a, div_a);
-    hipLaunchKernelGGL(k_adj_derived, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_in, A, vol_a,
-                       div_a, ein, ein_b, (cyc == 0 && stage == 0) ? 1 : 0);
+                       static_cast<const InstCtl*>(c->d_ctl), static_cast<const double*>(s_in.vol),
+                       static_cast<const double*>(s_in.div));
+    hipLaunchKernelGGL(k_adj_derived, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_in, A,
+                       static_cast<const double*>(s_in.vol), static_cast<const double*>(s_in.div),
+                       ein, in.e, (cyc == 0 && stage == 0) ? 1 : 0);
     hipLaunchKernelGGL(k_adj_gather, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
-                       static_cast<const double*>(A.cb), xdst[0], xdst[1], xdst[2], vdst[0],
-                       vdst[1], vdst[2]);
+                       static_cast<const double*>(A.cb), in.x[0], in.x[1], in.x[2], in.v[0],
+                       in.v[1], in.v[2]);
 }
 
 int ensure_extreme_buffers(fsgh_ctx* c)
@@ -989,17 +1091,11 @@ int ensure_reverse_buffers(fsgh_ctx* c)
         rc = adj_alloc(c, &p, SW);
         D.seg.push_back(p);
     }
-    for (int d = 0; d < 3; ++d) {
-        rc |= adj_alloc(c, &D.A.xo[d], N);
-        rc |= adj_alloc(c, &D.A.vo[d], N);
-        rc |= adj_alloc(c, &D.A.x0b[d], N);
-        rc |= adj_alloc(c, &D.A.v0b[d], N);
-        rc |= adj_alloc(c, &D.xab[d], N);
-        rc |= adj_alloc(c, &D.vab[d], N);
+    for (int k = 0; k < 5; ++k) {
+        rc |= adj_alloc(c, &D.sa[k], E);
+        rc |= adj_alloc(c, &D.sb[k], E);
     }
-    rc |= adj_alloc(c, &D.A.eo, E);
-    rc |= adj_alloc(c, &D.A.e0b, E);
-    rc |= adj_alloc(c, &D.esa, E);
+    for (int k = 0; k < 3; ++k) rc |= adj_alloc(c, &D.set[k], SW);
     rc |= adj_alloc(c, &D.A.Mb, E);
     rc |= adj_alloc(c, &D.A.mb, N);
     rc |= adj_alloc(c, &D.A.gb, E);
@@ -1011,8 +1107,6 @@ int ensure_reverse_buffers(fsgh_ctx* c)
     rc |= adj_alloc(c, &D.A.eadj, 6 * E);
     rc |= adj_alloc_t(c, &D.A.argext, E);
     rc |= adj_alloc(c, &D.Fa, 3 * C);
-    rc |= adj_alloc(c, &D.vol_in, 2 * E);
-    rc |= adj_alloc(c, &D.div_in, 2 * E);
     if (rc) return FSGH_E_NOMEM;
     D.A.lo = D.lo;
     D.A.hi = D.hi;
@@ -1044,7 +1138,7 @@ int ensure_tlm_buffers(fsgh_ctx* c)
     return FSGH_OK;
 }
 
-// TLM view: node bins b0 (t_n) / b1 (stage input and output); element
+// TLM view: node bins 0 (t_n) / 1 (stage input and output); element
 // divergence input slot `din`
 TLMState tlm_state(fsgh_ctx* c, int din)
 {
@@ -1137,10 +1231,8 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
         HIPCHK(c, hipMemcpyAsync(D.dt, dt_replay, static_cast<size_t>(ncycles) * D.ninst * 8,
                                  hipMemcpyHostToDevice, c->stream));
     // the set-up element state: density rho0, EOS pressure and sound speed, volume V0
-    copy(c, D.den0, c->d_den, E);
-    copy(c, D.pres0, c->d_pres, E);
-    copy(c, D.sspd0, c->d_sspd, E);
-    copy(c, D.vol0, c->d_vol, E);
+    segcopy(c, {Seg{D.den0, c->d_den, E}, Seg{D.pres0, c->d_pres, E}, Seg{D.sspd0, c->d_sspd, E},
+                Seg{D.vol0, c->d_vol, E}});
     if ((rc = init_ctl(c, params))) return rc;
     c->divcur = 0;
     DMesh m = make_mesh(c);
@@ -1151,17 +1243,21 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
     }
     for (int64_t k = 0; k < ncycles; ++k) {
         if (k % checkpoint_every == 0)
-            save_state(c, D.chk[static_cast<size_t>(k / checkpoint_every)], c->cur);
+            segcopy(c, bin_to_state(c, D.chk[static_cast<size_t>(k / checkpoint_every)], c->cur));
         const int old = c->cur;
         issue_cycle(c, old, dt_replay ? D.dt : nullptr, dt_replay ? nullptr : D.dt);
         c->cur = 1 - old;
     }
     c->divcur = 1;
     c->run_active = true;
-    size_t off = 0;
-    for (auto& a : state_arrays(c)) {
-        copy(c, D.fin + off, a.first, a.second);
-        off += a.second;
+    {
+        std::vector<Seg> v;
+        size_t off = 0;
+        for (auto& a : state_arrays(c)) {
+            v.push_back(Seg{D.fin + off, a.first, a.second});
+            off += a.second;
+        }
+        segcopy(c, v);
     }
     std::vector<InstCtl> h(static_cast<size_t>(c->L.ninst));
     HIPCHK(c, hipMemcpyAsync(h.data(), c->d_ctl, h.size() * sizeof(InstCtl), hipMemcpyDeviceToHost,
@@ -1190,7 +1286,8 @@ int fsgh_dt_history(fsgh_ctx* c, double* out, int64_t n)
 }
 
 int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const double* seed_e,
-                  double* grad_den0, double* grad_sie0, double* grad_gamma, double* grad_vel0)
+                  const double* seed_den, double* grad_den0, double* grad_sie0,
+                  double* grad_gamma, double* grad_vel0)
 {
     if (!c) return FSGH_E_INVALID;
     if (!c->adj || !c->adj->done) {
@@ -1202,20 +1299,44 @@ int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const
     if (rc) return rc;
     const size_t N = D.N, E = D.E, SW = 6 * N + E;
     DMesh m = make_mesh(c);
-    std::vector<double> t;
-    for (int d = 0; d < 3 && !rc; ++d) {
-        rc |= up_node(c, D.A.xo[d], seed_x, d, t);
-        rc |= up_node(c, D.A.vo[d], seed_v, d, t);
+    // the recomputation rewrites the control blocks (cycle, dt): keep the forward's
+    std::vector<InstCtl> ctl(static_cast<size_t>(c->L.ninst));
+    HIPCHK(c, hipMemcpyAsync(ctl.data(), c->d_ctl, ctl.size() * sizeof(InstCtl),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // roles of the three adjoint sets: O = output of the cycle being reversed,
+    // P = its input (accumulated), Q = input of its second stage (accumulated)
+    int iO = 0, iP = 1, iQ = 2;
+    {
+        const SetPtr O = set_ptr(D, D.set[iO]);
+        std::vector<double> t;
+        for (int d = 0; d < 3 && !rc; ++d) {
+            rc |= up_node(c, O.x[d], seed_x, d, t);
+            rc |= up_node(c, O.v[d], seed_v, d, t);
+        }
+        if (!rc) rc = up_elem(c, O.e, seed_e, t);
+        if (rc) {
+            set_error(c, "fsgh_backward: seed upload failed");
+            return FSGH_E_HIP;
+        }
     }
-    if (!rc) rc = up_elem(c, D.A.eo, seed_e, t);
-    if (rc) {
-        set_error(c, "fsgh_backward: seed upload failed");
-        return FSGH_E_HIP;
+    segcopy(c, {Seg{D.A.Mb, nullptr, E}, Seg{D.A.mb, nullptr, N}, Seg{D.A.gb, nullptr, E},
+                Seg{D.A.rhob, nullptr, E}});
+    if (seed_den) {
+        // the final density's seed, through M and the final coordinates
+        std::vector<double> t;
+        if (up_elem(c, D.grho, seed_den, t)) {
+            set_error(c, "fsgh_backward: seed upload failed");
+            return FSGH_E_HIP;
+        }
+        const SetPtr O = set_ptr(D, D.set[iO]);
+        DState sf = make_state(c, 1 - c->cur, c->cur);
+        hipLaunchKernelGGL(k_adj_seed_den, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, sf,
+                           static_cast<const double*>(D.grho), D.A.Mb, D.A.cb);
+        hipLaunchKernelGGL(k_adj_gather, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
+                           static_cast<const double*>(D.A.cb), O.x[0], O.x[1], O.x[2], O.v[0],
+                           O.v[1], O.v[2]);
     }
-    fill(c, D.A.Mb, E, 0.0);
-    fill(c, D.A.mb, N, 0.0);
-    fill(c, D.A.gb, E, 0.0);
-    fill(c, D.A.rhob, E, 0.0);
     const int64_t K = D.every;
     const int64_t nseg = (D.ncyc + K - 1) / K;
     for (int64_t sg = nseg - 1; sg >= 0; --sg) {
@@ -1223,77 +1344,63 @@ int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const
         // recompute the segment, keeping every cycle's state
         copy(c, D.seg[0], D.chk[static_cast<size_t>(sg)], SW);
         for (int64_t cy = c0; cy < c1; ++cy) {
-            recompute_cycle(c, cy, D.seg[static_cast<size_t>(cy - c0)]);
-            DState st = make_state(c, 0, 1);
-            st.div = c->d_div[1];
-            st.div_out = c->d_div[0];
+            recompute_cycle(c, cy, D.seg[static_cast<size_t>(cy - c0)], cy > c0);
+            DState st = elem_view(c, 0, 1, D.sb);
             launch_vel_bc_pos(m, st, c->d_ctl, 1.0, c->stream);
-            launch_energy_state_div(m, st, c->d_ctl, 1.0, nullptr, c->stream);
-            save_state(c, D.seg[static_cast<size_t>(cy - c0 + 1)], 1);
+            // stage-1 energy/EOS: the next cycle's stage-0 input state -> set A
+            DState se = elem_view(c, 0, 1, D.sa);
+            launch_energy_state_div(m, se, c->d_ctl, 1.0, nullptr, c->stream);
+            segcopy(c, bin_to_state(c, D.seg[static_cast<size_t>(cy - c0 + 1)], 1));
         }
         for (int64_t cy = c1 - 1; cy >= c0; --cy) {
-            const double* Sc = D.seg[static_cast<size_t>(cy - c0)];
             const double* Sn = D.seg[static_cast<size_t>(cy - c0 + 1)];
-            recompute_cycle(c, cy, Sc);
-            for (int d = 0; d < 3; ++d) {
-                fill(c, D.A.x0b[d], N, 0.0);
-                fill(c, D.A.v0b[d], N, 0.0);
-                fill(c, D.xab[d], N, 0.0);
-                fill(c, D.vab[d], N, 0.0);
-            }
-            fill(c, D.A.e0b, E, 0.0);
-            fill(c, D.esa, E, 0.0);
-            // stage 1 (alpha 1): input = stage-0 output (bin 1) + element arrays
-            DState s_in = make_state(c, 0, 1);
-            DState s_out = make_state(c, 0, 1);
+            recompute_cycle(c, cy, D.seg[static_cast<size_t>(cy - c0)], false);
+            const SetPtr O = set_ptr(D, D.set[iO]), P = set_ptr(D, D.set[iP]),
+                         Q = set_ptr(D, D.set[iQ]);
+            segcopy(c, {Seg{D.set[iP], nullptr, SW}, Seg{D.set[iQ], nullptr, SW}});
+            // stage 1 (alpha 1): input = stage-0 output (bin 1) + set B; its
+            // output velocities are those of S_{c+1}
+            DState s_in = elem_view(c, 0, 1, D.sb);
+            DState s_out = s_in;
             s_out.u1 = const_cast<double*>(Sn + 3 * N);
             s_out.v1 = const_cast<double*>(Sn + 4 * N);
             s_out.w1 = const_cast<double*>(Sn + 5 * N);
-            double* xa[3] = {D.xab[0], D.xab[1], D.xab[2]};
-            double* va[3] = {D.vab[0], D.vab[1], D.vab[2]};
-            reverse_stage(c, cy, 1, s_in, s_out, c->d_F, 1.0, c->d_sie[1], D.esa, xa, va);
-            // stage 0 (alpha 1/2): outputs = stage-1 inputs
-            for (int d = 0; d < 3; ++d) {
-                copy(c, D.A.xo[d], D.xab[d], N);
-                copy(c, D.A.vo[d], D.vab[d], N);
-            }
-            copy(c, D.A.eo, D.esa, E);
-            if (cy == 0) {
-                copy(c, c->d_den, D.den0, E);
-                copy(c, c->d_pres, D.pres0, E);
-                copy(c, c->d_sspd, D.sspd0, E);
-            } else {
-                DState s = make_state(c, 0, 0);
-                s.div = c->d_div[0];
-                hipLaunchKernelGGL(k_derived, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s);
-            }
-            DState s0_in = make_state(c, 0, 0);
-            DState s0_out = make_state(c, 0, 1);
-            double* x0[3] = {D.A.x0b[0], D.A.x0b[1], D.A.x0b[2]};
-            double* v0[3] = {D.A.v0b[0], D.A.v0b[1], D.A.v0b[2]};
-            reverse_stage(c, cy, 0, s0_in, s0_out, D.Fa, 0.5, c->d_sie[0], D.A.e0b, x0, v0);
-            // the cycle input adjoint is the previous cycle's output adjoint
-            for (int d = 0; d < 3; ++d) {
-                copy(c, D.A.xo[d], D.A.x0b[d], N);
-                copy(c, D.A.vo[d], D.A.v0b[d], N);
-            }
-            copy(c, D.A.eo, D.A.e0b, E);
+            reverse_stage(c, cy, 1, s_in, s_out, c->d_F, 1.0, c->d_sie[1], O, P, Q);
+            // stage 0 (alpha 1/2): input = S_c (bin 0) + its element state
+            double* S0[5];
+            stage0_set(c, cy, S0);
+            DState s0_in = elem_view(c, 0, 0, S0);
+            DState s0_out = elem_view(c, 0, 1, S0);
+            reverse_stage(c, cy, 0, s0_in, s0_out, D.Fa, 0.5, c->d_sie[0], Q, P, P);
+            // the input adjoint of this cycle is the output adjoint of the previous one
+            std::swap(iO, iP);
         }
     }
-    hipLaunchKernelGGL(k_adj_finish_elem, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, D.A, D.vol0,
-                       D.grho);
-    // restore the state the forward pass ended with
-    size_t off = 0;
-    for (auto& a : state_arrays(c)) {
-        copy(c, a.first, D.fin + off, a.second);
-        off += a.second;
+    const SetPtr G = set_ptr(D, D.set[iO]);
+    {
+        AdjView A = D.A;
+        hipLaunchKernelGGL(k_adj_finish_elem, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, A,
+                           static_cast<const double*>(D.vol0), D.grho);
     }
+    // restore the state the forward pass ended with
+    {
+        std::vector<Seg> v;
+        size_t off = 0;
+        for (auto& a : state_arrays(c)) {
+            v.push_back(Seg{a.first, D.fin + off, a.second});
+            off += a.second;
+        }
+        segcopy(c, v);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_ctl, ctl.data(), ctl.size() * sizeof(InstCtl),
+                             hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> t;
     rc |= down_elem(c, grad_den0, D.grho, t);
-    rc |= down_elem(c, grad_sie0, D.A.eo, t);
+    rc |= down_elem(c, grad_sie0, G.e, t);
     rc |= down_elem(c, grad_gamma, D.A.gb, t);
-    for (int d = 0; d < 3; ++d) rc |= down_node(c, grad_vel0, D.A.vo[d], d, t);
+    for (int d = 0; d < 3; ++d) rc |= down_node(c, grad_vel0, G.v[d], d, t);
     if (rc) {
         set_error(c, "fsgh_backward: gradient download failed");
         return FSGH_E_HIP;
@@ -1317,12 +1424,19 @@ int fsgh_tangent(fsgh_ctx* c, const double* d_den0, const double* d_sie0, const 
     std::vector<double> t;
     // initial values (the forward's first checkpoint and set-up state) and tangents
     const double* S0 = D.chk[0];
-    for (int k = 0; k < 6; ++k) copy(c, D.tn[1][k][0], S0 + k * N, N);
-    copy(c, D.te[1][0], S0 + 6 * N, E);
-    for (int d = 0; d < 3; ++d) {
-        fill(c, D.tn[1][d][1], N, 0.0);
-        rc |= up_node(c, D.tn[1][3 + d][1], d_vel0, d, t);
+    {
+        std::vector<Seg> v;
+        for (int k = 0; k < 6; ++k) v.push_back(Seg{D.tn[1][k][0], S0 + k * N, N});
+        v.push_back(Seg{D.te[1][0], S0 + 6 * N, E});
+        for (int d = 0; d < 3; ++d) v.push_back(Seg{D.tn[1][d][1], nullptr, N});
+        v.push_back(Seg{D.tel[0][0], D.den0, E});
+        v.push_back(Seg{D.tel[1][0], D.pres0, E});
+        v.push_back(Seg{D.tel[2][0], D.sspd0, E});
+        v.push_back(Seg{D.tel[3][0], D.vol0, E});
+        v.push_back(Seg{D.tel[3][1], nullptr, E});
+        segcopy(c, v);
     }
+    for (int d = 0; d < 3; ++d) rc |= up_node(c, D.tn[1][3 + d][1], d_vel0, d, t);
     rc |= up_elem(c, D.te[1][1], d_sie0, t);
     rc |= up_elem(c, D.tel[0][1], d_den0, t);
     rc |= up_elem(c, D.tg, d_gamma, t);
@@ -1330,13 +1444,9 @@ int fsgh_tangent(fsgh_ctx* c, const double* d_den0, const double* d_sie0, const 
         set_error(c, "fsgh_tangent: direction upload failed");
         return FSGH_E_HIP;
     }
-    copy(c, D.tel[0][0], D.den0, E);
-    copy(c, D.tel[1][0], D.pres0, E);
-    copy(c, D.tel[2][0], D.sspd0, E);
-    copy(c, D.tel[3][0], D.vol0, E);
-    fill(c, D.tel[3][1], E, 0.0);
-    hipLaunchKernelGGL(k_mul, dim3(nb_for(E)), dim3(kB), 0, c->stream, D.tM, D.vol0,
-                       D.tel[0][1], E);
+    hipLaunchKernelGGL(k_mul, dim3(nb_for(E)), dim3(kB), 0, c->stream, D.tM,
+                       static_cast<const double*>(D.vol0), static_cast<const double*>(D.tel[0][1]),
+                       E);
     int din = 0;
     {
         TLMState s = tlm_state(c, din);
@@ -1345,9 +1455,13 @@ int fsgh_tangent(fsgh_ctx* c, const double* d_den0, const double* d_sie0, const 
     }
     for (int64_t cy = 0; cy < D.ncyc; ++cy) {
         const double* dts = D.dt + static_cast<size_t>(cy) * D.ninst;
-        for (int k = 0; k < 6; ++k)
-            for (int j = 0; j < 2; ++j) copy(c, D.tn[0][k][j], D.tn[1][k][j], N);
-        for (int j = 0; j < 2; ++j) copy(c, D.te[0][j], D.te[1][j], E);
+        {
+            std::vector<Seg> v;
+            for (int k = 0; k < 6; ++k)
+                for (int j = 0; j < 2; ++j) v.push_back(Seg{D.tn[0][k][j], D.tn[1][k][j], N});
+            for (int j = 0; j < 2; ++j) v.push_back(Seg{D.te[0][j], D.te[1][j], E});
+            segcopy(c, v);
+        }
         for (int stage = 0; stage < 2; ++stage) {
             const double alpha = stage == 0 ? 0.5 : 1.0;
             TLMState s = tlm_state(c, din);

@@ -121,43 +121,54 @@ __global__ __launch_bounds__(kBlock) void k_force(DMesh m, DState s, const InstC
         small = ctl[bi.inst].small;
     }
     const int e = bi.start + static_cast<int>(threadIdx.x);
-    if (e >= bi.end) return;
+    // the block's corner forces are staged in LDS (row stride 13 x 16 B, free
+    // of bank conflicts) and written out as contiguous 16-byte stores
+    __shared__ double2 sF[kBlock * 13];
+    if (e < bi.end) {
+        int32_t nd[8];
+        load_conn(m.conn, e, nd);
+        double x[8], y[8], z[8], u[8], v[8], w[8];
+        gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+        gather3(s.u1, s.v1, s.w1, nd, u, v, w);
+        const double vol = s.vol[e];
+        const double div = s.div[e];
+        const double den = s.den[e];
+        const double sspd = s.sspd[e];
+        const double pres = s.pres[e];
+        const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
 
-    int32_t nd[8];
-    load_conn(m.conn, e, nd);
-    double x[8], y[8], z[8], u[8], v[8], w[8];
-    gather3(s.x1, s.y1, s.z1, nd, x, y, z);
-    gather3(s.u1, s.v1, s.w1, nd, u, v, w);
-    const double vol = s.vol[e];
-    const double div = s.div[e];
-    const double den = s.den[e];
-    const double sspd = s.sspd[e];
-    const double pres = s.pres[e];
-    const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
-
-    // Shock detector neighbour extreme (K/force_sgh.cpp:297-310).  r_face =
-    // 0.9 (d_nb + small)/(div + small) is monotone in d_nb (every rounding
-    // step is monotone), so the minimum over neighbours is reached at min d_nb
-    // when div+small >= 0 and at max d_nb otherwise.  Neighbours = every other
-    // element sharing a node (the reference's elems_in_elem).  k_div_extremes
-    // stores per node the two smallest and two largest divergences of its
-    // elements (with multiplicity); removing one copy of this element's own
-    // value leaves the extreme over the other elements of that node.
-    const bool lo = div + small >= 0.0;
-    const double2* __restrict__ xs = lo ? s.ext_lo : s.ext_hi;
-    double ext = lo ? HUGE_VAL : -HUGE_VAL;
+        // Shock detector neighbour extreme (K/force_sgh.cpp:297-310).  r_face =
+        // 0.9 (d_nb + small)/(div + small) is monotone in d_nb (every rounding
+        // step is monotone), so the minimum over neighbours is reached at min d_nb
+        // when div+small >= 0 and at max d_nb otherwise.  Neighbours = every other
+        // element sharing a node (the reference's elems_in_elem).  k_div_extremes
+        // stores per node the two smallest and two largest divergences of its
+        // elements (with multiplicity); removing one copy of this element's own
+        // value leaves the extreme over the other elements of that node.
+        const bool lo = div + small >= 0.0;
+        const double2* __restrict__ xs = lo ? s.ext_lo : s.ext_hi;
+        double ext = lo ? HUGE_VAL : -HUGE_VAL;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const double2 t = xs[nd[a]];
-        const double other = (t.x == div) ? t.y : t.x;
-        ext = lo ? fmin(ext, other) : fmax(ext, other);
+        for (int a = 0; a < 8; ++a) {
+            const double2 t = xs[nd[a]];
+            const double other = (t.x == div) ? t.y : t.x;
+            ext = lo ? fmin(ext, other) : fmax(ext, other);
+        }
+        double f[24];
+        force_elem(x, y, z, u, v, w, den, pres, sspd, vol, div, ext, mt.q1, mt.q1ex, mt.q2,
+                   mt.q2ex, fuzz, small, f);
+#pragma unroll
+        for (int i = 0; i < 12; ++i)
+            sF[threadIdx.x * 13 + i] = make_double2(f[2 * i], f[2 * i + 1]);
     }
-    double f[24];
-    force_elem(x, y, z, u, v, w, den, pres, sspd, vol, div, ext, mt.q1, mt.q1ex, mt.q2, mt.q2ex,
-               fuzz, small, f);
-    double2* out = reinterpret_cast<double2*>(s.F + 24 * static_cast<size_t>(e));
+    __syncthreads();
+    const int cnt = 12 * (bi.end - bi.start);
+    double2* out = reinterpret_cast<double2*>(s.F + 24 * static_cast<size_t>(bi.start));
 #pragma unroll
-    for (int i = 0; i < 12; ++i) out[i] = make_double2(f[2 * i], f[2 * i + 1]);
+    for (int r = 0; r < 12; ++r) {
+        const int idx = r * kBlock + static_cast<int>(threadIdx.x);
+        if (idx < cnt) out[idx] = sF[(idx / 12) * 13 + idx % 12];
+    }
 }
 
 // --------------------------------------------------------------------------

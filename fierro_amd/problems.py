@@ -139,6 +139,36 @@ def sedov(n=32, q2=1.333, dt_cfl=0.5, cycles=200, gamma=5.0 / 3.0):
     return spec
 
 
+
+def blast(n, E0, gamma, rho0=1.0, cycles=150, q2=1.333, dt_cfl=0.5):
+    """Point blast of the calibration sweep: the Sedov geometry of ``sedov``
+    (n^3 hexes on [0,1.2]^3, reflected x=y=z=0) with density rho0, the
+    energy E0 deposited in the origin element as e = E0 / (rho0 V), e=1e-10
+    elsewhere, and the given gamma."""
+    h = 1.2 / n
+    spec = sedov(n, q2=q2, dt_cfl=dt_cfl, cycles=cycles, gamma=gamma)
+    spec.name = f"blast{n}"
+    spec.fills = [fill(GLOBAL, den=rho0, sie=1.0e-10),
+                  fill(SPHERE, den=rho0, sie=E0 / (rho0 * h * h * h), radius1=0.0, radius2=h)]
+    return spec
+
+
+def blast_ensemble(count=64, seed=0, cycles=150):
+    """The calibration sweep: ``count`` blasts with n ~ U{48..96},
+    E0 ~ U(0.1, 1.0), gamma ~ U(1.3, 1.8), rho0 = 1, drawn per instance in
+    that order from numpy's PCG64(seed); one more draw from the same stream
+    is the target.  Returns (specs, params, target_spec, target_params),
+    params = [(n, E0, gamma, rho0)]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    params = []
+    for _ in range(count + 1):
+        n = int(rng.integers(48, 97))
+        E0 = float(rng.uniform(0.1, 1.0))
+        gamma = float(rng.uniform(1.3, 1.8))
+        params.append((n, E0, gamma, 1.0))
+    specs = [blast(n, E0, g, r, cycles=cycles) for n, E0, g, r in params]
+    return specs[:count], params[:count], specs[count], params[count]
+
 def reference_sedov(mesh_path):
     """The reference's default compiled-in problem (K/input.cpp:69-202 with
     K/driver.cpp defaults) on a given .geo mesh: 6 reflected planes of the

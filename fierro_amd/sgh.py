@@ -289,16 +289,17 @@ class SGHSolver:
         check(self.L.fsgh_dt_history(self.ctx, _p(out), out.size), self.ctx, "dt_history")
         return out
 
-    def backward(self, seed_x=None, seed_v=None, seed_e=None):
-        """Gradients of <seed, final state> with respect to the initial
-        density, specific internal energy, gamma and node velocity."""
+    def backward(self, seed_x=None, seed_v=None, seed_e=None, seed_den=None):
+        """Gradients of <seed, final state> (coordinates, velocities,
+        energy, density M/V) with respect to the initial density, specific
+        internal energy, gamma and node velocity."""
         def arr(a, shape):
             return None if a is None else np.ascontiguousarray(a, np.float64).reshape(shape)
         sx, sv = arr(seed_x, (self.N, 3)), arr(seed_v, (self.N, 3))
-        se = arr(seed_e, (self.E,))
+        se, sd = arr(seed_e, (self.E,)), arr(seed_den, (self.E,))
         g = dict(den0=np.zeros(self.E), sie0=np.zeros(self.E), gamma=np.zeros(self.E),
                  vel0=np.zeros((self.N, 3)))
-        ptr = [None if a is None else _p(a) for a in (sx, sv, se)]
+        ptr = [None if a is None else _p(a) for a in (sx, sv, se, sd)]
         check(self.L.fsgh_backward(self.ctx, *ptr, _p(g["den0"]), _p(g["sie0"]),
                                    _p(g["gamma"]), _p(g["vel0"])), self.ctx, "backward")
         return g

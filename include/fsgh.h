@@ -233,14 +233,14 @@ int fsgh_forward(fsgh_ctx *ctx, const fsgh_run_params *params, int64_t ncycles,
 /* The recorded time steps, n = ncycles * num_instances, [cycle][instance]. */
 int fsgh_dt_history(fsgh_ctx *ctx, double *out, int64_t n);
 /* Reverse pass: given the adjoint seeds of the final state (node
- * coordinates and velocities [N][3], element energy [E]; NULL = zero),
- * returns the gradients of <seed, final state> with respect to rho0 [E],
- * e0 [E], gamma [E] and the initial velocity [N][3] (any output may be
- * NULL).  Recomputes each checkpoint segment; leaves the forward's final
- * state in place.  Caller numbering throughout. */
+ * coordinates and velocities [N][3], element energy [E], element density
+ * M/V [E]; NULL = zero), returns the gradients of <seed, final state> with
+ * respect to rho0 [E], e0 [E], gamma [E] and the initial velocity [N][3]
+ * (any output may be NULL).  Recomputes each checkpoint segment; leaves the
+ * forward's final state in place.  Caller numbering throughout. */
 int fsgh_backward(fsgh_ctx *ctx, const double *seed_x, const double *seed_v,
-                  const double *seed_e, double *grad_den0, double *grad_sie0,
-                  double *grad_gamma, double *grad_vel0);
+                  const double *seed_e, const double *seed_den, double *grad_den0,
+                  double *grad_sie0, double *grad_gamma, double *grad_vel0);
 /* Tangent-linear model: the directional derivative of the final state
  * (coordinates [N][3], velocities [N][3], energy [E]) along the control
  * perturbation (d_den0 [E], d_sie0 [E], d_gamma [E], d_vel0 [N][3]; NULL =
