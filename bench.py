@@ -72,7 +72,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-cycles", type=int, default=100)
-    ap.add_argument("--workload", choices=["sedov", "sod-ensemble", "anvil-adjoint"],
+    ap.add_argument("--workload",
+                    choices=["sedov", "sod-ensemble", "anvil-adjoint", "calibration"],
                     default="sedov")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--loop-sample", type=int, default=8,
@@ -91,6 +92,8 @@ def main():
         return ensemble(args)
     if args.workload == "anvil-adjoint":
         return adjoint(args)
+    if args.workload == "calibration":
+        return calibration(args)
 
     from fierro_amd import bytemodel, problems as P
     from fierro_amd.sgh import SGHSolver
@@ -302,6 +305,58 @@ def adjoint(args):
         "plain_forward_s": t_plain, "checkpointed_forward_s": t_fw, "reverse_s": t_bw,
         "plain_zone_cycles_per_s": E * cycles / t_plain,
         "grad_norm_den0": float((grad["den0"] ** 2).sum() ** 0.5),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def calibration(args):
+    """Batched forward + adjoint calibration sweep: 64 point blasts, n^3 with
+    n ~ U{48..96}, E0 ~ U(0.1,1), gamma ~ U(1.3,1.8), rho0 = 1, 150 cycles,
+    seed 0; gradient of each instance's radial density-profile misfit against
+    the target (the next draw of the stream) with respect to (E0, gamma, rho0)
+    from one batched checkpointed forward (every 5 cycles) + one reverse
+    pass; compared with one plain batched forward (fused loop, hipGraph)."""
+    import numpy as np
+    from fierro_amd import calibration as CAL, problems as P
+    from fierro_amd.sgh import SGHSolver
+
+    cycles, every = 150, 5
+    specs, params, tspec, tparams = P.blast_ensemble(64, seed=0, cycles=cycles)
+    t0 = time.perf_counter()
+    target = CAL.target_profile(SGHSolver, tspec, cycles)
+    log(f"target profile ({tparams}) in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    g = SGHSolver(specs, renumber=True)
+    E = g.E
+    log(f"calibration setup: {len(specs)} instances E={E} N={g.N} in "
+        f"{time.perf_counter() - t0:.1f} s")
+    g.run_begin()
+    t = time.perf_counter()
+    g.run(cycles, use_graph=True)
+    t_plain = time.perf_counter() - t
+    g.close()
+    g = SGHSolver(specs, renumber=True)
+    t = time.perf_counter()
+    J, dE0, dgam, drho, _ = CAL.gradient(g, params, target, cycles, every)
+    t_grad = time.perf_counter() - t
+    g.close()
+    log(f"plain forward {t_plain:.3f} s, forward + reverse {t_grad:.3f} s")
+    out = {
+        "metric": "batched forward+adjoint time / plain batched forward time "
+                  "(64 blasts, 150 cycles)",
+        "value": t_grad / t_plain, "unit": "x", "n_gpus": 1, "steps": cycles, "warmup": 0,
+        "ms_per_step": 1e3 * t_grad / cycles, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded blast ensemble)",
+        "config": {"workload": "blast-calibration64", "elements": E, "cycles": cycles,
+                   "checkpoint_every": every, "seed": 0, "bins": CAL.NBINS,
+                   "rmax": CAL.RMAX},
+        "plain_forward_s": t_plain, "gradient_s": t_grad,
+        "plain_zone_cycles_per_s": E * cycles / t_plain,
+        "misfit_sum": float(np.sum(J)),
+        "grad_norms": {"E0": float(np.linalg.norm(dE0)), "gamma": float(np.linalg.norm(dgam)),
+                       "rho0": float(np.linalg.norm(drho))},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

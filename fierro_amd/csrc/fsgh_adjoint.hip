@@ -9,21 +9,32 @@
 //
 // Reverse pass, per RK stage (alpha, dt), hand-derived from the forward
 // operators in fsgh_physics.h:
-//   k_adj_energy   e1 = e0 - alpha dt/M sum F.(v1+v0)/2     -> Fb, hb (per corner), Mb, e0b
-//   k_adj_node     v1 = mask(v0 + alpha dt sum F/m), x1 = x0 + alpha dt (v1+v0)/2
-//                  (gathers hb in corner order)             -> x0b, v0b, Fb, mb
-//   k_adj_force    corner forces (MARS + shock detector)    -> per-corner (xb, vb), element
-//                  (den, pres, sspd, vol, div, ext) adjoints
-//   k_adj_derived  neighbour-divergence adjoint (the extreme neighbour chosen by the
-//                  forward), then vol/den/EOS/div of the stage input -> eb, Mb, gb, corner xb, vb
+//   k_adj_energy   e1 = e0 - alpha dt/M sum_c F_c.(v1+v0)/2: e0-bar, M-bar and
+//                  the element scalar g = -alpha dt/M e1-bar
+//   k_adj_node     v1 = mask(v0 + alpha dt sum F/m), x1 = x0 + alpha dt (v1+v0)/2,
+//                  with the energy's corner terms g (v1+v0)/2 and g F_c formed at
+//                  the node (CSR order)                  -> x0-bar, v0-bar, F-bar, m-bar
+//   k_arg_extremes per node, the two smallest / largest divergences with the
+//                  elements holding them
+//   k_adj_force    corner forces (MARS + shock detector) -> per-corner (x-bar, v-bar),
+//                  element (den, pres, sspd, vol, div, ext) adjoints, ext's element
+//   k_adj_ext_node per node and extreme slot, the ext-adjoints of the elements
+//                  that chose it (each element then adds the sums of the slots
+//                  it occupies to its divergence adjoint, in k_adj_derived)
+//   k_adj_derived  vol / den / EOS / div of the stage input -> e-bar, M-bar,
+//                  gamma-bar, corner x-bar, v-bar
 //   k_adj_gather   corner adjoints -> node adjoints, in corner order (no atomics)
 // Branches (compression test, min/max, clamps, the sound-speed floor) follow
 // the forward values; ties of the neighbour extreme go to the first element
 // found, the same choice the tangent-linear model makes.
 //
 // Checkpointing: the state (x, v, e) is stored every `checkpoint_every`
-// cycles; each segment is recomputed forward (storing its states) before it
-// is reversed; inside a cycle, stage 0 is recomputed from the cycle's state.
+// cycles.  The reverse pass recomputes one segment at a time, keeping every
+// cycle's intermediates (stage-0 output, both stages' corner forces and
+// input element states) in per-cycle slots written in place by the forward
+// kernels, then reverses the segment's cycles from those slots.  Device
+// memory of the reverse pass: (K + 1)(6N + 6E) + K(6N + 54E) doubles for
+// K = checkpoint_every, plus (ncycles / K)(6N + E) for the checkpoints.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -75,15 +86,37 @@ struct TA {
 // -------------------------------------------------------------------------
 // divergence extremes with the element that holds them (node-centric)
 // -------------------------------------------------------------------------
+// insert (v, e) into the two smallest / largest so far; ties keep the
+// earlier entry (strict comparisons), as a sequential scan would
 __device__ __forceinline__ void ins_lo(double& a, double& b, int& ia, int& ib, double v, int e)
 {
-    if (v < a) { b = a; ib = ia; a = v; ia = e; }
-    else if (v < b) { b = v; ib = e; }
+    const bool la = v < a, lb = v < b;
+    b = la ? a : (lb ? v : b);
+    ib = la ? ia : (lb ? e : ib);
+    a = la ? v : a;
+    ia = la ? e : ia;
 }
 __device__ __forceinline__ void ins_hi(double& a, double& b, int& ia, int& ib, double v, int e)
 {
-    if (v > a) { b = a; ib = ia; a = v; ia = e; }
-    else if (v > b) { b = v; ib = e; }
+    const bool ga = v > a, gb = v > b;
+    b = ga ? a : (gb ? v : b);
+    ib = ga ? ia : (gb ? e : ib);
+    a = ga ? v : a;
+    ia = ga ? e : ia;
+}
+
+// The first eight corner ids of node n's CSR list, with indices clamped to
+// the list (to entry 0 of the array for an empty list) so every load is in
+// bounds and none waits on a branch; returns the list length.  Callers use
+// entry j only when j < length.
+__device__ __forceinline__ int corners8(const DMesh& m, int n, int q0, int c8[8])
+{
+    const int cnt = m.cin_start[n + 1] - q0;
+    const int base = cnt > 0 ? q0 : 0;
+    const int last = cnt > 0 ? cnt - 1 : 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c8[j] = m.cin[base + (j < last ? j : last)];
+    return cnt;
 }
 
 // div: element divergence values; dtan (may be null): their tangents
@@ -96,12 +129,27 @@ __global__ __launch_bounds__(kB) void k_arg_extremes(DMesh m, const double* div,
     if (n >= bi.end) return;
     double a1 = HUGE_VAL, a2 = HUGE_VAL, b1 = -HUGE_VAL, b2 = -HUGE_VAL;
     int ia1 = -1, ia2 = -1, ib1 = -1, ib2 = -1;
-    for (int q = m.cin_start[n]; q < m.cin_start[n + 1]; ++q) {
-        const int e = m.cin[q] >> 3;
-        const double v = div[e];
-        if (isnan(v)) continue;
-        ins_lo(a1, a2, ia1, ia2, v, e);
-        ins_hi(b1, b2, ib1, ib2, v, e);
+    const int q0 = m.cin_start[n];
+    int c8[8];
+    const int cnt = corners8(m, n, q0, c8);
+    if (cnt > 0) {
+        double d[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) d[j] = div[c8[j] >> 3];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool ok = j < cnt && !isnan(d[j]);
+            const int e = c8[j] >> 3;
+            ins_lo(a1, a2, ia1, ia2, ok ? d[j] : HUGE_VAL, e);
+            ins_hi(b1, b2, ib1, ib2, ok ? d[j] : -HUGE_VAL, e);
+        }
+        for (int q = q0 + 8; q < q0 + cnt; ++q) {
+            const int e = m.cin[q] >> 3;
+            const double v = div[e];
+            if (isnan(v)) continue;
+            ins_lo(a1, a2, ia1, ia2, v, e);
+            ins_hi(b1, b2, ib1, ib2, v, e);
+        }
     }
     lo[n] = make_double2(a1, a2);
     hi[n] = make_double2(b1, b2);
@@ -115,14 +163,18 @@ __global__ __launch_bounds__(kB) void k_arg_extremes(DMesh m, const double* div,
 
 // The neighbour extreme of element e and the element holding it: drop e
 // itself from each node's pair (by id), keep the first strict improvement.
+// code (may be null): 4 * node + slot of the chosen entry (slot 0/1 = the
+// node's smallest / second smallest, 2/3 = largest / second largest), -1
+// when there is no neighbour.
 __device__ __forceinline__ double neighbour_extreme(const int32_t nd[8], int e, bool low,
                                                     const double2* lo, const double2* hi,
                                                     const int2* alo, const int2* ahi, int& arg,
                                                     const double2* tlo, const double2* thi,
-                                                    double* tan)
+                                                    double* tan, int* code = nullptr)
 {
     double ext = low ? HUGE_VAL : -HUGE_VAL, te = 0.0;
     arg = -1;
+    int cd = -1;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         const double2 t = low ? lo[nd[a]] : hi[nd[a]];
@@ -134,6 +186,7 @@ __device__ __forceinline__ double neighbour_extreme(const int32_t nd[8], int e, 
         if (better) {
             ext = o;
             arg = oid;
+            cd = 4 * nd[a] + (low ? 0 : 2) + (skip ? 1 : 0);
             if (tan != nullptr) {
                 const double2 tt = low ? tlo[nd[a]] : thi[nd[a]];
                 te = skip ? tt.y : tt.x;
@@ -141,6 +194,7 @@ __device__ __forceinline__ double neighbour_extreme(const int32_t nd[8], int e, 
         }
     }
     if (tan != nullptr) *tan = te;
+    if (code != nullptr) *code = arg >= 0 ? cd : -1;
     return ext;
 }
 
@@ -306,82 +360,170 @@ struct AdjView {
     double *xo[3], *vo[3], *eo;
     double *x0b[3], *v0b[3], *e0b;
     double *Mb, *mb, *gb, *rhob;
-    double *Fb, *hb, *cb;       // corners: 3, 3, 6
+    double* ge;                 // element: -alpha dt / M * e-bar of the stage output energy
+    double *Fb, *cb;            // corners: 3, 6
     double* eadj;               // element: den, pres, sspd, vol, div, ext (6 x E)
-    int32_t* argext;
+    int32_t* argext;            // element: 4 * node + slot of its chosen extreme, or -1
+    double* nacc;               // node: per-slot sums of ext-bar (4 x N)
     double2 *lo, *hi;
     int2 *alo, *ahi;
 };
 
-// e1 = e0 - alpha dt / M * sum_c F_c . (v1+v0)/2
+// e1 = e0 - alpha dt / M * P,  P = sum_c F_c . (v1+v0)/2: the element part
+// (e0-bar, M-bar and the scalar g = -alpha dt / M * e1-bar that the node
+// kernel turns into the corner terms g (v1+v0)/2 and g F_c).
+// assign != 0: the accumulators (e0-bar here, x0-bar / v0-bar in the node
+// kernel, the input adjoints in k_adj_derived / k_adj_gather) are written
+// instead of added to (the first stage reversed in a cycle)
 __global__ __launch_bounds__(kB) void k_adj_energy(DMesh m, DState s, const double* F,
-                                                   AdjView A, const double* dts, double alpha)
+                                                   AdjView A, const double* dts, double alpha,
+                                                   int assign)
 {
     const BlockItem bi = m.eblk[blockIdx.x];
     const int e = bi.start + static_cast<int>(threadIdx.x);
     if (e >= bi.end) return;
     const double k = alpha * dts[bi.inst];
     const double eb = A.eo[e];
-    A.e0b[e] += eb;
+    A.e0b[e] = assign ? eb : A.e0b[e] + eb;
+    const double M = s.emass[e];
+    A.ge[e] = -k / M * eb;
+    if (eb == 0.0) return;
     int32_t nd[8];
     ld_conn(m.conn, e, nd);
-    const double M = s.emass[e];
-    const double g = -k / M * eb;
+    const double* f = F + 24 * static_cast<size_t>(e);
     double P = 0.0;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        const size_t c = 8 * static_cast<size_t>(e) + a;
-        const double h[3] = {(s.u1[nd[a]] + s.u0[nd[a]]) * 0.5, (s.v1[nd[a]] + s.v0[nd[a]]) * 0.5,
-                             (s.w1[nd[a]] + s.w0[nd[a]]) * 0.5};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double f = F[3 * c + d];
-            P += f * h[d];
-            A.Fb[3 * c + d] = g * h[d];
-            A.hb[3 * c + d] = g * f;
-        }
+        P += f[3 * a + 0] * ((s.u1[nd[a]] + s.u0[nd[a]]) * 0.5);
+        P += f[3 * a + 1] * ((s.v1[nd[a]] + s.v0[nd[a]]) * 0.5);
+        P += f[3 * a + 2] * ((s.w1[nd[a]] + s.w0[nd[a]]) * 0.5);
     }
     A.Mb[e] += k / (M * M) * eb * P;
 }
 
-// v1 = mask(v0 + alpha dt F/m), x1 = x0 + alpha dt (v1 + v0)/2, plus the
-// energy update's dependence on v1 and v0 through hb.
+// v1 = mask(v0 + alpha dt F/m), x1 = x0 + alpha dt (v1 + v0)/2, and the
+// energy update's corner terms: F_c-bar = g_e (v1+v0)/2 + alpha dt/m v1-bar,
+// and (v1+v0)/2-bar = sum_c g_e F_c over the node's corners (CSR order).
 __global__ __launch_bounds__(kB) void k_adj_node(DMesh m, DState s, const double* F, AdjView A,
-                                                 const double* dts, double alpha)
+                                                 const double* dts, double alpha, int assign)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
     const int n = bi.start + static_cast<int>(threadIdx.x);
     if (n >= bi.end) return;
     const double k = alpha * dts[bi.inst];
-    const int q0 = m.cin_start[n], q1 = m.cin_start[n + 1];
+    const int q0 = m.cin_start[n];
+    int c8[8];
+    const int cnt = corners8(m, n, q0, c8);
     double sh[3] = {0.0, 0.0, 0.0}, sf[3] = {0.0, 0.0, 0.0};
-    for (int q = q0; q < q1; ++q) {
-        const size_t c = static_cast<size_t>(m.cin[q]);
+    double g8[8];
+    {
+        double f8[8][3];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            g8[j] = A.ge[c8[j] >> 3];
+            const double* f = F + 3 * static_cast<size_t>(c8[j]);
+            f8[j][0] = f[0];
+            f8[j][1] = f[1];
+            f8[j][2] = f[2];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (j < cnt) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    sh[d] += g8[j] * f8[j][d];
+                    sf[d] += f8[j][d];
+                }
+            }
+        }
+    }
+    for (int q = q0 + 8; q < q0 + cnt; ++q) {
+        const int c = m.cin[q];
+        const double g = A.ge[c >> 3];
+        const double* f = F + 3 * static_cast<size_t>(c);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            sh[d] += A.hb[3 * c + d];
-            sf[d] += F[3 * c + d];
+            sh[d] += g * f[d];
+            sf[d] += f[d];
         }
     }
     const double mass = s.nmass[n];
     const uint8_t mk = s.bcmask[n];
+    const double h[3] = {(s.u1[n] + s.u0[n]) * 0.5, (s.v1[n] + s.v0[n]) * 0.5,
+                         (s.w1[n] + s.w0[n]) * 0.5};
     double vm[3];
     double dot = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const double xb = A.xo[d][n];
-        A.x0b[d][n] += xb;
+        A.x0b[d][n] = assign ? xb : A.x0b[d][n] + xb;
         const double vt = A.vo[d][n] + 0.5 * k * xb + 0.5 * sh[d];
         vm[d] = (mk & (1u << d)) ? 0.0 : vt;
-        A.v0b[d][n] += 0.5 * k * xb + 0.5 * sh[d] + vm[d];
+        const double v0b = 0.5 * k * xb + 0.5 * sh[d] + vm[d];
+        A.v0b[d][n] = assign ? v0b : A.v0b[d][n] + v0b;
         dot += vm[d] * sf[d];
     }
     A.mb[n] += -k * dot / (mass * mass);
-    for (int q = q0; q < q1; ++q) {
-        const size_t c = static_cast<size_t>(m.cin[q]);
+    const double km = k / mass;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) A.Fb[3 * c + d] += k / mass * vm[d];
+    for (int j = 0; j < 8; ++j) {
+        if (j < cnt) {
+            double* fb = A.Fb + 3 * static_cast<size_t>(c8[j]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) fb[d] = g8[j] * h[d] + km * vm[d];
+        }
     }
+    for (int q = q0 + 8; q < q0 + cnt; ++q) {
+        const int c = m.cin[q];
+        const double g = A.ge[c >> 3];
+        double* fb = A.Fb + 3 * static_cast<size_t>(c);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) fb[d] = g * h[d] + km * vm[d];
+    }
+}
+
+// Shock-detector neighbour term: add to each element's divergence adjoint
+// the ext-adjoints of the elements whose extreme neighbour it is.  A
+// neighbour sharing several nodes is counted once, at the first shared node
+// in local order.
+// Shock-detector neighbour term, in two deterministic passes.  Element e2
+// took its extreme from slot s of node n (argext = 4 n + s); the element in
+// that slot gets ext-bar(e2) added to its divergence adjoint.  Node pass
+// (here): per node and slot, the sum of ext-bar over the elements (in CSR
+// order) whose choice was that node and slot.  Element pass (start of
+// k_adj_derived): each element adds the sums of the slots it occupies at its
+// 8 nodes.  Every e2 contributes once.
+__global__ __launch_bounds__(kB) void k_adj_ext_node(DMesh m, AdjView A)
+{
+    const BlockItem bi = m.nblk[blockIdx.x];
+    const int n = bi.start + static_cast<int>(threadIdx.x);
+    if (n >= bi.end) return;
+    const size_t E = static_cast<size_t>(m.E);
+    const int q0 = m.cin_start[n];
+    int c8[8];
+    const int cnt = corners8(m, n, q0, c8);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    int code[8];
+    double xb[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        code[j] = A.argext[c8[j] >> 3];
+        xb[j] = A.eadj[5 * E + (c8[j] >> 3)];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const bool mine = j < cnt && (code[j] >> 2) == n && code[j] >= 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] += (mine && (code[j] & 3) == k) ? xb[j] : 0.0;
+    }
+    for (int q = q0 + 8; q < q0 + cnt; ++q) {
+        const int e2 = m.cin[q] >> 3;
+        const int cd = A.argext[e2];
+        if (cd >= 0 && (cd >> 2) == n) acc[cd & 3] += A.eadj[5 * E + e2];
+    }
+    double2* out = reinterpret_cast<double2*>(A.nacc + 4 * static_cast<size_t>(n));
+    out[0] = make_double2(acc[0], acc[1]);
+    out[1] = make_double2(acc[2], acc[3]);
 }
 
 // Reverse of force_elem (K/force_sgh.cpp:35-359) at the stage input state.
@@ -402,8 +544,9 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
     const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
     const bool low = div + small >= 0.0;
     int arg;
+    int code;
     const double ext = neighbour_extreme(nd, e, low, A.lo, A.hi, A.alo, A.ahi, arg, nullptr,
-                                         nullptr, nullptr);
+                                         nullptr, nullptr, &code);
     // ---- forward recompute ----
     double B[24];
     hex_bmatrix(x, y, z, B);
@@ -534,7 +677,7 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
     A.eadj[3 * E + e] = volb;
     A.eadj[4 * E + e] = divb;
     A.eadj[5 * E + e] = (arg >= 0) ? extb : 0.0;
-    A.argext[e] = arg;
+    A.argext[e] = code;
 }
 
 // Neighbour divergence adjoint, then the stage-input state (vol, den, EOS,
@@ -544,7 +687,7 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
 __global__ __launch_bounds__(kB) void k_adj_derived(DMesh m, DState s, AdjView A,
                                                     const double* vol_a, const double* div_a,
                                                     const double* sie_in, double* sieb_in,
-                                                    int setup)
+                                                    int setup, int assign)
 {
     const BlockItem bi = m.eblk[blockIdx.x];
     const int e = bi.start + static_cast<int>(threadIdx.x);
@@ -552,20 +695,16 @@ __global__ __launch_bounds__(kB) void k_adj_derived(DMesh m, DState s, AdjView A
     const size_t E = static_cast<size_t>(m.E);
     int32_t nd[8];
     ld_conn(m.conn, e, nd);
-    // elements whose shock detector picked this element's divergence; each
-    // neighbour is counted once (at the first shared node in local order)
+    // divergence adjoint: own term plus the shock-detector neighbour term,
+    // the per-slot sums (k_adj_ext_node) of the slots this element occupies
     double divb = A.eadj[4 * E + e];
+#pragma unroll
     for (int a = 0; a < 8; ++a) {
-        for (int q = m.cin_start[nd[a]]; q < m.cin_start[nd[a] + 1]; ++q) {
-            const int e2 = m.cin[q] >> 3;
-            if (e2 == e || A.argext[e2] != e) continue;
-            int32_t n2[8];
-            ld_conn(m.conn, e2, n2);
-            bool earlier = false;
-            for (int b = 0; b < a && !earlier; ++b)
-                for (int k = 0; k < 8; ++k) earlier = earlier || n2[k] == nd[b];
-            if (!earlier) divb += A.eadj[5 * E + e2];
-        }
+        const int2 il = A.alo[nd[a]], ih = A.ahi[nd[a]];
+        const double2* p = reinterpret_cast<const double2*>(A.nacc + 4 * static_cast<size_t>(nd[a]));
+        const double2 s01 = p[0], s23 = p[1];
+        divb += (il.x == e ? s01.x : 0.0) + (il.y == e ? s01.y : 0.0) +
+                (ih.x == e ? s23.x : 0.0) + (ih.y == e ? s23.y : 0.0);
     }
     const double denb = A.eadj[0 * E + e], presb = A.eadj[1 * E + e], sspdb = A.eadj[2 * E + e];
     double volb = A.eadj[3 * E + e];
@@ -585,7 +724,7 @@ __global__ __launch_bounds__(kB) void k_adj_derived(DMesh m, DState s, AdjView A
         eb += sspdb * gam * (gam - 1.0) / (2.0 * c);
         gb += sspdb * (2.0 * gam - 1.0) * sie / (2.0 * c);
     }
-    sieb_in[e] += eb;
+    sieb_in[e] = assign ? eb : sieb_in[e] + eb;
     A.gb[e] += gb;
     // divergence (K/momentum.cpp:265-286): div = (sum_a v_a . B_a) / vol
     double B[24];
@@ -630,23 +769,49 @@ __global__ __launch_bounds__(kB) void k_adj_derived(DMesh m, DState s, AdjView A
 // corner adjoints -> node adjoints, summed in corner order
 __global__ __launch_bounds__(kB) void k_adj_gather(DMesh m, const double* cb, double* xa0,
                                                    double* xa1, double* xa2, double* va0,
-                                                   double* va1, double* va2)
+                                                   double* va1, double* va2, int assign)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
     const int n = bi.start + static_cast<int>(threadIdx.x);
     if (n >= bi.end) return;
+    const int q0 = m.cin_start[n];
+    int c8[8];
+    const int cnt = corners8(m, n, q0, c8);
     double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int q = m.cin_start[n]; q < m.cin_start[n + 1]; ++q) {
+    {
+        double v8[8][6];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const double2* p = reinterpret_cast<const double2*>(cb + 6 * static_cast<size_t>(c8[j]));
+            const double2 a = p[0], b = p[1], c = p[2];
+            v8[j][0] = a.x; v8[j][1] = a.y; v8[j][2] = b.x;
+            v8[j][3] = b.y; v8[j][4] = c.x; v8[j][5] = c.y;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j < cnt)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) s[k] += v8[j][k];
+    }
+    for (int q = q0 + 8; q < q0 + cnt; ++q) {
         const double* p = cb + 6 * static_cast<size_t>(m.cin[q]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) s[k] += p[k];
     }
-    xa0[n] += s[0];
-    xa1[n] += s[1];
-    xa2[n] += s[2];
-    va0[n] += s[3];
-    va1[n] += s[4];
-    va2[n] += s[5];
+    if (!assign) {
+        s[0] += xa0[n];
+        s[1] += xa1[n];
+        s[2] += xa2[n];
+        s[3] += va0[n];
+        s[4] += va1[n];
+        s[5] += va2[n];
+    }
+    xa0[n] = s[0];
+    xa1[n] = s[1];
+    xa2[n] = s[2];
+    va0[n] = s[3];
+    va1[n] = s[4];
+    va2[n] = s[5];
 }
 
 // Final element density den = M / V(x) as an objective term: its seed sd
@@ -765,19 +930,28 @@ struct AdjointData {
     size_t N = 0, E = 0;
     double* dt = nullptr;                 // [ncyc][ninst]
     std::vector<double*> chk;             // states every `every` cycles (x,y,z,u,v,w | e)
-    std::vector<double*> seg;             // recomputed states of one segment (every + 1)
     double *den0 = nullptr, *pres0 = nullptr, *sspd0 = nullptr, *vol0 = nullptr;
-    double* fin = nullptr;                // final forward state, restored after a reverse pass
     std::vector<double*> bufs;            // everything allocated, released together
     bool done = false;
-    // reverse pass
+    // reverse pass: one segment of `every` cycles recomputed and kept per
+    // bank; with two banks the next segment is recomputed on a second stream
+    // while the current one is reversed
     bool rev_ready = false;
-    double* sa[5] = {};                   // stage-0 input element state: den, pres, sspd, vol, div
-    double* sb[5] = {};                   // stage-1 input element state
-    double* Fa = nullptr;                 // stage-0 corner forces
+    struct Bank {
+        std::vector<double*> seg;         // cycle-start states S_c of the segment (every + 1)
+        std::vector<double*> sta;         // stage-0 outputs (x,y,z,u,v,w | e)
+        std::vector<double*> Fa, Fb1;     // stage-0 / stage-1 corner forces
+        std::vector<double*> ea, eb;      // stage-0 / stage-1 input element state (5 x E)
+        std::vector<double*> x0, x1;      // stage-0 / stage-1 divergence extremes (6 x N, two banks)
+        double* dt = nullptr;             // the segment's time steps [every][instances]
+    } bank[2];
+    hipGraphExec_t rgraph[4] = {};        // reverse pass of a full segment, per (bank, role parity)
+    int nbank = 1;
+    hipStream_t rs = nullptr;             // recompute stream (two banks)
+    hipEvent_t ready[2] = {}, freed[2] = {};
     double* set[3] = {};                  // adjoint state sets (x,y,z,u,v,w | e), rotating roles
     double* grho = nullptr;
-    AdjView A{};                          // shared per-run accumulators and scratch
+    AdjView A{};                          // per-run accumulators and per-stage scratch
     // tangent-linear model
     bool tlm_ready = false;
     double* tn[2][6][2] = {};             // node bins x (value, tangent)
@@ -824,7 +998,7 @@ void segcopy(fsgh_ctx* c, const std::vector<Seg>& v)
         SegList L{};
         long long off = 0;
         int k = 0;
-        for (size_t i = b; i < v.size() && k < kMaxSeg; ++i) {
+        for (size_t i = b; i < v.size() && i < b + kMaxSeg; ++i) {
             if (v[i].n == 0) continue;
             L.dst[k] = v[i].dst;
             L.src[k] = v[i].src;
@@ -839,18 +1013,8 @@ void segcopy(fsgh_ctx* c, const std::vector<Seg>& v)
                            c->stream, L);
     }
 }
-void fill0(fsgh_ctx* c, double* p, size_t n) { segcopy(c, {Seg{p, nullptr, n}}); }
-void copy(fsgh_ctx* c, double* dst, const double* src, size_t n) { segcopy(c, {Seg{dst, src, n}}); }
 
 // state layout of checkpoint/segment buffers and adjoint sets: x, y, z, u, v, w (N each), e (E)
-std::vector<Seg> state_to_bin(fsgh_ctx* c, const double* src, int bin)
-{
-    const size_t N = static_cast<size_t>(c->N), E = static_cast<size_t>(c->E);
-    std::vector<Seg> v;
-    for (int k = 0; k < 6; ++k) v.push_back(Seg{c->d_node[bin][k], src + k * N, N});
-    v.push_back(Seg{c->d_sie[bin], src + 6 * N, E});
-    return v;
-}
 std::vector<Seg> bin_to_state(fsgh_ctx* c, double* dst, int bin)
 {
     const size_t N = static_cast<size_t>(c->N), E = static_cast<size_t>(c->E);
@@ -859,21 +1023,6 @@ std::vector<Seg> bin_to_state(fsgh_ctx* c, double* dst, int bin)
     v.push_back(Seg{dst + 6 * N, c->d_sie[bin], E});
     return v;
 }
-
-// every mutable state array of the context (both bins + element state)
-std::vector<std::pair<double*, size_t>> state_arrays(fsgh_ctx* c)
-{
-    const size_t N = static_cast<size_t>(c->N), E = static_cast<size_t>(c->E);
-    std::vector<std::pair<double*, size_t>> v;
-    for (int b = 0; b < 2; ++b) {
-        for (int k = 0; k < 6; ++k) v.emplace_back(c->d_node[b][k], N);
-        v.emplace_back(c->d_sie[b], E);
-    }
-    for (double* p : {c->d_den, c->d_pres, c->d_sspd, c->d_vol, c->d_div[0], c->d_div[1]})
-        v.emplace_back(p, E);
-    return v;
-}
-size_t state_words(fsgh_ctx* c) { return 12 * static_cast<size_t>(c->N) + 8 * static_cast<size_t>(c->E); }
 
 int init_ctl(fsgh_ctx* c, const fsgh_run_params* p)
 {
@@ -946,22 +1095,32 @@ int down_node(fsgh_ctx* c, double* dst, const double* src, int comp, std::vector
     return FSGH_OK;
 }
 
-// state view with the element state taken from `set` (den, pres, sspd, vol, div)
-DState elem_view(fsgh_ctx* c, int b0, int b1, double* const* set)
+// A state view over arbitrary buffers: t_n state `s0` and stage state `s1`
+// (x,y,z,u,v,w | e layout), element state `el` (den, pres, sspd, vol, div),
+// corner forces F; mesh-level fields from the context.
+DState view(fsgh_ctx* c, const double* s0, const double* s1, double* const* el, double* F)
 {
-    DState s = make_state(c, b0, b1);
-    s.den = set[0];
-    s.pres = set[1];
-    s.sspd = set[2];
-    s.vol = set[3];
-    s.div = set[4];
-    s.div_out = set[4];
+    const size_t N = static_cast<size_t>(c->N);
+    DState s = make_state(c, 0, 1);
+    double* a = const_cast<double*>(s0);
+    double* b = const_cast<double*>(s1);
+    s.x0 = a; s.y0 = a + N; s.z0 = a + 2 * N; s.u0 = a + 3 * N; s.v0 = a + 4 * N; s.w0 = a + 5 * N;
+    s.x1 = b; s.y1 = b + N; s.z1 = b + 2 * N; s.u1 = b + 3 * N; s.v1 = b + 4 * N; s.w1 = b + 5 * N;
+    s.sie0 = a + 6 * N;
+    s.sie1 = b + 6 * N;
+    s.den = el[0];
+    s.pres = el[1];
+    s.sspd = el[2];
+    s.vol = el[3];
+    s.div = el[4];
+    s.div_out = el[4];
+    s.F = F;
     return s;
 }
 
-// the element state the stage-0 force of cycle `cyc` reads: the set-up state
-// (rho0, EOS pressure and sound speed, V0) for cycle 0, else set A
-void stage0_set(fsgh_ctx* c, int64_t cyc, double* out[5])
+// the stage-0 input element state of cycle `cyc` (segment slot j): the
+// set-up state (rho0, EOS pressure and sound speed, V0) for cycle 0
+void stage0_set(fsgh_ctx* c, const AdjointData::Bank& B, int64_t cyc, size_t j, double* out[5])
 {
     AdjointData& D = *c->adj;
     if (cyc == 0) {
@@ -970,48 +1129,106 @@ void stage0_set(fsgh_ctx* c, int64_t cyc, double* out[5])
         out[2] = D.sspd0;
         out[3] = D.vol0;
     } else {
-        for (int k = 0; k < 4; ++k) out[k] = D.sa[k];
+        for (int k = 0; k < 4; ++k) out[k] = B.ea[j] + k * D.E;
     }
-    out[4] = D.sa[4];
+    out[4] = B.ea[j] + 4 * D.E;
+}
+void elem_set(const AdjointData& D, double* base, double* out[5])
+{
+    for (int k = 0; k < 5; ++k) out[k] = base + k * D.E;
 }
 
-// Recompute cycle `cyc` from `state` (loaded into both bins) up to the
-// stage-1 forces.  Leaves: bin 0 = S_c, bin 1 = stage-0 output, Fa =
-// stage-0 forces, set A (or the set-up state) = stage-0 input element
-// state, set B = stage-1 input element state, d_F = stage-1 forces.
-// setA_ready: set A already holds the state derived from `state`.
-void recompute_cycle(fsgh_ctx* c, int64_t cyc, const double* state, bool setA_ready)
+// Per-node divergence extremes with their elements (k_arg_extremes output)
+// of one stage input, in one block of 6N doubles: lo, hi (double2), alo,
+// ahi (int2).
+struct ExtPtr {
+    double2 *lo, *hi;
+    int2 *alo, *ahi;
+};
+ExtPtr ext_ptr(size_t N, double* base)
+{
+    ExtPtr p;
+    p.lo = reinterpret_cast<double2*>(base);
+    p.hi = reinterpret_cast<double2*>(base + 2 * N);
+    p.alo = reinterpret_cast<int2*>(base + 4 * N);
+    p.ahi = reinterpret_cast<int2*>(base + 5 * N);
+    return p;
+}
+
+// Recompute the cycles [c0, c1) of one segment from checkpoint `chk` on
+// stream `st`, keeping every intermediate the reverse pass reads:
+// seg[j] = S_c, sta[j] = stage-0 output, Fa[j] / Fb1[j] = stage-0 / stage-1
+// forces, ea[j] / eb[j] = stage-0 / stage-1 input element state.  No
+// copies: every kernel reads and writes the slot buffers directly.
+void recompute_segment(fsgh_ctx* c, AdjointData::Bank& B, const double* chk, int64_t c0,
+                       int64_t c1, hipStream_t st)
 {
     AdjointData& D = *c->adj;
     DMesh m = make_mesh(c);
-    std::vector<Seg> ld = state_to_bin(c, state, 0);
-    for (const Seg& q : state_to_bin(c, state, 1)) ld.push_back(q);
-    segcopy(c, ld);
-    double* S0[5];
-    stage0_set(c, cyc, S0);
-    if (cyc == 0) {
-        launch_get_divergence(m, elem_view(c, 0, 1, S0), c->stream);
-    } else if (!setA_ready) {
-        hipLaunchKernelGGL(k_derived, dim3(m.n_eblk), dim3(kB), 0, c->stream, m,
-                           elem_view(c, 0, 1, D.sa));
+    {
+        // the checkpoint and the segment's time steps into the bank
+        SegList L{};
+        const long long nst = static_cast<long long>(6 * D.N + D.E);
+        const long long ndt = static_cast<long long>(c1 - c0) * D.ninst;
+        L.dst[0] = B.seg[0];
+        L.src[0] = chk;
+        L.dst[1] = B.dt;
+        L.src[1] = D.dt + static_cast<size_t>(c0) * D.ninst;
+        L.start[0] = 0;
+        L.start[1] = nst;
+        L.start[2] = nst + ndt;
+        L.count = 2;
+        hipLaunchKernelGGL(k_segcopy, dim3(nb_for(static_cast<size_t>(nst + ndt))), dim3(kB), 0,
+                           st, L);
     }
-    hipLaunchKernelGGL(k_set_cycle_dt, dim3(nb_for(static_cast<size_t>(c->L.ninst))), dim3(kB), 0,
-                       c->stream, c->d_ctl, c->L.ninst, static_cast<long long>(cyc),
-                       static_cast<const double*>(D.dt));
-    DState s0 = elem_view(c, 0, 0, S0);
-    s0.F = D.Fa;
-    launch_div_extremes(m, s0, c->d_ctl, c->stream);
-    launch_force(m, s0, c->d_ctl, 0.0, 0.0, c->stream);
-    DState st = elem_view(c, 0, 1, D.sb);    // energy/EOS of stage 0 write set B
-    st.F = D.Fa;
-    launch_vel_bc_pos(m, st, c->d_ctl, 0.5, c->stream);
-    launch_energy_state_div(m, st, c->d_ctl, 0.5, nullptr, c->stream);
-    DState s1 = elem_view(c, 0, 1, D.sb);
-    launch_div_extremes(m, s1, c->d_ctl, c->stream);
-    launch_force(m, s1, c->d_ctl, 0.0, 0.0, c->stream);
+    for (int64_t cy = c0; cy < c1; ++cy) {
+        const size_t j = static_cast<size_t>(cy - c0);
+        double* S0[5];
+        stage0_set(c, B, cy, j, S0);
+        double *EB[5], *EA1[5];
+        elem_set(D, B.eb[j], EB);
+        elem_set(D, B.ea[j + 1], EA1);
+        if (cy == 0) {
+            launch_get_divergence(m, view(c, B.seg[0], B.seg[0], S0, nullptr), st);
+        } else if (j == 0) {
+            hipLaunchKernelGGL(k_derived, dim3(m.n_eblk), dim3(kB), 0, st, m,
+                               view(c, B.seg[0], B.seg[0], S0, nullptr));
+        }
+        hipLaunchKernelGGL(k_set_cycle_dt, dim3(nb_for(static_cast<size_t>(c->L.ninst))),
+                           dim3(kB), 0, st, c->d_ctl, c->L.ninst, static_cast<long long>(cy),
+                           static_cast<const double*>(D.dt));
+        // stage 0 (alpha 1/2): S_c -> sta[j]; element state -> eb[j]
+        DState f0 = view(c, B.seg[j], B.seg[j], S0, B.Fa[j]);
+        launch_div_extremes(m, f0, c->d_ctl, st);
+        launch_force(m, f0, c->d_ctl, 0.0, 0.0, st);
+        if (!B.x0.empty()) {
+            const ExtPtr x = ext_ptr(D.N, B.x0[j]);
+            hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), dim3(kB), 0, st, m,
+                               static_cast<const double*>(S0[4]),
+                               static_cast<const double*>(nullptr), x.lo, x.hi, x.alo, x.ahi,
+                               static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
+        }
+        DState u0 = view(c, B.seg[j], B.sta[j], EB, B.Fa[j]);
+        launch_vel_bc_pos(m, u0, c->d_ctl, 0.5, st);
+        launch_energy_state_div(m, u0, c->d_ctl, 0.5, nullptr, st);
+        // stage 1 (alpha 1): sta[j] -> seg[j+1]; element state -> ea[j+1]
+        DState f1 = view(c, B.sta[j], B.sta[j], EB, B.Fb1[j]);
+        launch_div_extremes(m, f1, c->d_ctl, st);
+        launch_force(m, f1, c->d_ctl, 0.0, 0.0, st);
+        if (!B.x1.empty()) {
+            const ExtPtr x = ext_ptr(D.N, B.x1[j]);
+            hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), dim3(kB), 0, st, m,
+                               static_cast<const double*>(EB[4]),
+                               static_cast<const double*>(nullptr), x.lo, x.hi, x.alo, x.ahi,
+                               static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
+        }
+        DState u1 = view(c, B.seg[j], B.seg[j + 1], EA1, B.Fb1[j]);
+        launch_vel_bc_pos(m, u1, c->d_ctl, 1.0, st);
+        launch_energy_state_div(m, u1, c->d_ctl, 1.0, nullptr, st);
+    }
 }
 
-// adjoint set k: pointers to its x,y,z / u,v,w / e parts
+// adjoint set: pointers to its x,y,z / u,v,w / e parts
 struct SetPtr {
     double *x[3], *v[3], *e;
 };
@@ -1026,16 +1243,18 @@ SetPtr set_ptr(const AdjointData& D, double* base)
     return p;
 }
 
-// Reverse one RK stage.  out: adjoints of the stage outputs; acc: adjoints
-// of the cycle state (bin 0), accumulated; in: adjoints of the stage input
-// (x1..w1 of s_in and its element energy), accumulated.
+// Reverse one RK stage.  s_in: stage input (x1..w1, element state, its
+// energy in sie1); s_out: stage output velocities in u1..w1, cycle state in
+// u0..w0; F: the stage's corner forces.  out: adjoints of the stage outputs;
+// acc: adjoints of the cycle state; in: adjoints of the stage input; assign:
+// acc and in are written, not added to.  ext: the stage input's divergence
+// extremes if already computed (else computed here).
 void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, const DState& s_out,
-                   const double* F, double alpha, const double* ein, const SetPtr& out,
-                   const SetPtr& acc, const SetPtr& in)
+                   const double* F, double alpha, const SetPtr& out, const SetPtr& acc,
+                   const SetPtr& in, int assign, const ExtPtr* ext, const double* dts)
 {
     AdjointData& D = *c->adj;
     DMesh m = make_mesh(c);
-    const double* dts = D.dt + static_cast<size_t>(cyc) * D.ninst;
     AdjView A = D.A;
     for (int d = 0; d < 3; ++d) {
         A.xo[d] = out.x[d];
@@ -1045,23 +1264,32 @@ void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, cons
     }
     A.eo = out.e;
     A.e0b = acc.e;
+    if (ext) {
+        A.lo = ext->lo;
+        A.hi = ext->hi;
+        A.alo = ext->alo;
+        A.ahi = ext->ahi;
+    }
     hipLaunchKernelGGL(k_adj_energy, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_out, F, A, dts,
-                       alpha);
+                       alpha, assign);
     hipLaunchKernelGGL(k_adj_node, dim3(m.n_nblk), dim3(kB), 0, c->stream, m, s_out, F, A, dts,
-                       alpha);
-    hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
-                       static_cast<const double*>(s_in.div), static_cast<const double*>(nullptr),
-                       D.lo, D.hi, D.alo, D.ahi, static_cast<double2*>(nullptr),
-                       static_cast<double2*>(nullptr));
+                       alpha, assign);
+    if (!ext)
+        hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
+                           static_cast<const double*>(s_in.div),
+                           static_cast<const double*>(nullptr), A.lo, A.hi, A.alo, A.ahi,
+                           static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
     hipLaunchKernelGGL(k_adj_force, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_in, A,
                        static_cast<const InstCtl*>(c->d_ctl), static_cast<const double*>(s_in.vol),
                        static_cast<const double*>(s_in.div));
+    hipLaunchKernelGGL(k_adj_ext_node, dim3(m.n_nblk), dim3(kB), 0, c->stream, m, A);
     hipLaunchKernelGGL(k_adj_derived, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_in, A,
                        static_cast<const double*>(s_in.vol), static_cast<const double*>(s_in.div),
-                       ein, in.e, (cyc == 0 && stage == 0) ? 1 : 0);
+                       static_cast<const double*>(s_in.sie1), in.e,
+                       (cyc == 0 && stage == 0) ? 1 : 0, assign);
     hipLaunchKernelGGL(k_adj_gather, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
                        static_cast<const double*>(A.cb), in.x[0], in.x[1], in.x[2], in.v[0],
-                       in.v[1], in.v[2]);
+                       in.v[1], in.v[2], assign);
 }
 
 int ensure_extreme_buffers(fsgh_ctx* c)
@@ -1079,35 +1307,76 @@ int ensure_extreme_buffers(fsgh_ctx* c)
     return rc ? FSGH_E_NOMEM : FSGH_OK;
 }
 
+int alloc_bank(fsgh_ctx* c, AdjointData::Bank& B, size_t K, bool ext)
+{
+    AdjointData& D = *c->adj;
+    const size_t N = D.N, E = D.E, C = 8 * E, SW = 6 * N + E;
+    int rc = adj_alloc(c, &B.dt, K * static_cast<size_t>(D.ninst));
+    for (size_t k = 0; k <= K && !rc; ++k) {
+        double *p = nullptr, *q = nullptr;
+        rc |= adj_alloc(c, &p, SW);
+        rc |= adj_alloc(c, &q, 5 * E);
+        B.seg.push_back(p);
+        B.ea.push_back(q);
+    }
+    for (size_t k = 0; k < K && !rc; ++k) {
+        double *a = nullptr, *f0 = nullptr, *f1 = nullptr, *b = nullptr;
+        rc |= adj_alloc(c, &a, SW);
+        rc |= adj_alloc(c, &f0, 3 * C);
+        rc |= adj_alloc(c, &f1, 3 * C);
+        rc |= adj_alloc(c, &b, 5 * E);
+        B.sta.push_back(a);
+        B.Fa.push_back(f0);
+        B.Fb1.push_back(f1);
+        B.eb.push_back(b);
+        if (ext) {
+            double *g0 = nullptr, *g1 = nullptr;
+            rc |= adj_alloc(c, &g0, 6 * N);
+            rc |= adj_alloc(c, &g1, 6 * N);
+            B.x0.push_back(g0);
+            B.x1.push_back(g1);
+        }
+    }
+    return rc;
+}
+
 int ensure_reverse_buffers(fsgh_ctx* c)
 {
     AdjointData& D = *c->adj;
     if (D.rev_ready) return FSGH_OK;
     const size_t N = D.N, E = D.E, C = 8 * E;
     const size_t SW = 6 * N + E;
+    const size_t K = static_cast<size_t>(std::min<int64_t>(D.every, D.ncyc));
+    // a second bank (recompute overlapped with the reverse pass) when there
+    // is more than one segment and both banks take under a quarter of the
+    // free device memory
+    const size_t bank_bytes = 8 * ((K + 1) * (6 * N + 6 * E) + K * (18 * N + 54 * E));
+    size_t mfree = 0, mtotal = 0;
+    const bool two = D.ncyc > D.every && hipMemGetInfo(&mfree, &mtotal) == hipSuccess &&
+                     2 * bank_bytes < mfree / 4;
+    D.nbank = two ? 2 : 1;
     int rc = ensure_extreme_buffers(c);
-    for (int64_t k = 0; k <= D.every && !rc; ++k) {
-        double* p = nullptr;
-        rc = adj_alloc(c, &p, SW);
-        D.seg.push_back(p);
-    }
-    for (int k = 0; k < 5; ++k) {
-        rc |= adj_alloc(c, &D.sa[k], E);
-        rc |= adj_alloc(c, &D.sb[k], E);
+    for (int b = 0; b < D.nbank && !rc; ++b) rc |= alloc_bank(c, D.bank[b], K, two);
+    if (!rc && two) {
+        if (hipStreamCreateWithFlags(&D.rs, hipStreamNonBlocking) != hipSuccess) rc = FSGH_E_HIP;
+        for (int b = 0; b < 2 && !rc; ++b)
+            if (hipEventCreateWithFlags(&D.ready[b], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&D.freed[b], hipEventDisableTiming) != hipSuccess)
+                rc = FSGH_E_HIP;
     }
     for (int k = 0; k < 3; ++k) rc |= adj_alloc(c, &D.set[k], SW);
     rc |= adj_alloc(c, &D.A.Mb, E);
     rc |= adj_alloc(c, &D.A.mb, N);
     rc |= adj_alloc(c, &D.A.gb, E);
     rc |= adj_alloc(c, &D.A.rhob, E);
+    rc |= adj_alloc(c, &D.A.ge, E);
     rc |= adj_alloc(c, &D.grho, E);
     rc |= adj_alloc(c, &D.A.Fb, 3 * C);
-    rc |= adj_alloc(c, &D.A.hb, 3 * C);
     rc |= adj_alloc(c, &D.A.cb, 6 * C);
     rc |= adj_alloc(c, &D.A.eadj, 6 * E);
     rc |= adj_alloc_t(c, &D.A.argext, E);
-    rc |= adj_alloc(c, &D.Fa, 3 * C);
-    if (rc) return FSGH_E_NOMEM;
+    rc |= adj_alloc(c, &D.A.nacc, 4 * N);
+    if (rc) return rc == FSGH_E_HIP ? FSGH_E_HIP : FSGH_E_NOMEM;
     D.A.lo = D.lo;
     D.A.hi = D.hi;
     D.A.alo = D.alo;
@@ -1181,6 +1450,17 @@ TLMState tlm_state(fsgh_ctx* c, int din)
 void fsgh::free_adjoint(fsgh_ctx* c)
 {
     if (!c->adj) return;
+    AdjointData& D = *c->adj;
+    if (D.rs) {
+        hipStreamSynchronize(D.rs);
+        hipStreamDestroy(D.rs);
+    }
+    for (int b = 0; b < 2; ++b) {
+        if (D.ready[b]) hipEventDestroy(D.ready[b]);
+        if (D.freed[b]) hipEventDestroy(D.freed[b]);
+    }
+    for (hipGraphExec_t g : D.rgraph)
+        if (g) hipGraphExecDestroy(g);
     for (double* p : c->adj->bufs) hipFree(p);
     delete c->adj;
     c->adj = nullptr;
@@ -1221,7 +1501,6 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
     if (!rc) rc |= adj_alloc(c, &D.pres0, E);
     if (!rc) rc |= adj_alloc(c, &D.sspd0, E);
     if (!rc) rc |= adj_alloc(c, &D.vol0, E);
-    if (!rc) rc |= adj_alloc(c, &D.fin, state_words(c));
     if (rc) {
         free_adjoint(c);
         return FSGH_E_NOMEM;
@@ -1250,15 +1529,6 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
     }
     c->divcur = 1;
     c->run_active = true;
-    {
-        std::vector<Seg> v;
-        size_t off = 0;
-        for (auto& a : state_arrays(c)) {
-            v.push_back(Seg{D.fin + off, a.first, a.second});
-            off += a.second;
-        }
-        segcopy(c, v);
-    }
     std::vector<InstCtl> h(static_cast<size_t>(c->L.ninst));
     HIPCHK(c, hipMemcpyAsync(h.data(), c->d_ctl, h.size() * sizeof(InstCtl), hipMemcpyDeviceToHost,
                              c->stream));
@@ -1295,6 +1565,10 @@ int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const
         return FSGH_E_STATE;
     }
     AdjointData& D = *c->adj;
+    if (D.N >= (size_t(1) << 29)) {
+        set_error(c, "fsgh_backward: more than 2^29 nodes");
+        return FSGH_E_UNSUPPORTED;
+    }
     int rc = ensure_reverse_buffers(c);
     if (rc) return rc;
     const size_t N = D.N, E = D.E, SW = 6 * N + E;
@@ -1307,14 +1581,15 @@ int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const
     // roles of the three adjoint sets: O = output of the cycle being reversed,
     // P = its input (accumulated), Q = input of its second stage (accumulated)
     int iO = 0, iP = 1, iQ = 2;
+    std::vector<double> t;
     {
         const SetPtr O = set_ptr(D, D.set[iO]);
-        std::vector<double> t;
         for (int d = 0; d < 3 && !rc; ++d) {
             rc |= up_node(c, O.x[d], seed_x, d, t);
             rc |= up_node(c, O.v[d], seed_v, d, t);
         }
         if (!rc) rc = up_elem(c, O.e, seed_e, t);
+        if (!rc && seed_den) rc = up_elem(c, D.grho, seed_den, t);
         if (rc) {
             set_error(c, "fsgh_backward: seed upload failed");
             return FSGH_E_HIP;
@@ -1324,57 +1599,95 @@ int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const
                 Seg{D.A.rhob, nullptr, E}});
     if (seed_den) {
         // the final density's seed, through M and the final coordinates
-        std::vector<double> t;
-        if (up_elem(c, D.grho, seed_den, t)) {
-            set_error(c, "fsgh_backward: seed upload failed");
-            return FSGH_E_HIP;
-        }
         const SetPtr O = set_ptr(D, D.set[iO]);
         DState sf = make_state(c, 1 - c->cur, c->cur);
         hipLaunchKernelGGL(k_adj_seed_den, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, sf,
                            static_cast<const double*>(D.grho), D.A.Mb, D.A.cb);
         hipLaunchKernelGGL(k_adj_gather, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
                            static_cast<const double*>(D.A.cb), O.x[0], O.x[1], O.x[2], O.v[0],
-                           O.v[1], O.v[2]);
+                           O.v[1], O.v[2], 0);
     }
     const int64_t K = D.every;
     const int64_t nseg = (D.ncyc + K - 1) / K;
+    auto seg_range = [&](int64_t sg, int64_t& c0, int64_t& c1) {
+        c0 = sg * K;
+        c1 = std::min(D.ncyc, c0 + K);
+    };
+    auto recompute = [&](int64_t sg) {
+        int64_t c0, c1;
+        seg_range(sg, c0, c1);
+        const int b = D.nbank == 2 ? static_cast<int>(sg & 1) : 0;
+        hipStream_t st = D.nbank == 2 ? D.rs : c->stream;
+        recompute_segment(c, D.bank[b], D.chk[static_cast<size_t>(sg)], c0, c1, st);
+        if (D.nbank == 2) HIPCHK(c, hipEventRecord(D.ready[b], D.rs));
+        return FSGH_OK;
+    };
+    if (D.nbank == 2) {
+        // the recompute stream starts after the seeds and accumulators are set
+        HIPCHK(c, hipEventRecord(D.freed[0], c->stream));
+        HIPCHK(c, hipEventRecord(D.freed[1], c->stream));
+        HIPCHK(c, hipStreamWaitEvent(D.rs, D.freed[(nseg - 1) & 1], 0));
+    }
+    if ((rc = recompute(nseg - 1))) return rc;
     for (int64_t sg = nseg - 1; sg >= 0; --sg) {
-        const int64_t c0 = sg * K, c1 = std::min(D.ncyc, c0 + K);
-        // recompute the segment, keeping every cycle's state
-        copy(c, D.seg[0], D.chk[static_cast<size_t>(sg)], SW);
-        for (int64_t cy = c0; cy < c1; ++cy) {
-            recompute_cycle(c, cy, D.seg[static_cast<size_t>(cy - c0)], cy > c0);
-            DState st = elem_view(c, 0, 1, D.sb);
-            launch_vel_bc_pos(m, st, c->d_ctl, 1.0, c->stream);
-            // stage-1 energy/EOS: the next cycle's stage-0 input state -> set A
-            DState se = elem_view(c, 0, 1, D.sa);
-            launch_energy_state_div(m, se, c->d_ctl, 1.0, nullptr, c->stream);
-            segcopy(c, bin_to_state(c, D.seg[static_cast<size_t>(cy - c0 + 1)], 1));
+        int64_t c0, c1;
+        seg_range(sg, c0, c1);
+        const int b = D.nbank == 2 ? static_cast<int>(sg & 1) : 0;
+        const AdjointData::Bank& B = D.bank[b];
+        if (D.nbank == 2) {
+            HIPCHK(c, hipStreamWaitEvent(c->stream, D.ready[b], 0));
+            if (sg > 0) {
+                // the previous segment goes into the other bank once its last reader is done
+                HIPCHK(c, hipStreamWaitEvent(D.rs, D.freed[(sg - 1) & 1], 0));
+                if ((rc = recompute(sg - 1))) return rc;
+            }
         }
+        // A full segment other than the first is the same launch sequence
+        // for a given bank and role parity: captured once, replayed.
+        const bool graph = D.nbank == 2 && sg > 0 && c1 - c0 == K;
+        const int key = 2 * b + iO;
+        const bool replay = graph && D.rgraph[key] != nullptr;
+        if (graph && !replay)
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int64_t cy = c1 - 1; cy >= c0; --cy) {
-            const double* Sn = D.seg[static_cast<size_t>(cy - c0 + 1)];
-            recompute_cycle(c, cy, D.seg[static_cast<size_t>(cy - c0)], false);
-            const SetPtr O = set_ptr(D, D.set[iO]), P = set_ptr(D, D.set[iP]),
-                         Q = set_ptr(D, D.set[iQ]);
-            segcopy(c, {Seg{D.set[iP], nullptr, SW}, Seg{D.set[iQ], nullptr, SW}});
-            // stage 1 (alpha 1): input = stage-0 output (bin 1) + set B; its
-            // output velocities are those of S_{c+1}
-            DState s_in = elem_view(c, 0, 1, D.sb);
-            DState s_out = s_in;
-            s_out.u1 = const_cast<double*>(Sn + 3 * N);
-            s_out.v1 = const_cast<double*>(Sn + 4 * N);
-            s_out.w1 = const_cast<double*>(Sn + 5 * N);
-            reverse_stage(c, cy, 1, s_in, s_out, c->d_F, 1.0, c->d_sie[1], O, P, Q);
-            // stage 0 (alpha 1/2): input = S_c (bin 0) + its element state
-            double* S0[5];
-            stage0_set(c, cy, S0);
-            DState s0_in = elem_view(c, 0, 0, S0);
-            DState s0_out = elem_view(c, 0, 1, S0);
-            reverse_stage(c, cy, 0, s0_in, s0_out, D.Fa, 0.5, c->d_sie[0], Q, P, P);
+            const size_t j = static_cast<size_t>(cy - c0);
+            if (!replay) {
+                const SetPtr O = set_ptr(D, D.set[iO]), P = set_ptr(D, D.set[iP]),
+                             Q = set_ptr(D, D.set[iQ]);
+                double *S0[5], *EB[5];
+                stage0_set(c, B, cy, j, S0);
+                elem_set(D, B.eb[j], EB);
+                ExtPtr x0, x1;
+                if (!B.x0.empty()) {
+                    x0 = ext_ptr(N, B.x0[j]);
+                    x1 = ext_ptr(N, B.x1[j]);
+                }
+                const bool pre = !B.x0.empty();
+                const double* dts = B.dt + j * static_cast<size_t>(D.ninst);
+                // stage 1 (alpha 1): input sta[j] + eb[j]; output velocities of
+                // seg[j+1]; writes P (cycle input) and Q (stage-1 input)
+                reverse_stage(c, cy, 1, view(c, B.seg[j], B.sta[j], EB, nullptr),
+                              view(c, B.seg[j], B.seg[j + 1], EB, nullptr), B.Fb1[j], 1.0, O, P,
+                              Q, 1, pre ? &x1 : nullptr, dts);
+                // stage 0 (alpha 1/2): input seg[j] + its element state; output
+                // sta[j]; adds to P
+                reverse_stage(c, cy, 0, view(c, B.seg[j], B.seg[j], S0, nullptr),
+                              view(c, B.seg[j], B.sta[j], S0, nullptr), B.Fa[j], 0.5, Q, P, P, 0,
+                              pre ? &x0 : nullptr, dts);
+            }
             // the input adjoint of this cycle is the output adjoint of the previous one
             std::swap(iO, iP);
         }
+        if (graph && !replay) {
+            hipGraph_t gr;
+            HIPCHK(c, hipStreamEndCapture(c->stream, &gr));
+            const hipError_t ie = hipGraphInstantiate(&D.rgraph[key], gr, nullptr, nullptr, 0);
+            hipGraphDestroy(gr);
+            HIPCHK(c, ie);
+        }
+        if (graph) HIPCHK(c, hipGraphLaunch(D.rgraph[key], c->stream));
+        if (D.nbank == 2) HIPCHK(c, hipEventRecord(D.freed[b], c->stream));
+        else if (sg > 0 && (rc = recompute(sg - 1))) return rc;
     }
     const SetPtr G = set_ptr(D, D.set[iO]);
     {
@@ -1382,21 +1695,10 @@ int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const
         hipLaunchKernelGGL(k_adj_finish_elem, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, A,
                            static_cast<const double*>(D.vol0), D.grho);
     }
-    // restore the state the forward pass ended with
-    {
-        std::vector<Seg> v;
-        size_t off = 0;
-        for (auto& a : state_arrays(c)) {
-            v.push_back(Seg{a.first, D.fin + off, a.second});
-            off += a.second;
-        }
-        segcopy(c, v);
-    }
     HIPCHK(c, hipMemcpyAsync(c->d_ctl, ctl.data(), ctl.size() * sizeof(InstCtl),
                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::vector<double> t;
     rc |= down_elem(c, grad_den0, D.grho, t);
     rc |= down_elem(c, grad_sie0, G.e, t);
     rc |= down_elem(c, grad_gamma, D.A.gb, t);
