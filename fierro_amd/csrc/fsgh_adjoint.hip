@@ -557,15 +557,15 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
         for (int d = 0; d < 3; ++d) vbar[d] += 0.125 * vel[d][a];
     const double qa = div < 0.0 ? mt.q1 : mt.q1ex;
     const double qb = div < 0.0 ? mt.q2 : mt.q2ex;
-    double dl[8][3], mv[8], muc[8], nb[8], Mt[8], S[3] = {0.0, 0.0, 0.0}, W = 0.0;
+    // kept per corner: |v - v_bar|, |B_a|, mu |B_a|; the rest is recomputed
+    double mv[8], nb[8], Mt[8], S[3] = {0.0, 0.0, 0.0}, W = 0.0;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) dl[a][d] = vel[d][a] - vbar[d];
-        mv[a] = sqrt(dl[a][0] * dl[a][0] + dl[a][1] * dl[a][1] + dl[a][2] * dl[a][2]);
-        muc[a] = den * (qa * sspd + qb * mv[a]);
+        const double d0 = vel[0][a] - vbar[0], d1 = vel[1][a] - vbar[1], d2 = vel[2][a] - vbar[2];
+        mv[a] = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        const double muc = den * (qa * sspd + qb * mv[a]);
         nb[a] = sqrt(B[3 * a] * B[3 * a] + B[3 * a + 1] * B[3 * a + 1] + B[3 * a + 2] * B[3 * a + 2]);
-        Mt[a] = muc[a] * nb[a];
+        Mt[a] = muc * nb[a];
 #pragma unroll
         for (int d = 0; d < 3; ++d) S[d] += Mt[a] * vel[d][a];
         W += Mt[a];
@@ -622,8 +622,9 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
     double denb = 0.0, sspdb = 0.0, vbarb[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
+        const double muc = den * (qa * sspd + qb * mv[a]);
         const double mucb = Mb[a] * nb[a];
-        const double nbb = Mb[a] * muc[a];
+        const double nbb = Mb[a] * muc;
         if (nb[a] > 0.0) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) Bb[3 * a + d] += nbb * B[3 * a + d] / nb[a];
@@ -634,7 +635,7 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
         if (mv[a] > 0.0) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double db = mvb * dl[a][d] / mv[a];
+                const double db = mvb * (vel[d][a] - vbar[d]) / mv[a];
                 velb[d][a] += db;
                 vbarb[d] -= db;
             }
@@ -658,6 +659,10 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
         extb = rb * 0.9 / bden;
         divb -= rb * r / bden;
     }
+    // reload the coordinates for the B-matrix VJP rather than keep them live
+    // through the reverse sweep (the compiler barrier forbids reusing them)
+    asm volatile("" ::: "memory");
+    g3(s.x1, s.y1, s.z1, nd, x, y, z);
     double xb[8] = {0}, yb[8] = {0}, zb[8] = {0};
     hex_bmatrix_vjp(x, y, z, Bb, xb, yb, zb);
 #pragma unroll
