@@ -21,11 +21,14 @@ NODE_COORDS, NODE_VEL, NODE_MASS = 0, 1, 2
 ELEM_DEN, ELEM_PRES, ELEM_SSPD, ELEM_SIE, ELEM_VOL, ELEM_DIV, ELEM_MASS = 3, 4, 5, 6, 7, 8, 9
 ELEM_MAT_ID, ELEM_GAMMA, ELEM_CSMIN, CORNER_FORCE = 10, 11, 12, 13
 
+EOS_IDEAL_GAS, EOS_USER = 0, 1
+STRENGTH_NONE = 0
+
 # every symbol include/fsgh.h declares
 EXPORTS = [
     "fsgh_create", "fsgh_destroy", "fsgh_last_error", "fsgh_device_count",
-    "fsgh_get_corners_in_node", "fsgh_get_renumbering", "fsgh_set_materials", "fsgh_set_bcs",
-    "fsgh_bc_set_size", "fsgh_get_bc_set", "fsgh_setup_fills", "fsgh_upload", "fsgh_download", "fsgh_rk_init",
+    "fsgh_get_corners_in_node", "fsgh_get_renumbering", "fsgh_get_device_csr", "fsgh_set_materials", "fsgh_set_bcs",
+    "fsgh_bc_set_size", "fsgh_get_bc_set", "fsgh_setup_fills", "fsgh_set_material_models", "fsgh_upload", "fsgh_download", "fsgh_rk_init",
     "fsgh_get_timestep", "fsgh_get_divergence", "fsgh_get_force_sgh",
     "fsgh_update_velocity_sgh", "fsgh_boundary_velocity", "fsgh_update_energy_sgh",
     "fsgh_update_position_sgh", "fsgh_get_vol", "fsgh_update_state", "fsgh_energy",
@@ -91,7 +94,9 @@ def load() -> C.CDLL:
         "fsgh_destroy": [vp],
         "fsgh_get_corners_in_node": [vp, vp, vp],
         "fsgh_get_renumbering": [vp, vp, vp],
+        "fsgh_get_device_csr": [vp, vp, vp],
         "fsgh_set_materials": [vp, i32, vp],
+        "fsgh_set_material_models": [vp, i32, i32, i32],
         "fsgh_set_bcs": [vp, i32, i32, vp],
         "fsgh_bc_set_size": [vp, i32, i32, C.POINTER(i64)],
         "fsgh_get_bc_set": [vp, i32, i32, vp],

@@ -13,7 +13,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfsgh.so")
-SOURCES = ["fsgh_kernels.hip", "fsgh_mesh.hip", "fsgh_api.hip", "fsgh_adjoint.hip"]
+SOURCES = ["fsgh_kernels.hip", "fsgh_mesh.hip", "fsgh_api.hip", "fsgh_adjoint.hip",
+           "fsgh_setup.hip"]
 ARCH = os.environ.get("FSGH_ARCH", "gfx950")
 
 FLAGS = [

@@ -303,7 +303,8 @@ __global__ __launch_bounds__(kB) void k_tlm_energy_state(DMesh m, TLMState s, co
     const Dual sie = energy_update(f, u1, v1, w1, u0, v0, w0, s.e0.ld(e), M, alpha, dt);
     s.e1.st(e, sie);
     Dual vol, den, p, c, div;
-    derived_state(x, y, z, u1, v1, w1, M, sie, s.g.ld(e), s.csmin[e], vol, den, p, c, div);
+    const int32_t kind = s.mats[s.nmat > 1 ? s.mat[e] : 0].eos;
+    derived_state(x, y, z, u1, v1, w1, M, sie, s.g.ld(e), s.csmin[e], kind, vol, den, p, c, div);
     s.vol.st(e, vol);
     s.den.st(e, den);
     s.pres.st(e, p);
@@ -329,7 +330,8 @@ __global__ __launch_bounds__(kB) void k_tlm_setup_elem(DMesh m, TLMState s)
         w[a] = s.v1[2].ld(nd[a]);
     }
     Dual p, c;
-    ideal_gas(s.g.ld(e), s.csmin[e], s.den.ld(e), s.e1.ld(e), p, c);
+    eos(s.mats[s.nmat > 1 ? s.mat[e] : 0].eos, s.g.ld(e), s.csmin[e], s.den.ld(e), s.e1.ld(e), p,
+        c);
     s.pres.st(e, p);
     s.sspd.st(e, c);
     double B[24];
@@ -720,15 +722,17 @@ __global__ __launch_bounds__(kB) void k_adj_derived(DMesh m, DState s, AdjView A
     const double gam = s.gamma[e], csmin = s.csmin[e];
     const double sie = sie_in[e];
     const double den = s.den[e];
-    // EOS (K/eos.cpp:35-43)
-    double eb = presb * (gam - 1.0) * den;
-    double dnb = denb + presb * (gam - 1.0) * sie;
-    double gb = presb * sie * den;
-    const double c = sqrt(gam * (gam - 1.0) * sie);
-    if (!(c < csmin) && c > 0.0) {
-        eb += sspdb * gam * (gam - 1.0) / (2.0 * c);
-        gb += sspdb * (2.0 * gam - 1.0) * sie / (2.0 * c);
-    }
+    // the material's EOS (K/eos.cpp:35-43 for the ideal gas): its partial
+    // derivatives in density, energy and gamma by three dual-number
+    // evaluations, so any model of fsgh::eos is transposed exactly
+    const int32_t kind = s.mats[s.nmat > 1 ? s.mat[e] : 0].eos;
+    Dual pr, cs;
+    eos(kind, Dual(gam), csmin, Dual(den, 1.0), Dual(sie), pr, cs);
+    double dnb = denb + presb * pr.d + sspdb * cs.d;
+    eos(kind, Dual(gam), csmin, Dual(den), Dual(sie, 1.0), pr, cs);
+    const double eb = presb * pr.d + sspdb * cs.d;
+    eos(kind, Dual(gam, 1.0), csmin, Dual(den), Dual(sie), pr, cs);
+    const double gb = presb * pr.d + sspdb * cs.d;
     sieb_in[e] = assign ? eb : sieb_in[e] + eb;
     A.gb[e] += gb;
     // divergence (K/momentum.cpp:265-286): div = (sum_a v_a . B_a) / vol
@@ -906,8 +910,8 @@ __global__ __launch_bounds__(kB) void k_derived(DMesh m, DState s)
     g3(s.x1, s.y1, s.z1, nd, x, y, z);
     g3(s.u1, s.v1, s.w1, nd, u, v, w);
     double vol, den, p, c, div;
-    derived_state(x, y, z, u, v, w, s.emass[e], s.sie1[e], s.gamma[e], s.csmin[e], vol, den, p,
-                  c, div);
+    derived_state(x, y, z, u, v, w, s.emass[e], s.sie1[e], s.gamma[e], s.csmin[e],
+                  s.mats[s.nmat > 1 ? s.mat[e] : 0].eos, vol, den, p, c, div);
     s.vol[e] = vol;
     s.den[e] = den;
     s.pres[e] = p;

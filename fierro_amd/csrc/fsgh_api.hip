@@ -11,7 +11,7 @@
 
 #define FSGH_HD __host__ __device__
 #include "fsgh_ctx.h"
-#include "hex_geometry.inc"
+#include "fsgh_physics.h"
 
 using namespace fsgh;
 
@@ -273,42 +273,24 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
                     return FSGH_E_INVALID;
                 }
             }
-    // numbering
-    if (c->renumber) {
-        morton_renumber(c->L, c->conn.data(), c->coords.data(), c->ne_of_oe, c->nn_of_on);
-    } else {
-        c->ne_of_oe.resize(static_cast<size_t>(c->E));
-        c->nn_of_on.resize(static_cast<size_t>(c->N));
-        std::iota(c->ne_of_oe.begin(), c->ne_of_oe.end(), 0);
-        std::iota(c->nn_of_on.begin(), c->nn_of_on.end(), 0);
-    }
-    c->oe_of_ne.resize(static_cast<size_t>(c->E));
-    c->on_of_nn.resize(static_cast<size_t>(c->N));
-    for (int64_t e = 0; e < c->E; ++e) c->oe_of_ne[static_cast<size_t>(c->ne_of_oe[e])] = static_cast<int32_t>(e);
-    for (int64_t n = 0; n < c->N; ++n) c->on_of_nn[static_cast<size_t>(c->nn_of_on[n])] = static_cast<int32_t>(n);
-    corner_csr(c->N, c->E, c->conn.data(), c->cstart, c->corners);
-
-    // internal connectivity
-    std::vector<int32_t> iconn(static_cast<size_t>(8 * c->E));
-    for (int64_t e = 0; e < c->E; ++e)
-        for (int a = 0; a < 8; ++a)
-            iconn[static_cast<size_t>(8 * c->ne_of_oe[e] + a)] = c->nn_of_on[c->conn[8 * e + a]];
-    std::vector<int32_t> icstart(static_cast<size_t>(c->N + 1)), icin(static_cast<size_t>(8 * c->E));
-    icstart[0] = 0;
-    for (int64_t nn = 0; nn < c->N; ++nn) {
-        const int64_t on = c->on_of_nn[nn];
-        icstart[static_cast<size_t>(nn + 1)] =
-            icstart[static_cast<size_t>(nn)] +
-            static_cast<int32_t>(c->cstart[static_cast<size_t>(on + 1)] - c->cstart[static_cast<size_t>(on)]);
-    }
-    for (int64_t nn = 0; nn < c->N; ++nn) {
-        const int64_t on = c->on_of_nn[nn];
-        int32_t w = icstart[static_cast<size_t>(nn)];
-        // keep the reference's ascending original-element order of the sum
-        for (int64_t q = c->cstart[on]; q < c->cstart[on + 1]; ++q) {
-            const int64_t oc = c->corners[static_cast<size_t>(q)];
-            icin[static_cast<size_t>(w++)] = 8 * c->ne_of_oe[oc >> 3] + static_cast<int32_t>(oc & 7);
+    // Host-only context: numbering and the caller-numbered corner CSR on the
+    // CPU.  A device context builds the numbering and the internal
+    // connectivity / CSR on the GPU (fsgh_setup.hip) once its buffers exist.
+    if (device < 0) {
+        // numbering
+        if (c->renumber) {
+            morton_renumber(c->L, c->conn.data(), c->coords.data(), c->ne_of_oe, c->nn_of_on);
+        } else {
+            c->ne_of_oe.resize(static_cast<size_t>(c->E));
+            c->nn_of_on.resize(static_cast<size_t>(c->N));
+            std::iota(c->ne_of_oe.begin(), c->ne_of_oe.end(), 0);
+            std::iota(c->nn_of_on.begin(), c->nn_of_on.end(), 0);
         }
+        c->oe_of_ne.resize(static_cast<size_t>(c->E));
+        c->on_of_nn.resize(static_cast<size_t>(c->N));
+        for (int64_t e = 0; e < c->E; ++e) c->oe_of_ne[static_cast<size_t>(c->ne_of_oe[e])] = static_cast<int32_t>(e);
+        for (int64_t n = 0; n < c->N; ++n) c->on_of_nn[static_cast<size_t>(c->nn_of_on[n])] = static_cast<int32_t>(n);
+        corner_csr(c->N, c->E, c->conn.data(), c->cstart, c->corners);
     }
     std::vector<BlockItem> etab, ntab;
     block_table(c->L.elem_off, etab, c->eblk_begin);
@@ -368,9 +350,7 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
     rc |= dalloc(c, &c->d_scratch, 2 * static_cast<size_t>(c->nscratch));
     rc |= dalloc(c, &c->d_flag, 1);
     if (rc != FSGH_OK) return FSGH_E_NOMEM;
-    HIPCHK(c, hipMemcpyAsync(c->d_conn, iconn.data(), iconn.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_cin_start, icstart.data(), icstart.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_cin, icin.data(), icin.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if ((rc = gpu_build_maps(c)) != FSGH_OK) return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_eblk, etab.data(), etab.size() * sizeof(BlockItem), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_nblk, ntab.data(), ntab.size() * sizeof(BlockItem), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -394,7 +374,7 @@ int fsgh_destroy(fsgh_ctx* c)
                     c->d_den, c->d_pres, c->d_sspd, c->d_vol, c->d_div[0], c->d_div[1], c->d_ext_lo, c->d_ext_hi, c->d_emass, c->d_gamma,
                     c->d_csmin, c->d_F, c->d_mat, c->d_bcmask, c->d_mats, c->d_ctl, c->d_partial,
                     c->d_scalar, c->d_scratch, c->d_flag, c->d_bcnodes, c->d_bcmasks,
-                    c->d_sie[0], c->d_sie[1]};
+                    c->d_sie[0], c->d_sie[1], c->d_on_of_nn, c->d_oe_of_ne};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (int b = 0; b < 2; ++b)
@@ -409,8 +389,24 @@ int fsgh_get_corners_in_node(fsgh_ctx* c, int64_t* start, int64_t* corners)
 {
     CHECK_CTX(c);
     if (!start || !corners) return FSGH_E_INVALID;
+    // a device context keeps only the internal CSR; the caller-numbered one
+    // is built on first request
+    if (c->cstart.empty()) corner_csr(c->N, c->E, c->conn.data(), c->cstart, c->corners);
     std::copy(c->cstart.begin(), c->cstart.end(), start);
     std::copy(c->corners.begin(), c->corners.end(), corners);
+    return FSGH_OK;
+}
+
+int fsgh_get_device_csr(fsgh_ctx* c, int32_t* start, int32_t* corners)
+{
+    CHECK_CTX(c);
+    if (c->host_only) return FSGH_E_STATE;
+    if (!start || !corners) return FSGH_E_INVALID;
+    HIPCHK(c, hipMemcpyAsync(start, c->d_cin_start, static_cast<size_t>(c->N + 1) * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(corners, c->d_cin, static_cast<size_t>(8 * c->E) * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return FSGH_OK;
 }
 
@@ -430,14 +426,37 @@ int fsgh_set_materials(fsgh_ctx* c, int32_t nmat, const fsgh_material* mats)
     if (c->d_mats) hipFree(c->d_mats);
     c->d_mats = nullptr;
     std::vector<Material> h(static_cast<size_t>(nmat));
-    for (int32_t i = 0; i < nmat; ++i) h[i] = {mats[i].q1, mats[i].q1ex, mats[i].q2, mats[i].q2ex};
+    for (int32_t i = 0; i < nmat; ++i)
+        h[i] = {mats[i].q1, mats[i].q1ex, mats[i].q2, mats[i].q2ex, FSGH_EOS_IDEAL_GAS, 0};
     int rc = dalloc(c, &c->d_mats, h.size());
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_mats, h.data(), h.size() * sizeof(Material), hipMemcpyHostToDevice,
                              c->stream));
+    c->mats = h;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->nmat = nmat;
     drop_graphs(c);
+    return FSGH_OK;
+}
+
+int fsgh_set_material_models(fsgh_ctx* c, int32_t mat, int32_t eos_kind, int32_t strength)
+{
+    CHECK_CTX(c);
+    if (c->host_only) return FSGH_E_STATE;
+    if (c->nmat <= 0 || c->mats.empty()) {
+        set_error(c, "materials not set (fsgh_set_materials)");
+        return FSGH_E_STATE;
+    }
+    if (mat < 0 || mat >= c->nmat) return FSGH_E_INVALID;
+    if (strength != FSGH_STRENGTH_NONE) {
+        set_error(c, "strength models (hypo/hyper-elastic) are not in this build");
+        return FSGH_E_UNSUPPORTED;
+    }
+    if (eos_kind != FSGH_EOS_IDEAL_GAS && eos_kind != FSGH_EOS_USER) return FSGH_E_INVALID;
+    c->mats[static_cast<size_t>(mat)].eos = eos_kind;
+    HIPCHK(c, hipMemcpyAsync(c->d_mats + mat, &c->mats[static_cast<size_t>(mat)], sizeof(Material),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return FSGH_OK;
 }
 
@@ -483,7 +502,12 @@ int fsgh_set_bcs(fsgh_ctx* c, int32_t inst, int32_t nsets, const fsgh_boundary* 
         }
     }
     auto& out = c->bcsets[static_cast<size_t>(inst)];
-    boundary_sets(c->L, inst, c->conn.data(), c->coords.data(), c->cstart, c->corners, nsets, sets, out);
+    if (c->host_only) {
+        boundary_sets(c->L, inst, c->conn.data(), c->coords.data(), c->cstart, c->corners, nsets,
+                      sets, out);
+    } else if (int rc = gpu_boundary_sets(c, inst, nsets, sets, out)) {
+        return rc;
+    }
     for (int64_t n = c->L.node_off[inst]; n < c->L.node_off[inst + 1]; ++n) c->bcmask[static_cast<size_t>(n)] = 0;
     for (int32_t s = 0; s < nsets; ++s) {
         uint8_t bits = 0;
@@ -524,11 +548,21 @@ int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill*
     if (inst < 0 || inst >= c->L.ninst || nfills < 0 || (nfills && !fills) || nmat <= 0 ||
         nsv < 2 || !state_vars || !mat_nsv)
         return FSGH_E_INVALID;
+    if (c->nmat <= 0 || nmat > c->nmat) {
+        set_error(c, "fsgh_setup_fills: materials not set or fewer than num_materials");
+        return FSGH_E_STATE;
+    }
     for (int32_t f = 0; f < nfills; ++f) {
         if (fills[f].mat_id < 0 || fills[f].mat_id >= nmat) return FSGH_E_INVALID;
         if (fills[f].velocity < 0 || fills[f].velocity > 4 || fills[f].volume < 0 || fills[f].volume > 3)
             return FSGH_E_UNSUPPORTED;
     }
+    if (nfills <= 64) {
+        drop_graphs(c);
+        return gpu_setup_fills(c, inst, nfills, fills, nmat, nsv, state_vars, mat_nsv);
+    }
+    // more than 64 fills: the same computation on the host
+    if (c->cstart.empty()) corner_csr(c->N, c->E, c->conn.data(), c->cstart, c->corners);
     const int64_t e0 = c->L.elem_off[inst], e1 = c->L.elem_off[inst + 1];
     const int64_t n0 = c->L.node_off[inst], n1 = c->L.node_off[inst + 1];
     const size_t NE = static_cast<size_t>(e1 - e0), NN = static_cast<size_t>(n1 - n0);
@@ -580,11 +614,9 @@ int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill*
             const int32_t m = F.mat_id;
             if (mat_nsv[m] > 0) gam[k] = state_vars[static_cast<size_t>(m) * nsv + 0];
             if (mat_nsv[m] > 1) csm[k] = state_vars[static_cast<size_t>(m) * nsv + 1];
-            // ideal gas (K/eos.cpp:30-43)
-            pres[k] = (gam[k] - 1.0) * sie[k] * den[k];
-            double cs = std::sqrt(gam[k] * (gam[k] - 1.0) * sie[k]);
-            if (cs < csm[k]) cs = csm[k];
-            sspd[k] = cs;
+            // the material's EOS (K/setup.cpp:225-232)
+            eos(c->mats[static_cast<size_t>(m)].eos, gam[k], csm[k], den[k], sie[k], pres[k],
+                sspd[k]);
             for (int a = 0; a < 8; ++a) {
                 const int64_t n = c->conn[static_cast<size_t>(8 * e + a)];
                 double* v = &vel[3 * static_cast<size_t>(n - n0)];

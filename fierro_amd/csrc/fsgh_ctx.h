@@ -40,7 +40,9 @@ struct fsgh_ctx {
     double *d_csmin = nullptr, *d_F = nullptr;
     int32_t* d_mat = nullptr;
     uint8_t* d_bcmask = nullptr;
+    int32_t *d_on_of_nn = nullptr, *d_oe_of_ne = nullptr;   // internal -> caller ids
     fsgh::Material* d_mats = nullptr;
+    std::vector<fsgh::Material> mats;   // host copy of the material table
     int32_t nmat = 0;
     fsgh::InstCtl* d_ctl = nullptr;
     double* d_partial = nullptr;
@@ -78,6 +80,12 @@ DState make_state(fsgh_ctx* c, int b0, int b1);
 DMesh make_mesh(fsgh_ctx* c);
 void issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record);
 void free_adjoint(fsgh_ctx* c);
+// mesh preparation on the device (fsgh_setup.hip)
+int gpu_build_maps(fsgh_ctx* c);
+int gpu_boundary_sets(fsgh_ctx* c, int32_t inst, int32_t nsets, const fsgh_boundary* bc,
+                      std::vector<std::vector<int64_t>>& out);
+int gpu_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill* fills,
+                    int32_t nmat, int32_t nsv, const double* state_vars, const int32_t* mat_nsv);
 }  // namespace fsgh
 
 #define CHECK_CTX(c)                                   \

@@ -42,6 +42,8 @@ struct InstCtl {
 
 struct Material {
     double q1, q1ex, q2, q2ex;
+    int32_t eos;               // FSGH_EOS_IDEAL_GAS or FSGH_EOS_USER
+    int32_t pad_;
 };
 
 // Device view of the mesh (new numbering, int32 indices).

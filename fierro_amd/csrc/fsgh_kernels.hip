@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_update_state(DMesh m, DState s)
     const double den = s.emass[e] / s.vol[e];
     s.den[e] = den;
     double p, c;
-    ideal_gas(s.gamma[e], s.csmin[e], den, s.sie1[e], p, c);
+    eos(s.mats[s.nmat > 1 ? s.mat[e] : 0].eos, s.gamma[e], s.csmin[e], den, s.sie1[e], p, c);
     s.pres[e] = p;
     s.sspd[e] = c;
 }
@@ -525,7 +525,8 @@ __global__ __launch_bounds__(kBlock) void k_energy_state_div(DMesh m, DState s,
         s.sie1[e] = sie;
         gather3(s.x1, s.y1, s.z1, nd, x, y, z);
         double vol, den, p, cs, div;
-        derived_state(x, y, z, u1, v1, w1, emass, sie, s.gamma[e], s.csmin[e], vol, den, p, cs,
+        const int32_t kind = s.mats[s.nmat > 1 ? s.mat[e] : 0].eos;
+        derived_state(x, y, z, u1, v1, w1, emass, sie, s.gamma[e], s.csmin[e], kind, vol, den, p, cs,
                       div);
         s.vol[e] = vol;
         s.den[e] = den;

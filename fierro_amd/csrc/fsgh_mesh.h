@@ -14,6 +14,31 @@ struct Layout {
     std::vector<int64_t> node_off, elem_off;
 };
 
+#ifndef FSGH_HD
+#define FSGH_HD __host__ __device__
+#endif
+
+// interleave the low 21 bits of v with two zero bits between each
+FSGH_HD inline uint64_t morton_spread3(uint32_t v)
+{
+    uint64_t x = v & 0x1fffffu;
+    x = (x | x << 32) & 0x1f00000000ffffULL;
+    x = (x | x << 16) & 0x1f0000ff0000ffULL;
+    x = (x | x << 8) & 0x100f00f00f00f00fULL;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+    x = (x | x << 2) & 0x1249249249249249ULL;
+    return x;
+}
+
+// a centroid coordinate in [lo, hi] quantised to 21 bits
+FSGH_HD inline uint32_t morton_quantize(double c, double lo, double hi)
+{
+    if (!(hi > lo)) return 0;
+    double t = (c - lo) / (hi - lo);
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    return static_cast<uint32_t>(t * 2097151.0);
+}
+
 // Morton renumbering inside each instance.  new_of_old_elem / new_of_old_node
 // map caller ids to internal ids; old_of_new_* are their inverses.
 void morton_renumber(const Layout& L, const int64_t* nodes_in_elem, const double* coords,

@@ -9,26 +9,6 @@ namespace fsgh {
 
 namespace {
 
-// interleave the low 21 bits of v with two zero bits between each
-inline uint64_t spread3(uint32_t v)
-{
-    uint64_t x = v & 0x1fffffu;
-    x = (x | x << 32) & 0x1f00000000ffffULL;
-    x = (x | x << 16) & 0x1f0000ff0000ffULL;
-    x = (x | x << 8) & 0x100f00f00f00f00fULL;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
-    x = (x | x << 2) & 0x1249249249249249ULL;
-    return x;
-}
-
-inline uint32_t quantize(double c, double lo, double hi)
-{
-    if (!(hi > lo)) return 0;
-    double t = (c - lo) / (hi - lo);
-    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
-    return static_cast<uint32_t>(t * 2097151.0);
-}
-
 // node lids of the six faces of a hex (K/mesh.h:353-358)
 const int kFace[6][4] = {{0, 4, 7, 3}, {1, 2, 6, 5}, {0, 1, 5, 4},
                          {2, 3, 7, 6}, {0, 3, 2, 1}, {4, 5, 6, 7}};
@@ -57,7 +37,7 @@ void morton_renumber(const Layout& L, const int64_t* conn, const double* xyz,
             for (int a = 0; a < 8; ++a)
                 for (int d = 0; d < 3; ++d) c[d] += xyz[3 * conn[8 * e + a] + d];
             uint64_t code = 0;
-            for (int d = 0; d < 3; ++d) code |= spread3(quantize(c[d] / 8.0, lo[d], hi[d])) << d;
+            for (int d = 0; d < 3; ++d) code |= morton_spread3(morton_quantize(c[d] / 8.0, lo[d], hi[d])) << d;
             key[static_cast<size_t>(e - e0)] = {code, static_cast<int32_t>(e)};
         }
         std::sort(key.begin(), key.end());

@@ -12,6 +12,7 @@
 #define FSGH_HD __host__ __device__
 #endif
 #include "fsgh_dual.h"
+#include "fsgh_user_eos.h"
 #include "hex_geometry.inc"
 
 namespace fsgh {
@@ -43,6 +44,19 @@ FSGH_HD inline void ideal_gas(T gamma, double csmin, T den, T sie, T& pres, T& s
     T c = sqrt(gamma * (gamma - 1.0) * sie);
     if (c < csmin) c = csmin;
     sspd = c;
+}
+
+
+// the material's equation of state (the eos_model pointer of K/state.h:140-147)
+constexpr int32_t kEosIdealGas = 0;
+constexpr int32_t kEosUser = 1;
+template <typename T>
+FSGH_HD inline void eos(int32_t kind, T gamma, double csmin, T den, T sie, T& pres, T& sspd)
+{
+    if (kind == kEosUser)
+        user_eos(gamma, csmin, den, sie, pres, sspd);
+    else
+        ideal_gas(gamma, csmin, den, sie, pres, sspd);
 }
 
 // Corner forces of one element, 3D, strength-free (K/force_sgh.cpp:35-359).
@@ -143,18 +157,19 @@ FSGH_HD inline T energy_update(const T* f, const T* u1, const T* v1, const T* w1
 }
 
 // Element state from coordinates, velocities, mass and energy: volume
-// (K/geometry.cpp:298-306), density and ideal-gas EOS (K/properties.cpp:39,99),
-// divergence (K/momentum.cpp:229-287).
+// (K/geometry.cpp:298-306), density and the material's EOS
+// (K/properties.cpp:39,99), divergence (K/momentum.cpp:229-287).
 template <typename T>
 FSGH_HD inline void derived_state(const T* x, const T* y, const T* z, const T* u, const T* v,
-                                  const T* w, T mass, T sie, T gamma, double csmin, T& vol,
-                                  T& den, T& pres, T& sspd, T& div)
+                                  const T* w, T mass, T sie, T gamma, double csmin,
+                                  int32_t eos_kind, T& vol, T& den, T& pres, T& sspd, T& div)
 {
     T B[24];
     vol = hex_bmatrix_volume(x, y, z, B);
     den = mass / vol;
-    ideal_gas(gamma, csmin, den, sie, pres, sspd);
     div = divergence(B, u, v, w, vol);
+    // the EOS last: the B matrix and velocities are dead by then
+    eos(eos_kind, gamma, csmin, den, sie, pres, sspd);
 }
 
 }  // namespace fsgh

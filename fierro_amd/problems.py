@@ -24,6 +24,7 @@ GLOBAL, BOX, CYLINDER, SPHERE = 0, 1, 2, 3
 CARTESIAN, RADIAL, SPHERICAL = 0, 1, 2
 X_PLANE, Y_PLANE, Z_PLANE = 0, 1, 2
 FIXED, REFLECTED = 0, 1
+EOS_IDEAL_GAS, EOS_USER = 0, 1      # the material models of K/state.h:140-170
 
 # K/driver.cpp:35-62 defaults
 DEFAULT_RUN = dict(time_final=1.0e16, dt_start=1.0e-8, dt_max=1.0e-2, dt_min=1.0e-8,
@@ -50,6 +51,7 @@ class ProblemSpec:
     bcs: list                       # [(surface, value, hydro_bc)]
     run: dict = field(default_factory=lambda: dict(DEFAULT_RUN))
     num_state_vars: int = 6
+    eos: list = None                # per material: EOS_IDEAL_GAS (default) or EOS_USER
 
     def mesh_arrays(self, oracle_gen=None, oracle_read=None):
         """(coords N x 3, nodes_in_elem E x 8).  The oracle passes its own
@@ -168,6 +170,21 @@ def blast_ensemble(count=64, seed=0, cycles=150):
         params.append((n, E0, gamma, 1.0))
     specs = [blast(n, E0, g, r, cycles=cycles) for n, E0, g, r in params]
     return specs[:count], params[:count], specs[count], params[count]
+
+
+def sedov_user_region(n=8, cycles=100):
+    """``sedov`` with a second material in the box [0.6,1.2] x [0,1.2] x
+    [0,1.2] whose equation of state is the user model (material plugin hook,
+    K/state.h:140-170, K/user_mat.cpp:14-40); same q coefficients."""
+    spec = sedov(n, cycles=cycles)
+    spec.name = f"sedov{n}_user"
+    spec.materials = [spec.materials[0], spec.materials[0]]
+    spec.state_vars = _gas(5.0 / 3.0, nmat=2)
+    spec.mat_num_state_vars = [3, 3]
+    spec.fills = spec.fills + [fill(BOX, mat_id=1, den=2.0, sie=1.0e-3,
+                                    x1=0.6, x2=1.2, y1=0.0, y2=1.2, z1=0.0, z2=1.2)]
+    spec.eos = [EOS_IDEAL_GAS, EOS_USER]
+    return spec
 
 def reference_sedov(mesh_path):
     """The reference's default compiled-in problem (K/input.cpp:69-202 with

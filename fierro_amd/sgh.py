@@ -93,6 +93,12 @@ class SGHSolver:
         arr = (_abi.Material * len(mats))(*[_abi.Material(*m) for m in mats])
         check(self.L.fsgh_set_materials(self.ctx, len(mats), arr), self.ctx, "set_materials")
         for i, s in enumerate(self.specs):
+            for k, kind in enumerate(s.eos or []):
+                if kind != _abi.EOS_IDEAL_GAS:
+                    check(self.L.fsgh_set_material_models(self.ctx, index[i] + k, int(kind),
+                                                          _abi.STRENGTH_NONE),
+                          self.ctx, "set_material_models")
+        for i, s in enumerate(self.specs):
             bcs = (_abi.Boundary * max(len(s.bcs), 1))(
                 *[_abi.Boundary(int(b[0]), int(b[2]), float(b[1])) for b in s.bcs])
             check(self.L.fsgh_set_bcs(self.ctx, i, len(s.bcs), bcs), self.ctx, "set_bcs")
@@ -154,6 +160,20 @@ class SGHSolver:
         n = np.zeros(self.N, np.int64)
         check(self.L.fsgh_get_renumbering(self.ctx, _p(e), _p(n)), self.ctx, "renumbering")
         return e, n
+
+    def device_csr(self):
+        """The device's corner CSR in internal numbering (diagnostic)."""
+        start = np.zeros(self.N + 1, np.int32)
+        corners = np.zeros(8 * self.E, np.int32)
+        check(self.L.fsgh_get_device_csr(self.ctx, _p(start), _p(corners)), self.ctx, "device_csr")
+        return start, corners
+
+    def bc_set(self, inst, s):
+        c = C.c_int64()
+        check(self.L.fsgh_bc_set_size(self.ctx, inst, s, C.byref(c)), self.ctx, "bc size")
+        out = np.zeros(c.value, np.int64)
+        check(self.L.fsgh_get_bc_set(self.ctx, inst, s, _p(out)), self.ctx, "bc set")
+        return out
 
     def bc_set_sizes(self, inst=0):
         out = []

@@ -86,14 +86,31 @@ int fsgh_get_corners_in_node(fsgh_ctx *ctx, int64_t *start, int64_t *corners);
 /* Morton permutations: new_of_old_elem[E], new_of_old_node[N] (identity when
  * renumber == 0). */
 int fsgh_get_renumbering(fsgh_ctx *ctx, int64_t *new_of_old_elem, int64_t *new_of_old_node);
+/* Diagnostic: the device's corner CSR in internal numbering (start [N+1],
+ * corner ids 8 * internal element + local node [8E]); each node's list is in
+ * ascending caller element order.  Device contexts only. */
+int fsgh_get_device_csr(fsgh_ctx *ctx, int32_t *start, int32_t *corners);
 
 /* ------------------------------------------------------------ materials --- */
 typedef struct fsgh_material {
     double q1, q1ex, q2, q2ex;         /* K/state.h:174-177 */
 } fsgh_material;
 /* EOS is the ideal gas (K/eos.cpp:12-46) with gamma = statev[e][0] and
- * min sound speed = statev[e][1]. */
+ * min sound speed = statev[e][1], unless fsgh_set_material_models selects
+ * another model for a material. */
 int fsgh_set_materials(fsgh_ctx *ctx, int32_t num_materials, const fsgh_material *mats);
+/* Material model hooks: the reference's per-material eos_model and
+ * strength_model pointers (K/state.h:140-170, set in K/input.cpp:115-124).
+ * eos: FSGH_EOS_IDEAL_GAS (K/eos.cpp:12-46) or FSGH_EOS_USER, the device
+ * function fsgh::user_eos of fierro_amd/csrc/fsgh_user_eos.h compiled into
+ * the library (the counterpart of user_eos_model, K/user_mat.cpp:14-40).
+ * strength: FSGH_STRENGTH_NONE only (hypo/hyper-elastic models:
+ * FSGH_E_UNSUPPORTED).  Call after fsgh_set_materials and before
+ * fsgh_setup_fills. */
+#define FSGH_EOS_IDEAL_GAS 0
+#define FSGH_EOS_USER 1
+#define FSGH_STRENGTH_NONE 0
+int fsgh_set_material_models(fsgh_ctx *ctx, int32_t material, int32_t eos, int32_t strength);
 
 /* ----------------------------------------------------- boundary sets --- */
 typedef struct fsgh_boundary {

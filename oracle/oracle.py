@@ -37,7 +37,7 @@ class OrcState(C.Structure):
                 ("elem_den", P), ("elem_pres", P), ("elem_sspd", P),
                 ("elem_sie", P), ("elem_vol", P), ("elem_div", P),
                 ("elem_mass", P), ("elem_mat_id", P), ("elem_statev", P),
-                ("num_state_vars", C.c_int64), ("corner_force", P)]
+                ("num_state_vars", C.c_int64), ("corner_force", P), ("mat_eos", P)]
 
 
 class OrcMaterial(C.Structure):
@@ -215,8 +215,11 @@ class OracleProblem:
                                _ptr(self.elem_den), _ptr(self.elem_pres), _ptr(self.elem_sspd),
                                _ptr(self.elem_sie), _ptr(self.elem_vol), _ptr(self.elem_div),
                                _ptr(self.elem_mass), _ptr(self.elem_mat_id),
-                               _ptr(self.elem_statev), nsv, _ptr(self.corner_force))
+                               _ptr(self.elem_statev), nsv, _ptr(self.corner_force), None)
         mats = spec.materials
+        if spec.eos:
+            self.mat_eos = np.ascontiguousarray(spec.eos, np.int64)
+            self._state.mat_eos = _ptr(self.mat_eos)
         self._mat = (OrcMaterial * len(mats))(*[OrcMaterial(*m) for m in mats])
         self.setup()
 

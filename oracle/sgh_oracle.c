@@ -335,9 +335,17 @@ void orc_update_position_sgh(const orc_mesh *m, orc_state *s, double rk_alpha, d
     }
 }
 
-/* ideal_gas (K/eos.cpp:12-46) */
+/* The element's EOS, selected by its material (the eos_model pointer,
+ * K/state.h:140-147): ideal_gas (K/eos.cpp:12-46) or the user model
+ * user_eos_model (K/user_mat.cpp:14-40), which with the strength-free stress
+ * gives p = -tr(stress)/3 = 0 and c = 1e-8. */
 static void ideal_gas(orc_state *s, int64_t e, double den, double sie)
 {
+    if (s->mat_eos && s->mat_eos[s->elem_mat_id[e]] == 1) {
+        s->elem_pres[e] = 0.0; /* -(stress_00 + stress_11 + stress_22) / 3 with zero stress */
+        s->elem_sspd[e] = 1.0e-8;
+        return;
+    }
     double gamma = s->elem_statev[e * s->num_state_vars + 0];
     double csmin = s->elem_statev[e * s->num_state_vars + 1];
     s->elem_pres[e] = (gamma - 1.0) * sie * den;

@@ -57,6 +57,7 @@ typedef struct orc_state {
     double *elem_statev;   /* E x num_state_vars; col 0 = gamma, col 1 = min sound speed */
     int64_t num_state_vars;
     double *corner_force;  /* 8E x 3 */
+    const int64_t *mat_eos; /* per material: 0 ideal gas, 1 user model; NULL = all ideal gas */
 } orc_state;
 
 /* material_t (K/state.h:130-178), ideal-gas EOS, no strength model */

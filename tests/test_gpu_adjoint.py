@@ -117,9 +117,11 @@ def test_dt_history_matches_oracle(oracle, Solver):
     assert np.allclose(g.dt_history()[:, 0], ro["dt_history"], rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("case", ["anvil", "sedov"])
+@pytest.mark.parametrize("case", ["anvil", "sedov", "sedov_user"])
 def test_dot_product(Solver, case):
-    spec = P.anvil(4, 4, 12, cycles=20) if case == "anvil" else P.sedov(6, cycles=20)
+    spec = {"anvil": lambda: P.anvil(4, 4, 12, cycles=20),
+            "sedov": lambda: P.sedov(6, cycles=20),
+            "sedov_user": lambda: P.sedov_user_region(6, cycles=20)}[case]()
     g = Solver(spec)
     g.forward(20, checkpoint_every=7)
     rng = np.random.default_rng(3)

@@ -52,6 +52,7 @@ CASES = {
     "sedov8": lambda: P.sedov(8),
     "sedov12": lambda: P.sedov(12),
     "sod40": lambda: P.sod_tube(40, 0.125, 0.1, 1.4),
+    "sedov8_user": lambda: P.sedov_user_region(8),
 }
 
 
@@ -136,6 +137,35 @@ def test_fused_cycles_sedov(oracle, Solver, use_graph):
     assert l1 <= 1e-9, l1
     for k in STATE:
         assert rel(g.download(k), oracle_fields(pr)[k]) <= 1e-9, k
+
+
+def test_fused_cycles_user_eos_region(oracle, Solver):
+    """Two materials, one with the user EOS hook (K/user_mat.cpp:14-40): the
+    fused loop against the oracle after 60 cycles."""
+    spec = P.sedov_user_region(10, cycles=60)
+    pr = oracle.OracleProblem(spec)
+    ro = pr.solve()
+    g = Solver(spec, renumber=True)
+    rg = g.sgh_solve(use_graph=True)[0]
+    assert rg["cycles"] == ro["cycles"] == 60
+    assert abs(rg["te1"] - ro["te1"]) <= 1e-12 * abs(ro["te1"])
+    mat = g.download("elem_mat_id")
+    assert set(np.unique(mat)) == {0, 1}
+    sspd = g.download("elem_sspd")
+    assert np.all(sspd[mat == 1] == 1.0e-8)
+    assert np.all(g.download("elem_pres")[mat == 1] == 0.0)
+    for k in STATE:
+        assert rel(g.download(k), oracle_fields(pr)[k]) <= 1e-9, k
+
+
+def test_material_model_errors(Solver):
+    """Strength models are outside this build; bad ids are rejected."""
+    from fierro_amd import _abi
+    g = Solver(P.sedov(4, cycles=1))
+    assert g.L.fsgh_set_material_models(g.ctx, 0, _abi.EOS_IDEAL_GAS, 1) == -6
+    assert g.L.fsgh_set_material_models(g.ctx, 5, _abi.EOS_USER, 0) == -1
+    assert g.L.fsgh_set_material_models(g.ctx, 0, 7, 0) == -1
+    assert g.L.fsgh_set_material_models(g.ctx, 0, _abi.EOS_USER, 0) == 0
 
 
 @pytest.mark.parametrize("case", ["noh", "anvil"])

@@ -297,3 +297,21 @@ def test_sedov_setup_and_first_dt(oracle):
     c0 = np.sqrt(g * (g - 1) * 0.25 / h ** 3)
     dt = pr.get_timestep(0.0, 1e16, 1e-5)
     assert dt == pytest.approx(min(1.1e-5, 0.5 * h / c0), rel=1e-14)
+
+
+def test_user_eos_region(oracle):
+    """The user EOS hook (K/user_mat.cpp:14-40, strength-free: p = 0,
+    c = 1e-8) applies to material 1 only, at set-up and in update_state; the
+    ideal-gas material keeps p = (gamma-1) rho e."""
+    spec = P.sedov_user_region(6, cycles=1)
+    pr = oracle.OracleProblem(spec)
+    mat = pr.elem_mat_id
+    assert set(np.unique(mat)) == {0, 1}
+    for _ in range(2):
+        assert np.all(pr.elem_pres[mat == 1] == 0.0)
+        assert np.all(pr.elem_sspd[mat == 1] == 1.0e-8)
+        g = pr.elem_statev[:, 0]
+        ok = mat == 0
+        assert np.allclose(pr.elem_pres[ok], (g[ok] - 1) * pr.elem_sie[1][ok] * pr.elem_den[ok],
+                           rtol=1e-15, atol=0)
+        pr.stage(1e-4, 0.5)
