@@ -76,6 +76,8 @@ def main():
                     choices=["sedov", "sod-ensemble", "anvil-adjoint", "calibration"],
                     default="sedov")
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--tune", choices=["auto", "default"], default="auto",
+                    help="auto: pick the kernels' launch variants with fsgh_autotune first")
     ap.add_argument("--loop-sample", type=int, default=8,
                     help="time the per-instance loop on every k-th instance")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -103,6 +105,7 @@ def main():
     t0 = time.perf_counter()
     g = SGHSolver(spec, renumber=True)
     log(f"setup {n}^3: E={g.E} N={g.N} in {time.perf_counter() - t0:.1f} s")
+    tuning = tune(g, args)
     g.run_begin()
     g.run(args.warmup, use_graph=True)
     log(f"warmup {args.warmup} cycles done")
@@ -158,6 +161,7 @@ def main():
         "cycle_alg_GBs": cyc_bytes * args.steps / wall / 1e9,
         "value_graph": E * args.steps / graph_s,
         "energy": {"ie": ie, "ke": ke, "time": st["time_value"]},
+        "tuning": tuning,
     }
     del g
     if not args.no_cpu:
@@ -169,6 +173,19 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def tune(g, args):
+    """Autotune the fused-cycle kernels' launch variants (outside the timed
+    region; the state is unchanged) and return the choice per kernel."""
+    if args.tune == "default":
+        return g.get_tuning()
+    t = time.perf_counter()
+    res = g.autotune(reps=5)
+    log(f"autotune in {time.perf_counter() - t:.1f} s: " + ", ".join(
+        f"{k} {v['block']}/{v['min_waves']}/{v['xcd_order']} {v['ms']:.3f} ms"
+        f" (default {v['ms_default']:.3f})" for k, v in res.items()))
+    return res
 
 
 def copy_calibration(mib, reps=3):
@@ -205,6 +222,7 @@ def ensemble(args):
     g = SGHSolver(specs, renumber=True)
     log(f"ensemble setup: {len(specs)} instances E={g.E} N={g.N} in "
         f"{time.perf_counter() - t0:.1f} s")
+    tuning = tune(g, args)
     g.run_begin()
     g.run(args.warmup, use_graph=True)
     g.profile(True)
@@ -230,6 +248,8 @@ def ensemble(args):
     loop_zc, loop_s = 0, 0.0
     for s in specs[::args.loop_sample]:
         p = SGHSolver(s, renumber=True)
+        p.set_tuning({k: {q: v[q] for q in ("block", "min_waves", "xcd_order")}
+                      for k, v in tuning.items()})
         p.run_begin()
         p.run(args.warmup, use_graph=True)
         t = time.perf_counter()
@@ -253,6 +273,7 @@ def ensemble(args):
         "loop_plain": {"value": loop, "unit": "zone-cycles/s",
                        "sample": f"every {args.loop_sample}th instance, hipGraph per cycle"},
         "batched_over_loop": value / loop,
+        "tuning": tuning,
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -35,7 +35,12 @@ EXPORTS = [
     "fsgh_run_begin", "fsgh_run", "fsgh_run_status", "fsgh_profile_enable",
     "fsgh_profile_read", "fsgh_check", "fsgh_stream",
     "fsgh_forward", "fsgh_dt_history", "fsgh_backward", "fsgh_tangent",
+    "fsgh_autotune", "fsgh_set_tuning", "fsgh_get_tuning",
 ]
+
+KERNEL_DIV_EXTREMES, KERNEL_FORCE, KERNEL_VEL_BC_POS, KERNEL_ENERGY_STATE_DIV = 0, 1, 2, 3
+NUM_TUNED_KERNELS = 4
+TUNED_KERNEL_NAMES = ["div_extremes", "force", "vel_bc_pos", "energy_state_div"]
 
 
 class MeshDesc(C.Structure):
@@ -68,6 +73,11 @@ class RunParams(C.Structure):
                 ("dt_max", C.c_double), ("dt_min", C.c_double), ("dt_cfl", C.c_double),
                 ("graphics_dt_ival", C.c_double), ("graphics_cyc_ival", C.c_int64),
                 ("cycle_stop", C.c_int64), ("fuzz", C.c_double), ("small", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("block", C.c_int32), ("min_waves", C.c_int32),
+                ("xcd_order", C.c_int32), ("ms", C.c_double), ("ms_default", C.c_double)]
 
 
 class InstanceStatus(C.Structure):
@@ -124,6 +134,9 @@ def load() -> C.CDLL:
         "fsgh_dt_history": [vp, vp, i64],
         "fsgh_backward": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "fsgh_tangent": [vp, vp, vp, vp, vp, vp, vp, vp],
+        "fsgh_autotune": [vp, i32, C.POINTER(Tuning)],
+        "fsgh_set_tuning": [vp, C.POINTER(Tuning), i32],
+        "fsgh_get_tuning": [vp, C.POINTER(Tuning), i32],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -1053,8 +1053,8 @@ int init_ctl(fsgh_ctx* c, const fsgh_run_params* p)
         k.cycle_stop = q.cycle_stop;
         k.graphics_cyc_ival = q.graphics_cyc_ival;
         k.final_bin = c->cur;
-        k.eblk_begin = c->eblk_begin[static_cast<size_t>(i)];
-        k.eblk_end = c->eblk_begin[static_cast<size_t>(i) + 1];
+        k.eblk_begin = dt_block_begin(c)[static_cast<size_t>(i)];
+        k.eblk_end = dt_block_begin(c)[static_cast<size_t>(i) + 1];
     }
     if (hipMemcpyAsync(c->d_ctl, h.data(), h.size() * sizeof(InstCtl), hipMemcpyHostToDevice,
                        c->stream) != hipSuccess)
@@ -1174,6 +1174,8 @@ void recompute_segment(fsgh_ctx* c, AdjointData::Bank& B, const double* chk, int
 {
     AdjointData& D = *c->adj;
     DMesh m = make_mesh(c);
+    const DMesh mx = tuned_mesh(c, kTunedDivExtremes), mf = tuned_mesh(c, kTunedForce);
+    const DMesh mv = tuned_mesh(c, kTunedVelBcPos), me = tuned_mesh(c, kTunedEnergy);
     {
         // the checkpoint and the segment's time steps into the bank
         SegList L{};
@@ -1208,8 +1210,8 @@ void recompute_segment(fsgh_ctx* c, AdjointData::Bank& B, const double* chk, int
                            static_cast<const double*>(D.dt));
         // stage 0 (alpha 1/2): S_c -> sta[j]; element state -> eb[j]
         DState f0 = view(c, B.seg[j], B.seg[j], S0, B.Fa[j]);
-        launch_div_extremes(m, f0, c->d_ctl, st);
-        launch_force(m, f0, c->d_ctl, 0.0, 0.0, st);
+        launch_div_extremes(mx, f0, c->d_ctl, st);
+        launch_force(mf, f0, c->d_ctl, 0.0, 0.0, st);
         if (!B.x0.empty()) {
             const ExtPtr x = ext_ptr(D.N, B.x0[j]);
             hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), dim3(kB), 0, st, m,
@@ -1218,12 +1220,12 @@ void recompute_segment(fsgh_ctx* c, AdjointData::Bank& B, const double* chk, int
                                static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
         }
         DState u0 = view(c, B.seg[j], B.sta[j], EB, B.Fa[j]);
-        launch_vel_bc_pos(m, u0, c->d_ctl, 0.5, st);
-        launch_energy_state_div(m, u0, c->d_ctl, 0.5, nullptr, st);
+        launch_vel_bc_pos(mv, u0, c->d_ctl, 0.5, st);
+        launch_energy_state_div(me, u0, c->d_ctl, 0.5, nullptr, st);
         // stage 1 (alpha 1): sta[j] -> seg[j+1]; element state -> ea[j+1]
         DState f1 = view(c, B.sta[j], B.sta[j], EB, B.Fb1[j]);
-        launch_div_extremes(m, f1, c->d_ctl, st);
-        launch_force(m, f1, c->d_ctl, 0.0, 0.0, st);
+        launch_div_extremes(mx, f1, c->d_ctl, st);
+        launch_force(mf, f1, c->d_ctl, 0.0, 0.0, st);
         if (!B.x1.empty()) {
             const ExtPtr x = ext_ptr(D.N, B.x1[j]);
             hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), dim3(kB), 0, st, m,
@@ -1232,8 +1234,8 @@ void recompute_segment(fsgh_ctx* c, AdjointData::Bank& B, const double* chk, int
                                static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
         }
         DState u1 = view(c, B.seg[j], B.seg[j + 1], EA1, B.Fb1[j]);
-        launch_vel_bc_pos(m, u1, c->d_ctl, 1.0, st);
-        launch_energy_state_div(m, u1, c->d_ctl, 1.0, nullptr, st);
+        launch_vel_bc_pos(mv, u1, c->d_ctl, 1.0, st);
+        launch_energy_state_div(me, u1, c->d_ctl, 1.0, nullptr, st);
     }
 }
 
@@ -1527,7 +1529,9 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
     {
         DState s = make_state(c, 1 - c->cur, c->cur);
         launch_get_divergence(m, s, c->stream);
-        if (!dt_replay) launch_dt_partial(m, s, c->d_ctl, c->d_partial, 0, 0, 0, 0, 1, c->stream);
+        if (!dt_replay)
+            launch_dt_partial(tuned_mesh(c, kTunedEnergy), s, c->d_ctl, c->d_partial, 0, 0, 0, 0,
+                              1, c->stream);
     }
     for (int64_t k = 0; k < ncycles; ++k) {
         if (k % checkpoint_every == 0)

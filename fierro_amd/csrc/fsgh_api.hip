@@ -55,6 +55,8 @@ DMesh fsgh::make_mesh(fsgh_ctx* c)
     m.nblk = c->d_nblk;
     m.n_eblk = c->n_eblk;
     m.n_nblk = c->n_nblk;
+    m.bs = kBlock;
+    m.minw = 1;
     return m;
 }
 
@@ -346,6 +348,7 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
     rc |= dalloc(c, &c->d_bcmask, N);
     rc |= dalloc(c, &c->d_ctl, static_cast<size_t>(c->L.ninst));
     rc |= dalloc(c, &c->d_partial, etab.size());
+    c->partial_cap = std::max<size_t>(etab.size(), 1);
     rc |= dalloc(c, &c->d_scalar, 4);
     rc |= dalloc(c, &c->d_scratch, 2 * static_cast<size_t>(c->nscratch));
     rc |= dalloc(c, &c->d_flag, 1);
@@ -377,6 +380,8 @@ int fsgh_destroy(fsgh_ctx* c)
                     c->d_sie[0], c->d_sie[1], c->d_on_of_nn, c->d_oe_of_ne};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    for (auto& kv : c->tables)
+        if (kv.second.d) hipFree(kv.second.d);
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 6; ++k)
             if (c->d_node[b][k]) hipFree(c->d_node[b][k]);
@@ -951,8 +956,8 @@ int fsgh_run_begin(fsgh_ctx* c, const fsgh_run_params* p)
         k.graphics_cyc_ival = q.graphics_cyc_ival;
         k.stopped = q.cycle_stop == 0 ? 1 : 0;
         k.final_bin = c->cur;
-        k.eblk_begin = c->eblk_begin[static_cast<size_t>(i)];
-        k.eblk_end = c->eblk_begin[static_cast<size_t>(i) + 1];
+        k.eblk_begin = dt_block_begin(c)[static_cast<size_t>(i)];
+        k.eblk_end = dt_block_begin(c)[static_cast<size_t>(i) + 1];
     }
     HIPCHK(c, hipMemcpyAsync(c->d_ctl, h.data(), h.size() * sizeof(InstCtl), hipMemcpyHostToDevice,
                              c->stream));
@@ -962,8 +967,10 @@ int fsgh_run_begin(fsgh_ctx* c, const fsgh_run_params* p)
     // the first stage's divergence comes from the current state
     // (K/sgh_solve.cpp:364); later stages take it from the fused energy kernel
     timed(c, "get_divergence", [&] { launch_get_divergence(m, s, c->stream); });
-    timed(c, "dt_partial",
-          [&] { launch_dt_partial(m, s, c->d_ctl, c->d_partial, 0, 0, 0, 0, 1, c->stream); });
+    timed(c, "dt_partial", [&] {
+        launch_dt_partial(tuned_mesh(c, kTunedEnergy), s, c->d_ctl, c->d_partial, 0, 0, 0, 0, 1,
+                          c->stream);
+    });
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->run_active = true;
     OP_EPILOGUE(c);
@@ -975,7 +982,8 @@ int fsgh_run_begin(fsgh_ctx* c, const fsgh_run_params* p)
 void fsgh::issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record)
 {
     const int nw = 1 - old;
-    DMesh m = make_mesh(c);
+    const DMesh mx = tuned_mesh(c, kTunedDivExtremes), mf = tuned_mesh(c, kTunedForce);
+    const DMesh mv = tuned_mesh(c, kTunedVelBcPos), me = tuned_mesh(c, kTunedEnergy);
     DState st0 = make_state(c, old, old);   // stage 0 force reads the t_n state
     DState st = make_state(c, old, nw);
     DState sd[2] = {st, st};
@@ -991,13 +999,11 @@ void fsgh::issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt
         const double alpha = 1.0 / (2.0 - static_cast<double>(stage));   // K/sgh_solve.cpp:353
         const InstCtl* ctl = c->d_ctl;
         const DState& sf = stage == 0 ? st0 : sd[1];
-        timed(c, "div_extremes", [&] { launch_div_extremes(m, sf, ctl, c->stream); });
-        timed(c, "force", [&] {
-            launch_force(m, sf, ctl, 0.0, 0.0, c->stream);
-        });
-        timed(c, "vel_bc_pos", [&] { launch_vel_bc_pos(m, st, ctl, alpha, c->stream); });
+        timed(c, "div_extremes", [&] { launch_div_extremes(mx, sf, ctl, c->stream); });
+        timed(c, "force", [&] { launch_force(mf, sf, ctl, 0.0, 0.0, c->stream); });
+        timed(c, "vel_bc_pos", [&] { launch_vel_bc_pos(mv, st, ctl, alpha, c->stream); });
         timed(c, "energy_state_div", [&] {
-            launch_energy_state_div(m, sd[stage], ctl, alpha, stage == 1 ? c->d_partial : nullptr,
+            launch_energy_state_div(me, sd[stage], ctl, alpha, stage == 1 ? c->d_partial : nullptr,
                                     c->stream);
         });
     }

@@ -46,6 +46,16 @@ struct fsgh_ctx {
     int32_t nmat = 0;
     fsgh::InstCtl* d_ctl = nullptr;
     double* d_partial = nullptr;
+    size_t partial_cap = 0;                 // doubles in d_partial
+    // launch configuration of the fused-cycle kernels (fsgh_autotune) and the
+    // work tables they need, keyed by (element/node, workgroup size, XCD order)
+    fsgh::KernelCfg tune[fsgh::kNumTuned];
+    struct WorkTable {
+        fsgh::BlockItem* d = nullptr;
+        int32_t n = 0;
+        std::vector<int32_t> inst_begin;
+    };
+    std::map<int32_t, WorkTable> tables;
     double* d_scalar = nullptr;             // 2 doubles
     double* d_scratch = nullptr;            // energy partial sums
     int32_t nscratch = 1024;
@@ -78,6 +88,13 @@ namespace fsgh {
 // shared by the ABI translation units
 DState make_state(fsgh_ctx* c, int b0, int b1);
 DMesh make_mesh(fsgh_ctx* c);
+// the mesh view with the work tables of a tuned kernel's configuration
+DMesh tuned_mesh(fsgh_ctx* c, int kernel);
+// per-instance logical block ranges of the element table that the fused
+// energy kernel (and the dt partial minima) use
+const std::vector<int32_t>& dt_block_begin(fsgh_ctx* c);
+// build (once) the work table of a configuration; FSGH_OK or an error code
+int ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd);
 void issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record);
 void free_adjoint(fsgh_ctx* c);
 // mesh preparation on the device (fsgh_setup.hip)

@@ -55,7 +55,17 @@ struct DMesh {
     const BlockItem* eblk;     // element work items, XCD-ordered
     const BlockItem* nblk;     // node work items, XCD-ordered
     int32_t n_eblk, n_nblk;
+    int32_t bs;                // entries per work item = workgroup size of the tables above
+    int32_t minw;              // occupancy floor (waves per SIMD) of the kernel variant to run
 };
+
+// Launch configuration of one fused-cycle kernel, chosen by fsgh_autotune.
+struct KernelCfg {
+    int32_t bs = kBlock;       // workgroup size (64, 128, 256; 512 for the node kernels)
+    int32_t minw = 1;          // waves-per-SIMD floor asked of the register allocator
+    int32_t xcd = 1;           // 1: consecutive work items dealt to one XCD
+};
+enum TunedKernel { kTunedDivExtremes = 0, kTunedForce, kTunedVelBcPos, kTunedEnergy, kNumTuned };
 
 // Device view of the state (structure of arrays); bin 0 = t_n, bin 1 = current.
 struct DState {
@@ -81,6 +91,8 @@ void set_error(fsgh_ctx* ctx, const std::string& msg);
 namespace fsgh {
 void launch_get_vol(const DMesh&, const DState&, hipStream_t);
 void launch_get_divergence(const DMesh&, const DState&, hipStream_t);
+// the (bs, minw) pairs compiled for a tuned kernel (TunedKernel id)
+bool tuned_variant_ok(int kernel, int32_t bs, int32_t minw);
 void launch_force(const DMesh&, const DState&, const InstCtl*, double fuzz, double small,
                   hipStream_t);
 void launch_div_extremes(const DMesh&, const DState&, const InstCtl*, hipStream_t);

@@ -54,17 +54,18 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// block-wide min over kBlock threads; result valid in thread 0
+// block-wide min over BS threads; result valid in thread 0
+template <int BS = kBlock>
 __device__ __forceinline__ double block_min(double v)
 {
-    __shared__ double red[kBlock / 64];
+    __shared__ double red[BS / 64];
     v = wave_min(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) red[wave] = v;
     __syncthreads();
     double r = red[0];
 #pragma unroll
-    for (int i = 1; i < kBlock / 64; ++i) r = fmin(r, red[i]);
+    for (int i = 1; i < BS / 64; ++i) r = fmin(r, red[i]);
     return r;
 }
 
@@ -111,8 +112,11 @@ __device__ __forceinline__ double elem_dt(double dist_min, double sspd, double d
 // --------------------------------------------------------------------------
 // corner force, 3D, strength-free (K/force_sgh.cpp:35-359)
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_force(DMesh m, DState s, const InstCtl* ctl,
-                                                  double fuzz, double small)
+// BS threads per workgroup (one element each); MINW = waves-per-SIMD floor
+// requested from the register allocator (1 = the compiler's choice)
+template <int BS, int MINW>
+__global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const InstCtl* ctl,
+                                                    double fuzz, double small)
 {
     const BlockItem bi = m.eblk[blockIdx.x];
     if (ctl != nullptr) {
@@ -123,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_force(DMesh m, DState s, const InstC
     const int e = bi.start + static_cast<int>(threadIdx.x);
     // the block's corner forces are staged in LDS (row stride 13 x 16 B, free
     // of bank conflicts) and written out as contiguous 16-byte stores
-    __shared__ double2 sF[kBlock * 13];
+    __shared__ double2 sF[BS * 13];
     if (e < bi.end) {
         int32_t nd[8];
         load_conn(m.conn, e, nd);
@@ -166,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_force(DMesh m, DState s, const InstC
     double2* out = reinterpret_cast<double2*>(s.F + 24 * static_cast<size_t>(bi.start));
 #pragma unroll
     for (int r = 0; r < 12; ++r) {
-        const int idx = r * kBlock + static_cast<int>(threadIdx.x);
+        const int idx = r * BS + static_cast<int>(threadIdx.x);
         if (idx < cnt) out[idx] = sF[(idx / 12) * 13 + idx % 12];
     }
 }
@@ -252,7 +256,8 @@ __device__ __forceinline__ void insert_max2(double& a, double& b, double v)
 // around it, counted with multiplicity (feeds the shock detector).  A NaN
 // divergence is left out, as fmin/fmax leave it out of the reference's
 // neighbour minimum.
-__global__ __launch_bounds__(kBlock) void k_div_extremes(DMesh m, DState s, const InstCtl* ctl)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_div_extremes(DMesh m, DState s, const InstCtl* ctl)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
     if (ctl != nullptr && ctl[bi.inst].stopped) return;
@@ -384,10 +389,11 @@ __global__ __launch_bounds__(kBlock) void k_rk_init_elems(DMesh m, DState s)
 }
 
 // per-element CFL dt, block minimum into partial[logical block]
-__global__ __launch_bounds__(kBlock) void k_dt_partial(DMesh m, DState s, const InstCtl* ctl,
-                                                       double* partial, double dt_cfl,
-                                                       double dt_max, double dt_min, double fuzz,
-                                                       int use_ctl)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_dt_partial(DMesh m, DState s, const InstCtl* ctl,
+                                                   double* partial, double dt_cfl,
+                                                   double dt_max, double dt_min, double fuzz,
+                                                   int use_ctl)
 {
     const BlockItem bi = m.eblk[blockIdx.x];
     const int e = bi.start + static_cast<int>(threadIdx.x);
@@ -407,7 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_dt_partial(DMesh m, DState s, const 
         gather3(s.x1, s.y1, s.z1, nd, x, y, z);
         d = elem_dt(min_pair_distance(x, y, z), s.sspd[e], cfl, dmax, dmin, fz);
     }
-    d = block_min(d);
+    d = block_min<BS>(d);
     if (threadIdx.x == 0) partial[bi.lid] = d;
 }
 
@@ -474,8 +480,9 @@ __global__ __launch_bounds__(1024) void k_cycle_dt(InstCtl* ctl, int32_t ninst,
 
 // velocity + boundary + position, node-centric (K/momentum.cpp:23-49,
 // K/boundary.cpp:16-55, K/geometry.cpp:17-24)
-__global__ __launch_bounds__(kBlock) void k_vel_bc_pos(DMesh m, DState s, const InstCtl* ctl,
-                                                       double alpha)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_vel_bc_pos(DMesh m, DState s, const InstCtl* ctl,
+                                                   double alpha)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
     const InstCtl& c = ctl[bi.inst];
@@ -503,9 +510,10 @@ __global__ __launch_bounds__(kBlock) void k_vel_bc_pos(DMesh m, DState s, const 
 // energy + volume + EOS + next divergence (+ next cycle's dt candidates),
 // element-centric (K/energy_sgh.cpp:16-52, K/geometry.cpp:255-306,
 // K/properties.cpp:30-106, K/momentum.cpp:229-287, K/time_integration.cpp:74-140)
-__global__ __launch_bounds__(kBlock) void k_energy_state_div(DMesh m, DState s,
-                                                             const InstCtl* ctl, double alpha,
-                                                             double* dt_partial)
+template <int BS, int MINW>
+__global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s,
+                                                               const InstCtl* ctl, double alpha,
+                                                               double* dt_partial)
 {
     const BlockItem bi = m.eblk[blockIdx.x];
     const InstCtl& c = ctl[bi.inst];
@@ -537,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_energy_state_div(DMesh m, DState s,
             dtl = elem_dt(min_pair_distance(x, y, z), cs, c.dt_cfl, c.dt_max, c.dt_min, c.fuzz);
     }
     if (dt_partial != nullptr) {
-        dtl = block_min(dtl);
+        dtl = block_min<BS>(dtl);
         if (threadIdx.x == 0) dt_partial[bi.lid] = dtl;
     }
 }
@@ -656,16 +664,46 @@ void launch_get_divergence(const DMesh& m, const DState& s, hipStream_t st)
 {
     if (m.n_eblk) hipLaunchKernelGGL(k_get_divergence, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s);
 }
+// Variant dispatch of the tuned kernels: the work tables in `m` were built
+// for workgroups of m.bs entries; m.minw selects the occupancy floor.
+#define FSGH_HEAVY_VARIANTS(X) X(64, 1) X(64, 3) X(128, 1) X(128, 3) X(256, 1) X(256, 3)
+#define FSGH_LIGHT_VARIANTS(X) X(64) X(128) X(256) X(512)
+
+bool tuned_variant_ok(int kernel, int32_t bs, int32_t minw)
+{
+    const bool heavy = kernel == kTunedForce || kernel == kTunedEnergy;
+#define FSGH_H(B, W) if (bs == B && minw == W) return true;
+#define FSGH_L(B) if (bs == B && minw == 1) return true;
+    if (heavy) { FSGH_HEAVY_VARIANTS(FSGH_H) }
+    else { FSGH_LIGHT_VARIANTS(FSGH_L) }
+#undef FSGH_H
+#undef FSGH_L
+    return false;
+}
+
 void launch_force(const DMesh& m, const DState& s, const InstCtl* ctl, double fuzz, double small,
                   hipStream_t st)
 {
-    if (m.n_eblk)
-        hipLaunchKernelGGL(k_force, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, ctl, fuzz, small);
+    if (!m.n_eblk) return;
+#define FSGH_H(B, W)                                                                       \
+    if (m.bs == B && m.minw == W) {                                                        \
+        hipLaunchKernelGGL((k_force<B, W>), dim3(m.n_eblk), dim3(B), 0, st, m, s, ctl, fuzz, \
+                           small);                                                         \
+        return;                                                                            \
+    }
+    FSGH_HEAVY_VARIANTS(FSGH_H)
+#undef FSGH_H
 }
 void launch_div_extremes(const DMesh& m, const DState& s, const InstCtl* ctl, hipStream_t st)
 {
-    if (m.n_nblk)
-        hipLaunchKernelGGL(k_div_extremes, dim3(m.n_nblk), dim3(kBlock), 0, st, m, s, ctl);
+    if (!m.n_nblk) return;
+#define FSGH_L(B)                                                                          \
+    if (m.bs == B) {                                                                       \
+        hipLaunchKernelGGL((k_div_extremes<B>), dim3(m.n_nblk), dim3(B), 0, st, m, s, ctl);  \
+        return;                                                                            \
+    }
+    FSGH_LIGHT_VARIANTS(FSGH_L)
+#undef FSGH_L
 }
 void launch_update_velocity(const DMesh& m, const DState& s, double alpha, double dt,
                             hipStream_t st)
@@ -704,9 +742,15 @@ void launch_dt_partial(const DMesh& m, const DState& s, const InstCtl* ctl, doub
                        double dt_cfl, double dt_max, double dt_min, double fuzz, int use_ctl,
                        hipStream_t st)
 {
-    if (m.n_eblk)
-        hipLaunchKernelGGL(k_dt_partial, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, ctl, partial,
-                           dt_cfl, dt_max, dt_min, fuzz, use_ctl);
+    if (!m.n_eblk) return;
+#define FSGH_L(B)                                                                          \
+    if (m.bs == B) {                                                                       \
+        hipLaunchKernelGGL((k_dt_partial<B>), dim3(m.n_eblk), dim3(B), 0, st, m, s, ctl,     \
+                           partial, dt_cfl, dt_max, dt_min, fuzz, use_ctl);                \
+        return;                                                                            \
+    }
+    FSGH_LIGHT_VARIANTS(FSGH_L)
+#undef FSGH_L
 }
 void launch_reduce_min(const double* partial, int32_t n, double* out, hipStream_t st)
 {
@@ -721,15 +765,28 @@ void launch_cycle_dt(InstCtl* ctl, int32_t ninst, const double* partial, const d
 void launch_vel_bc_pos(const DMesh& m, const DState& s, const InstCtl* ctl, double alpha,
                        hipStream_t st)
 {
-    if (m.n_nblk)
-        hipLaunchKernelGGL(k_vel_bc_pos, dim3(m.n_nblk), dim3(kBlock), 0, st, m, s, ctl, alpha);
+    if (!m.n_nblk) return;
+#define FSGH_L(B)                                                                          \
+    if (m.bs == B) {                                                                       \
+        hipLaunchKernelGGL((k_vel_bc_pos<B>), dim3(m.n_nblk), dim3(B), 0, st, m, s, ctl,     \
+                           alpha);                                                         \
+        return;                                                                            \
+    }
+    FSGH_LIGHT_VARIANTS(FSGH_L)
+#undef FSGH_L
 }
 void launch_energy_state_div(const DMesh& m, const DState& s, const InstCtl* ctl, double alpha,
                              double* dt_partial, hipStream_t st)
 {
-    if (m.n_eblk)
-        hipLaunchKernelGGL(k_energy_state_div, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, ctl,
-                           alpha, dt_partial);
+    if (!m.n_eblk) return;
+#define FSGH_H(B, W)                                                                       \
+    if (m.bs == B && m.minw == W) {                                                        \
+        hipLaunchKernelGGL((k_energy_state_div<B, W>), dim3(m.n_eblk), dim3(B), 0, st, m, s, \
+                           ctl, alpha, dt_partial);                                        \
+        return;                                                                            \
+    }
+    FSGH_HEAVY_VARIANTS(FSGH_H)
+#undef FSGH_H
 }
 void launch_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin, hipStream_t st)
 {

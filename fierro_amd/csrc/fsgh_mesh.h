@@ -49,10 +49,11 @@ void morton_renumber(const Layout& L, const int64_t* nodes_in_elem, const double
 void corner_csr(int64_t N, int64_t E, const int64_t* nodes_in_elem, std::vector<int64_t>& start,
                 std::vector<int64_t>& corners);
 
-// Work items of kBlock entries per instance, dealt to XCDs so that logical
-// block L runs on XCD floor(L / ceil(nb/8))-ish: physical block b -> XCD b % 8.
+// Work items of `bs` entries per instance.  With `xcd` set they are dealt to
+// XCDs so that each XCD sweeps one contiguous range of logical blocks
+// (physical block b runs on XCD b % 8); otherwise physical = logical order.
 void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
-                 std::vector<int32_t>& inst_begin);
+                 std::vector<int32_t>& inst_begin, int32_t bs = kBlock, bool xcd = true);
 
 // Boundary node sets of one instance (K/setup.cpp:431-665): nodes of the
 // boundary faces whose 4 nodes all satisfy the surface test.  Each set is

@@ -71,24 +71,28 @@ void corner_csr(int64_t N, int64_t E, const int64_t* conn, std::vector<int64_t>&
 }
 
 void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
-                 std::vector<int32_t>& inst_begin)
+                 std::vector<int32_t>& inst_begin, int32_t bs, bool xcd)
 {
     std::vector<BlockItem> logical;
     const int32_t ninst = static_cast<int32_t>(off.size()) - 1;
     inst_begin.assign(static_cast<size_t>(ninst + 1), 0);
     for (int32_t i = 0; i < ninst; ++i) {
         inst_begin[static_cast<size_t>(i)] = static_cast<int32_t>(logical.size());
-        for (int64_t s = off[i]; s < off[i + 1]; s += kBlock) {
+        for (int64_t s = off[i]; s < off[i + 1]; s += bs) {
             BlockItem b;
             b.inst = i;
             b.start = static_cast<int32_t>(s);
-            b.end = static_cast<int32_t>(std::min<int64_t>(s + kBlock, off[i + 1]));
+            b.end = static_cast<int32_t>(std::min<int64_t>(s + bs, off[i + 1]));
             b.lid = static_cast<int32_t>(logical.size());
             logical.push_back(b);
         }
     }
     inst_begin[static_cast<size_t>(ninst)] = static_cast<int32_t>(logical.size());
     const int64_t nb = static_cast<int64_t>(logical.size());
+    if (!xcd) {
+        table = logical;
+        return;
+    }
     table.resize(logical.size());
     int64_t base[kNumXcd + 1];
     base[0] = 0;

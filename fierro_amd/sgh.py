@@ -336,6 +336,40 @@ class SGHSolver:
                                   _p(t["x"]), _p(t["v"]), _p(t["e"])), self.ctx, "tangent")
         return t
 
+    # -- launch configuration --
+    @staticmethod
+    def _tuning_dicts(arr):
+        return {_abi.TUNED_KERNEL_NAMES[t.kernel]: {k: getattr(t, k) for k, _ in _abi.Tuning._fields_
+                                                   if k != "kernel"} for t in arr}
+
+    def autotune(self, reps=10):
+        """Time every compiled launch variant of the four fused-cycle kernels on
+        the current state and keep the fastest (the state is left unchanged).
+        Returns {kernel: {block, min_waves, xcd_order, ms, ms_default}}."""
+        arr = (_abi.Tuning * _abi.NUM_TUNED_KERNELS)()
+        check(self.L.fsgh_autotune(self.ctx, int(reps), arr), self.ctx, "autotune")
+        return self._tuning_dicts(arr)
+
+    def set_tuning(self, cfg):
+        """cfg: {kernel name: {block, min_waves, xcd_order}} (missing keys keep
+        the current value)."""
+        cur = self.get_tuning()
+        items = []
+        for name, v in cfg.items():
+            k = _abi.TUNED_KERNEL_NAMES.index(name)
+            c = dict(cur[name])
+            c.update(v)
+            items.append(_abi.Tuning(k, int(c["block"]), int(c["min_waves"]),
+                                     int(c["xcd_order"]), 0.0, 0.0))
+        arr = (_abi.Tuning * len(items))(*items)
+        check(self.L.fsgh_set_tuning(self.ctx, arr, len(items)), self.ctx, "set_tuning")
+
+    def get_tuning(self):
+        arr = (_abi.Tuning * _abi.NUM_TUNED_KERNELS)()
+        check(self.L.fsgh_get_tuning(self.ctx, arr, _abi.NUM_TUNED_KERNELS), self.ctx,
+              "get_tuning")
+        return self._tuning_dicts(arr)
+
     # -- profiling --
     def profile(self, enable=True):
         check(self.L.fsgh_profile_enable(self.ctx, 1 if enable else 0), self.ctx, "profile")

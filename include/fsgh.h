@@ -228,6 +228,43 @@ int fsgh_run_begin(fsgh_ctx *ctx, const fsgh_run_params *params);
 int fsgh_run(fsgh_ctx *ctx, int64_t ncycles, int32_t flags);
 int fsgh_run_status(fsgh_ctx *ctx, int32_t instance, fsgh_instance_status *out);
 
+/* ------------------------------------------------- launch autotuning --- */
+/* No counterpart in the reference (Kokkos picks its own launch shapes).  The
+ * four kernels of the fused cycle run in one of several compiled variants:
+ * workgroup size (elements or nodes per workgroup), an occupancy floor asked
+ * of the register allocator (waves per SIMD; 1 = the compiler's choice) for
+ * the two element kernels, and whether consecutive work ranges are dealt to
+ * one XCD.  Every variant computes bit-identical results. */
+#define FSGH_KERNEL_DIV_EXTREMES 0
+#define FSGH_KERNEL_FORCE 1
+#define FSGH_KERNEL_VEL_BC_POS 2
+#define FSGH_KERNEL_ENERGY_STATE_DIV 3
+#define FSGH_NUM_TUNED_KERNELS 4
+
+typedef struct fsgh_tuning {
+    int32_t kernel;       /* FSGH_KERNEL_* */
+    int32_t block;        /* 64, 128, 256 (default); 512 for the two node kernels */
+    int32_t min_waves;    /* 1 (default) or 3 (force and energy kernels only) */
+    int32_t xcd_order;    /* 1 (default): consecutive ranges on one XCD; 0: launch order */
+    double ms;            /* fsgh_autotune: mean time per launch of this choice */
+    double ms_default;    /* fsgh_autotune: the same for the default variant */
+} fsgh_tuning;
+
+/* Times every variant of each kernel on the context's current state (reps
+ * launches per variant, twice, the lower mean kept; reps <= 0 means 10) and
+ * keeps the fastest.  The state is saved before and restored after, so the
+ * call changes no field; it needs a set-up state (after fsgh_setup_fills or
+ * uploads) and temporary device memory of about one copy of the state.
+ * out: FSGH_NUM_TUNED_KERNELS entries, or NULL.  Usable before or between
+ * fsgh_run calls. */
+int fsgh_autotune(fsgh_ctx *ctx, int32_t reps, fsgh_tuning *out);
+/* Applies saved choices (e.g. from an earlier fsgh_autotune on the same
+ * mesh size); each entry names its kernel.  FSGH_E_INVALID for a variant that
+ * is not compiled. */
+int fsgh_set_tuning(fsgh_ctx *ctx, const fsgh_tuning *cfg, int32_t count);
+/* The current choice per kernel (ms fields 0 unless set by fsgh_autotune). */
+int fsgh_get_tuning(fsgh_ctx *ctx, fsgh_tuning *out, int32_t count);
+
 /* ------------------------------------ discrete adjoint / tangent model --- */
 /* No counterpart in the reference (which has no derivative of the SGH cycle).
  * The model is the fused cycle of fsgh_run with the time steps frozen to the

@@ -97,3 +97,23 @@ def test_host_only_context_error_paths():
 def test_device_count_without_gpu_is_safe():
     n = _abi.load().fsgh_device_count()
     assert n >= 0
+
+
+def test_tuning_entry_points_on_host_context():
+    """Launch tuning needs a device context; the defaults read back on any."""
+    L = _abi.load()
+    coords, conn = P.box_mesh(2, 2, 2)
+    ctx = C.c_void_p()
+    desc = _abi.MeshDesc(coords.shape[0], conn.shape[0], 1, 1, None, None)
+    assert L.fsgh_create(C.byref(desc), _p(conn), _p(coords), -1, C.byref(ctx)) == 0
+    try:
+        out = (_abi.Tuning * _abi.NUM_TUNED_KERNELS)()
+        assert L.fsgh_get_tuning(ctx, out, _abi.NUM_TUNED_KERNELS) == 0
+        assert [(t.kernel, t.block, t.min_waves, t.xcd_order) for t in out] == \
+            [(k, 256, 1, 1) for k in range(_abi.NUM_TUNED_KERNELS)]
+        assert L.fsgh_autotune(ctx, 1, out) == -5
+        cfg = (_abi.Tuning * 1)(_abi.Tuning(_abi.KERNEL_FORCE, 128, 1, 1, 0.0, 0.0))
+        assert L.fsgh_set_tuning(ctx, cfg, 1) == -5
+        assert L.fsgh_get_tuning(ctx, None, 1) == -1
+    finally:
+        assert L.fsgh_destroy(ctx) == 0
