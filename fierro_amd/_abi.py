@@ -36,6 +36,7 @@ EXPORTS = [
     "fsgh_profile_read", "fsgh_check", "fsgh_stream",
     "fsgh_forward", "fsgh_dt_history", "fsgh_backward", "fsgh_tangent",
     "fsgh_autotune", "fsgh_set_tuning", "fsgh_get_tuning",
+    "fsgh_rz_extensive_mass", "fsgh_rz_update_node_mass",
 ]
 
 KERNEL_DIV_EXTREMES, KERNEL_FORCE, KERNEL_VEL_BC_POS, KERNEL_ENERGY_STATE_DIV = 0, 1, 2, 3
@@ -46,7 +47,8 @@ TUNED_KERNEL_NAMES = ["div_extremes", "force", "vel_bc_pos", "energy_state_div"]
 class MeshDesc(C.Structure):
     _fields_ = [("num_nodes", C.c_int64), ("num_elems", C.c_int64),
                 ("num_instances", C.c_int32), ("renumber", C.c_int32),
-                ("instance_node_off", C.c_void_p), ("instance_elem_off", C.c_void_p)]
+                ("instance_node_off", C.c_void_p), ("instance_elem_off", C.c_void_p),
+                ("num_dims", C.c_int32), ("reserved_", C.c_int32)]
 
 
 class Material(C.Structure):
@@ -72,7 +74,8 @@ class RunParams(C.Structure):
     _fields_ = [("time_final", C.c_double), ("dt_start", C.c_double),
                 ("dt_max", C.c_double), ("dt_min", C.c_double), ("dt_cfl", C.c_double),
                 ("graphics_dt_ival", C.c_double), ("graphics_cyc_ival", C.c_int64),
-                ("cycle_stop", C.c_int64), ("fuzz", C.c_double), ("small", C.c_double)]
+                ("cycle_stop", C.c_int64), ("fuzz", C.c_double), ("small", C.c_double),
+                ("tiny", C.c_double)]
 
 
 class Tuning(C.Structure):
@@ -137,6 +140,8 @@ def load() -> C.CDLL:
         "fsgh_autotune": [vp, i32, C.POINTER(Tuning)],
         "fsgh_set_tuning": [vp, C.POINTER(Tuning), i32],
         "fsgh_get_tuning": [vp, C.POINTER(Tuning), i32],
+        "fsgh_rz_extensive_mass": [vp],
+        "fsgh_rz_update_node_mass": [vp, dbl],
     }
     for name, args in sig.items():
         f = getattr(L, name)

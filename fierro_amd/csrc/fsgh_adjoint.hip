@@ -1483,6 +1483,10 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
                  int64_t checkpoint_every, const double* dt_replay)
 {
     if (!c) return FSGH_E_INVALID;
+    if (c->dims == 2) {
+        set_error(c, "the adjoint covers the 3D cycle only");
+        return FSGH_E_UNSUPPORTED;
+    }
     if (c->host_only) {
         set_error(c, "host-only context");
         return FSGH_E_STATE;
@@ -1584,7 +1588,7 @@ int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const
     }
     int rc = ensure_reverse_buffers(c);
     if (rc) return rc;
-    const size_t N = D.N, E = D.E, SW = 6 * N + E;
+    const size_t N = D.N, E = D.E;
     DMesh m = make_mesh(c);
     // the recomputation rewrites the control blocks (cycle, dt): keep the forward's
     std::vector<InstCtl> ctl(static_cast<size_t>(c->L.ninst));

@@ -57,6 +57,8 @@ DMesh fsgh::make_mesh(fsgh_ctx* c)
     m.n_nblk = c->n_nblk;
     m.bs = kBlock;
     m.minw = 1;
+    m.cshift = c->dims == 2 ? 2 : 3;
+    m.pad_ = 0;
     return m;
 }
 
@@ -235,11 +237,16 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
     if (!d || !nodes_in_elem || !coords || !out) return FSGH_E_INVALID;
     *out = nullptr;
     if (d->num_nodes <= 0 || d->num_elems <= 0 || d->num_instances < 1) return FSGH_E_INVALID;
-    if (8 * d->num_elems >= (int64_t(1) << 31) || d->num_nodes >= (int64_t(1) << 31))
+    const int dims = d->num_dims == 0 ? 3 : d->num_dims;
+    if (dims != 2 && dims != 3) return FSGH_E_INVALID;
+    const int npe = dims == 2 ? 4 : 8;
+    if (npe * d->num_elems >= (int64_t(1) << 31) || d->num_nodes >= (int64_t(1) << 31))
         return FSGH_E_UNSUPPORTED;
     auto* c = new fsgh_ctx();
     c->N = d->num_nodes;
     c->E = d->num_elems;
+    c->dims = dims;
+    c->npe = npe;
     c->device = device;
     c->renumber = d->renumber;
     c->L.ninst = d->num_instances;
@@ -264,12 +271,20 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
         delete c;
         return FSGH_E_INVALID;
     }
-    c->conn.assign(nodes_in_elem, nodes_in_elem + 8 * c->E);
-    c->coords.assign(coords, coords + 3 * c->N);
+    c->conn.assign(nodes_in_elem, nodes_in_elem + npe * c->E);
+    if (dims == 3) {
+        c->coords.assign(coords, coords + 3 * c->N);
+    } else {
+        c->coords.assign(static_cast<size_t>(3 * c->N), 0.0);   // (axial, radius, 0)
+        for (int64_t n = 0; n < c->N; ++n) {
+            c->coords[static_cast<size_t>(3 * n)] = coords[2 * n];
+            c->coords[static_cast<size_t>(3 * n + 1)] = coords[2 * n + 1];
+        }
+    }
     for (int32_t i = 0; i < c->L.ninst; ++i)
         for (int64_t e = c->L.elem_off[i]; e < c->L.elem_off[i + 1]; ++e)
-            for (int a = 0; a < 8; ++a) {
-                const int64_t n = c->conn[static_cast<size_t>(8 * e + a)];
+            for (int a = 0; a < npe; ++a) {
+                const int64_t n = c->conn[static_cast<size_t>(npe * e + a)];
                 if (n < c->L.node_off[i] || n >= c->L.node_off[i + 1]) {
                     delete c;
                     return FSGH_E_INVALID;
@@ -281,7 +296,7 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
     if (device < 0) {
         // numbering
         if (c->renumber) {
-            morton_renumber(c->L, c->conn.data(), c->coords.data(), c->ne_of_oe, c->nn_of_on);
+            morton_renumber(c->L, c->conn.data(), c->coords.data(), c->ne_of_oe, c->nn_of_on, npe);
         } else {
             c->ne_of_oe.resize(static_cast<size_t>(c->E));
             c->nn_of_on.resize(static_cast<size_t>(c->N));
@@ -292,7 +307,7 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
         c->on_of_nn.resize(static_cast<size_t>(c->N));
         for (int64_t e = 0; e < c->E; ++e) c->oe_of_ne[static_cast<size_t>(c->ne_of_oe[e])] = static_cast<int32_t>(e);
         for (int64_t n = 0; n < c->N; ++n) c->on_of_nn[static_cast<size_t>(c->nn_of_on[n])] = static_cast<int32_t>(n);
-        corner_csr(c->N, c->E, c->conn.data(), c->cstart, c->corners);
+        corner_csr(c->N, c->E, c->conn.data(), c->cstart, c->corners, npe);
     }
     std::vector<BlockItem> etab, ntab;
     block_table(c->L.elem_off, etab, c->eblk_begin);
@@ -352,8 +367,12 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
     rc |= dalloc(c, &c->d_scalar, 4);
     rc |= dalloc(c, &c->d_scratch, 2 * static_cast<size_t>(c->nscratch));
     rc |= dalloc(c, &c->d_flag, 1);
+    if (dims == 2) {
+        rc |= dalloc(c, &c->d_ext, N);
+        rc |= dalloc(c, &c->d_rin, N);
+    }
     if (rc != FSGH_OK) return FSGH_E_NOMEM;
-    if ((rc = gpu_build_maps(c)) != FSGH_OK) return rc;
+    if ((rc = dims == 2 ? rz_build_maps(c) : gpu_build_maps(c)) != FSGH_OK) return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_eblk, etab.data(), etab.size() * sizeof(BlockItem), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_nblk, ntab.data(), ntab.size() * sizeof(BlockItem), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -377,7 +396,8 @@ int fsgh_destroy(fsgh_ctx* c)
                     c->d_den, c->d_pres, c->d_sspd, c->d_vol, c->d_div[0], c->d_div[1], c->d_ext_lo, c->d_ext_hi, c->d_emass, c->d_gamma,
                     c->d_csmin, c->d_F, c->d_mat, c->d_bcmask, c->d_mats, c->d_ctl, c->d_partial,
                     c->d_scalar, c->d_scratch, c->d_flag, c->d_bcnodes, c->d_bcmasks,
-                    c->d_sie[0], c->d_sie[1], c->d_on_of_nn, c->d_oe_of_ne};
+                    c->d_sie[0], c->d_sie[1], c->d_on_of_nn, c->d_oe_of_ne, c->d_ext, c->d_rin,
+                    c->d_axis};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (auto& kv : c->tables)
@@ -396,7 +416,7 @@ int fsgh_get_corners_in_node(fsgh_ctx* c, int64_t* start, int64_t* corners)
     if (!start || !corners) return FSGH_E_INVALID;
     // a device context keeps only the internal CSR; the caller-numbered one
     // is built on first request
-    if (c->cstart.empty()) corner_csr(c->N, c->E, c->conn.data(), c->cstart, c->corners);
+    if (c->cstart.empty()) corner_csr(c->N, c->E, c->conn.data(), c->cstart, c->corners, c->npe);
     std::copy(c->cstart.begin(), c->cstart.end(), start);
     std::copy(c->corners.begin(), c->corners.end(), corners);
     return FSGH_OK;
@@ -409,7 +429,7 @@ int fsgh_get_device_csr(fsgh_ctx* c, int32_t* start, int32_t* corners)
     if (!start || !corners) return FSGH_E_INVALID;
     HIPCHK(c, hipMemcpyAsync(start, c->d_cin_start, static_cast<size_t>(c->N + 1) * 4,
                              hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(corners, c->d_cin, static_cast<size_t>(8 * c->E) * 4,
+    HIPCHK(c, hipMemcpyAsync(corners, c->d_cin, static_cast<size_t>(c->npe * c->E) * 4,
                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FSGH_OK;
@@ -499,6 +519,10 @@ int fsgh_set_bcs(fsgh_ctx* c, int32_t inst, int32_t nsets, const fsgh_boundary* 
     if (inst < 0 || inst >= c->L.ninst || nsets < 0 || (nsets > 0 && !sets)) return FSGH_E_INVALID;
     for (int32_t s = 0; s < nsets; ++s) {
         if (sets[s].surface < 0 || sets[s].surface > 4) return FSGH_E_INVALID;
+        if (c->dims == 2 && (sets[s].surface == 2 || sets[s].surface == 4)) {
+            set_error(c, "2D context: boundary surfaces are x-plane, y-plane or cylinder");
+            return FSGH_E_UNSUPPORTED;
+        }
         // the reference indexes velocity component `surface` for reflected sets,
         // which is only meaningful for planes (K/boundary.cpp:130-135)
         if (sets[s].hydro_bc == 1 && sets[s].surface > 2) {
@@ -507,9 +531,9 @@ int fsgh_set_bcs(fsgh_ctx* c, int32_t inst, int32_t nsets, const fsgh_boundary* 
         }
     }
     auto& out = c->bcsets[static_cast<size_t>(inst)];
-    if (c->host_only) {
+    if (c->host_only || c->dims == 2) {
         boundary_sets(c->L, inst, c->conn.data(), c->coords.data(), c->cstart, c->corners, nsets,
-                      sets, out);
+                      sets, out, c->npe);
     } else if (int rc = gpu_boundary_sets(c, inst, nsets, sets, out)) {
         return rc;
     }
@@ -561,6 +585,10 @@ int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill*
         if (fills[f].mat_id < 0 || fills[f].mat_id >= nmat) return FSGH_E_INVALID;
         if (fills[f].velocity < 0 || fills[f].velocity > 4 || fills[f].volume < 0 || fills[f].volume > 3)
             return FSGH_E_UNSUPPORTED;
+    }
+    if (c->dims == 2) {
+        drop_graphs(c);
+        return rz_setup_fills(c, inst, nfills, fills, nmat, nsv, state_vars, mat_nsv);
     }
     if (nfills <= 64) {
         drop_graphs(c);
@@ -710,13 +738,14 @@ static int field_io(fsgh_ctx* c, int32_t field, void* host, size_t bytes, bool u
     switch (field) {
     case FSGH_NODE_COORDS:
     case FSGH_NODE_VEL: {
-        if (bytes != 2 * N * 3 * 8) return FSGH_E_INVALID;
+        const int D = c->dims;
+        if (bytes != 2 * N * D * 8) return FSGH_E_INVALID;
         const int k0 = field == FSGH_NODE_COORDS ? 0 : 3;
         for (int b = 0; b < 2; ++b)
-            for (int d = 0; d < 3; ++d) {
-                double* h = static_cast<double*>(host) + b * N * 3;
-                rc = up ? put_nodes(c, c->d_node[phys[b]][k0 + d], h, 3, d)
-                        : get_nodes(c, c->d_node[phys[b]][k0 + d], h, 3, d);
+            for (int d = 0; d < D; ++d) {
+                double* h = static_cast<double*>(host) + b * N * D;
+                rc = up ? put_nodes(c, c->d_node[phys[b]][k0 + d], h, D, d)
+                        : get_nodes(c, c->d_node[phys[b]][k0 + d], h, D, d);
                 if (rc) return rc;
             }
         if (up) drop_graphs(c);
@@ -740,19 +769,21 @@ static int field_io(fsgh_ctx* c, int32_t field, void* host, size_t bytes, bool u
         return up ? put_elems(c, c->d_mat, static_cast<int64_t*>(host), 1, 0)
                   : get_elems(c, c->d_mat, static_cast<int64_t*>(host), 1, 0);
     case FSGH_CORNER_FORCE: {
-        if (bytes != 24 * E * 8) return FSGH_E_INVALID;
-        std::vector<double> t(24 * E);
+        // per element: npe corners x num_dims components, contiguous
+        const size_t K = static_cast<size_t>(c->npe * c->dims);
+        if (bytes != K * E * 8) return FSGH_E_INVALID;
+        std::vector<double> t(K * E);
         double* h = static_cast<double*>(host);
         if (up) {
             for (size_t e = 0; e < E; ++e)
-                std::memcpy(&t[24 * static_cast<size_t>(c->ne_of_oe[e])], h + 24 * e, 24 * 8);
+                std::memcpy(&t[K * static_cast<size_t>(c->ne_of_oe[e])], h + K * e, K * 8);
             HIPCHK(c, hipMemcpyAsync(c->d_F, t.data(), t.size() * 8, hipMemcpyHostToDevice, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
         } else {
             HIPCHK(c, hipMemcpyAsync(t.data(), c->d_F, t.size() * 8, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             for (size_t e = 0; e < E; ++e)
-                std::memcpy(h + 24 * e, &t[24 * static_cast<size_t>(c->ne_of_oe[e])], 24 * 8);
+                std::memcpy(h + K * e, &t[K * static_cast<size_t>(c->ne_of_oe[e])], K * 8);
         }
         return FSGH_OK;
     }
@@ -820,7 +851,12 @@ int fsgh_get_timestep(fsgh_ctx* c, double time_value, double graphics_time, doub
     DState s = ref_state(c);
     double d = *dt * 1.1;
     timed(c, "dt_partial", [&] {
-        launch_dt_partial(m, s, nullptr, c->d_partial, dt_cfl, dt_max, dt_min, fuzz, 0, c->stream);
+        if (c->dims == 2)
+            launch_rz_dt_partial(m, s, nullptr, c->d_partial, dt_cfl, dt_max, dt_min, fuzz, 0,
+                                 c->stream);
+        else
+            launch_dt_partial(m, s, nullptr, c->d_partial, dt_cfl, dt_max, dt_min, fuzz, 0,
+                              c->stream);
     });
     timed(c, "dt_reduce", [&] { launch_reduce_min(c->d_partial, c->n_eblk, c->d_scalar, c->stream); });
     double min_dt_calc = 0.0;
@@ -838,7 +874,10 @@ int fsgh_get_divergence(fsgh_ctx* c)
     OP_PROLOGUE(c);
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
-    timed(c, "get_divergence", [&] { launch_get_divergence(m, s, c->stream); });
+    timed(c, "get_divergence", [&] {
+        if (c->dims == 2) launch_rz_get_divergence(m, s, c->stream);   // K/momentum.cpp:298-378
+        else launch_get_divergence(m, s, c->stream);
+    });
     OP_EPILOGUE(c);
 }
 
@@ -851,7 +890,10 @@ int fsgh_get_force_sgh(fsgh_ctx* c, double fuzz, double small, double dt, double
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
     timed(c, "div_extremes", [&] { launch_div_extremes(m, s, nullptr, c->stream); });
-    timed(c, "force", [&] { launch_force(m, s, nullptr, fuzz, small, c->stream); });
+    timed(c, "force", [&] {
+        if (c->dims == 2) launch_rz_force(m, s, nullptr, fuzz, small, c->stream);   // K/force_sgh.cpp:407-808
+        else launch_force(m, s, nullptr, fuzz, small, c->stream);
+    });
     OP_EPILOGUE(c);
 }
 
@@ -860,7 +902,10 @@ int fsgh_update_velocity_sgh(fsgh_ctx* c, double rk_alpha, double dt)
     OP_PROLOGUE(c);
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
-    timed(c, "update_velocity", [&] { launch_update_velocity(m, s, rk_alpha, dt, c->stream); });
+    timed(c, "update_velocity", [&] {
+        if (c->dims == 2) launch_rz_update_velocity(m, s, rk_alpha, dt, c->stream);
+        else launch_update_velocity(m, s, rk_alpha, dt, c->stream);
+    });
     OP_EPILOGUE(c);
 }
 
@@ -878,7 +923,10 @@ int fsgh_update_energy_sgh(fsgh_ctx* c, double rk_alpha, double dt)
     OP_PROLOGUE(c);
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
-    timed(c, "update_energy", [&] { launch_update_energy(m, s, rk_alpha, dt, c->stream); });
+    timed(c, "update_energy", [&] {
+        if (c->dims == 2) launch_rz_update_energy(m, s, rk_alpha, dt, c->stream);
+        else launch_update_energy(m, s, rk_alpha, dt, c->stream);
+    });
     OP_EPILOGUE(c);
 }
 
@@ -896,7 +944,10 @@ int fsgh_get_vol(fsgh_ctx* c)
     OP_PROLOGUE(c);
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
-    timed(c, "get_vol", [&] { launch_get_vol(m, s, c->stream); });
+    timed(c, "get_vol", [&] {
+        if (c->dims == 2) launch_rz_get_vol(m, s, c->stream);   // K/geometry.cpp:381-431
+        else launch_get_vol(m, s, c->stream);
+    });
     OP_EPILOGUE(c);
 }
 
@@ -920,12 +971,34 @@ int fsgh_energy(fsgh_ctx* c, int32_t inst, double* ie, double* ke)
     std::vector<int32_t> eo(c->L.elem_off.begin(), c->L.elem_off.end());
     std::vector<int32_t> no(c->L.node_off.begin(), c->L.node_off.end());
     launch_energy_sums(m, s, eo.data(), no.data(), inst, c->d_scalar, c->d_scratch, c->nscratch,
-                       c->stream);
+                       c->dims == 2 ? 1 : 0, c->stream);
     double out[2];
     HIPCHK(c, hipMemcpyAsync(out, c->d_scalar, 16, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *ie = out[0];
     *ke = out[1];
+    OP_EPILOGUE(c);
+}
+
+int fsgh_rz_extensive_mass(fsgh_ctx* c)
+{
+    OP_PROLOGUE(c);
+    if (c->dims != 2) return FSGH_E_STATE;
+    DMesh m = make_mesh(c);
+    DState s = ref_state(c);
+    timed(c, "rz_extensive_mass", [&] { launch_rz_ext(m, s, c->d_ext, c->stream); });
+    OP_EPILOGUE(c);
+}
+
+int fsgh_rz_update_node_mass(fsgh_ctx* c, double tiny)
+{
+    OP_PROLOGUE(c);
+    if (c->dims != 2) return FSGH_E_STATE;
+    DMesh m = make_mesh(c);
+    DState s = ref_state(c);
+    timed(c, "rz_node_mass", [&] {
+        launch_rz_node_mass(m, s, c->d_ext, tiny, c->d_axis, c->n_axis, c->stream);
+    });
     OP_EPILOGUE(c);
 }
 
@@ -951,6 +1024,7 @@ int fsgh_run_begin(fsgh_ctx* c, const fsgh_run_params* p)
         k.graphics_dt_ival = q.graphics_dt_ival;
         k.fuzz = q.fuzz;
         k.small = q.small;
+        k.tiny = q.tiny;
         k.cycle = 0;
         k.cycle_stop = q.cycle_stop;
         k.graphics_cyc_ival = q.graphics_cyc_ival;
@@ -966,11 +1040,20 @@ int fsgh_run_begin(fsgh_ctx* c, const fsgh_run_params* p)
     DState s = ref_state(c);
     // the first stage's divergence comes from the current state
     // (K/sgh_solve.cpp:364); later stages take it from the fused energy kernel
-    timed(c, "get_divergence", [&] { launch_get_divergence(m, s, c->stream); });
-    timed(c, "dt_partial", [&] {
-        launch_dt_partial(tuned_mesh(c, kTunedEnergy), s, c->d_ctl, c->d_partial, 0, 0, 0, 0, 1,
-                          c->stream);
-    });
+    if (c->dims == 2) {
+        // node_extensive_mass at the start of sgh_solve (K/sgh_solve.cpp:107-116)
+        timed(c, "rz_extensive_mass", [&] { launch_rz_ext(m, s, c->d_ext, c->stream); });
+        timed(c, "get_divergence", [&] { launch_rz_get_divergence(m, s, c->stream); });
+        timed(c, "dt_partial", [&] {
+            launch_rz_dt_partial(m, s, c->d_ctl, c->d_partial, 0, 0, 0, 0, 1, c->stream);
+        });
+    } else {
+        timed(c, "get_divergence", [&] { launch_get_divergence(m, s, c->stream); });
+        timed(c, "dt_partial", [&] {
+            launch_dt_partial(tuned_mesh(c, kTunedEnergy), s, c->d_ctl, c->d_partial, 0, 0, 0, 0,
+                              1, c->stream);
+        });
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->run_active = true;
     OP_EPILOGUE(c);
@@ -1029,12 +1112,15 @@ int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t flags)
             if (!c->graph[old]) {
                 hipGraph_t g;
                 HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                issue_cycle(c, old, nullptr, nullptr);
+                if (c->dims == 2) issue_cycle_rz(c, old, nullptr, nullptr);
+                else issue_cycle(c, old, nullptr, nullptr);
                 HIPCHK(c, hipStreamEndCapture(c->stream, &g));
                 HIPCHK(c, hipGraphInstantiate(&c->graph[old], g, nullptr, nullptr, 0));
                 HIPCHK(c, hipGraphDestroy(g));
             }
             HIPCHK(c, hipGraphLaunch(c->graph[old], c->stream));
+        } else if (c->dims == 2) {
+            issue_cycle_rz(c, old, nullptr, nullptr);
         } else {
             issue_cycle(c, old, nullptr, nullptr);
         }

@@ -18,8 +18,10 @@ struct fsgh_ctx {
     fsgh::Layout L;
     int64_t N = 0, E = 0;
     int renumber = 0;
-    std::vector<int64_t> conn;              // caller numbering, E x 8
-    std::vector<double> coords;             // caller numbering, N x 3 (initial)
+    int dims = 3;                           // 3: hexes; 2: axisymmetric RZ quads
+    int npe = 8;                            // nodes (= corners) per element
+    std::vector<int64_t> conn;              // caller numbering, E x npe
+    std::vector<double> coords;             // caller numbering, N x 3 (initial; z = 0 in 2D)
     std::vector<int32_t> ne_of_oe, nn_of_on, oe_of_ne, on_of_nn;
     std::vector<int64_t> cstart, corners;   // caller-numbering corner CSR
     std::vector<int32_t> eblk_begin, nblk_begin;
@@ -81,6 +83,13 @@ struct fsgh_ctx {
     std::vector<std::string> prof_order;
     // discrete adjoint / tangent state (allocated by fsgh_forward)
     AdjointData* adj = nullptr;
+    // 2D RZ: node extensive mass (areal mass x radius), the stage-input node
+    // radius of the fused loop, and the mesh boundary nodes (internal id,
+    // instance) that the on-axis mass fix visits
+    double* d_ext = nullptr;
+    double* d_rin = nullptr;
+    int2* d_axis = nullptr;
+    int32_t n_axis = 0;
 };
 
 
@@ -96,6 +105,11 @@ const std::vector<int32_t>& dt_block_begin(fsgh_ctx* c);
 // build (once) the work table of a configuration; FSGH_OK or an error code
 int ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd);
 void issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record);
+// 2D RZ contexts (fsgh_rz.hip)
+int rz_build_maps(fsgh_ctx* c);
+int rz_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill* fills,
+                   int32_t nmat, int32_t nsv, const double* state_vars, const int32_t* mat_nsv);
+void issue_cycle_rz(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record);
 void free_adjoint(fsgh_ctx* c);
 // mesh preparation on the device (fsgh_setup.hip)
 int gpu_build_maps(fsgh_ctx* c);

@@ -31,6 +31,7 @@ struct InstCtl {
     double graphics_time;
     double time_final, dt_max, dt_min, dt_cfl, graphics_dt_ival;
     double fuzz, small;
+    double tiny;             // on-axis radius threshold of the RZ node mass (K/driver.cpp:61)
     long long cycle;         // cycles completed
     long long cycle_stop, graphics_cyc_ival;
     int32_t stopped;
@@ -57,6 +58,8 @@ struct DMesh {
     int32_t n_eblk, n_nblk;
     int32_t bs;                // entries per work item = workgroup size of the tables above
     int32_t minw;              // occupancy floor (waves per SIMD) of the kernel variant to run
+    int32_t cshift;            // log2(corners per element): 3 for hexes, 2 for RZ quads
+    int32_t pad_;
 };
 
 // Launch configuration of one fused-cycle kernel, chosen by fsgh_autotune.
@@ -116,9 +119,27 @@ void launch_vel_bc_pos(const DMesh&, const DState&, const InstCtl*, double alpha
 void launch_energy_state_div(const DMesh&, const DState&, const InstCtl*, double alpha,
                              double* dt_partial, hipStream_t);
 void launch_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin, hipStream_t);
+// rz != 0: kinetic energy weighted by the node radius (K/sgh_solve.cpp:92-97)
 void launch_energy_sums(const DMesh&, const DState&, const int32_t* eoff, const int32_t* noff,
-                        int32_t inst, double* out2, double* scratch, int32_t nscratch,
+                        int32_t inst, double* out2, double* scratch, int32_t nscratch, int rz,
                         hipStream_t);
+// ---- 2D axisymmetric (RZ) path (fsgh_rz.hip) ----
+void launch_rz_get_vol(const DMesh&, const DState&, hipStream_t);
+void launch_rz_get_divergence(const DMesh&, const DState&, hipStream_t);
+void launch_rz_force(const DMesh&, const DState&, const InstCtl*, double fuzz, double small,
+                     hipStream_t);
+void launch_rz_update_velocity(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
+void launch_rz_update_energy(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
+void launch_rz_dt_partial(const DMesh&, const DState&, const InstCtl*, double* partial,
+                          double dt_cfl, double dt_max, double dt_min, double fuzz, int use_ctl,
+                          hipStream_t);
+void launch_rz_ext(const DMesh&, const DState&, double* ext, hipStream_t);
+void launch_rz_node_mass(const DMesh&, const DState&, const double* ext, double tiny,
+                         const int2* axis, int32_t naxis, hipStream_t);
+void launch_rz_node(const DMesh&, const DState&, const InstCtl*, double alpha, const double* rsrc,
+                    double* rin, const double* ext, const int2* axis, int32_t naxis, hipStream_t);
+void launch_rz_energy_state_div(const DMesh&, const DState&, const InstCtl*, double alpha,
+                                const double* rin, double* dt_partial, hipStream_t);
 void launch_swap_bins_range(const DState&, int32_t n0, int32_t n1, int32_t e0, int32_t e1,
                             hipStream_t);
 void launch_check(const DMesh&, const DState&, int32_t* flag, hipStream_t);

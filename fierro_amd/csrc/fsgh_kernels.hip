@@ -269,7 +269,7 @@ __global__ __launch_bounds__(BS) void k_div_extremes(DMesh m, DState s, const In
         const int last = cnt - 1;
         double d[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) d[j] = s.div[m.cin[q0 + (j < cnt ? j : last)] >> 3];
+        for (int j = 0; j < 8; ++j) d[j] = s.div[m.cin[q0 + (j < cnt ? j : last)] >> m.cshift];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const bool ok = j < cnt && !isnan(d[j]);
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(BS) void k_div_extremes(DMesh m, DState s, const In
             insert_max2(mx1, mx2, ok ? d[j] : -HUGE_VAL);
         }
         for (int q = q0 + 8; q < q0 + cnt; ++q) {
-            const double v = s.div[m.cin[q] >> 3];
+            const double v = s.div[m.cin[q] >> m.cshift];
             if (!isnan(v)) {
                 insert_min2(mn1, mn2, v);
                 insert_max2(mx1, mx2, v);
@@ -575,7 +575,7 @@ __global__ void k_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin)
 // per-instance energy tallies, fixed-order two-level sums (K/sgh_solve.cpp:77-101)
 __global__ __launch_bounds__(kBlock) void k_energy_partial(DState s, int32_t e0, int32_t e1,
                                                            int32_t n0, int32_t n1, double* scratch,
-                                                           int32_t nscratch)
+                                                           int32_t nscratch, int rz)
 {
     double ie = 0.0, ke = 0.0;
     const int stride = nscratch * kBlock;
@@ -585,8 +585,12 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(DState s, int32_t e0,
         double k = 0;
         k += s.u1[n] * s.u1[n];
         k += s.v1[n] * s.v1[n];
-        k += s.w1[n] * s.w1[n];
-        ke += s.nmass[n] * k;
+        if (rz) {
+            ke += s.nmass[n] * s.y1[n] * k;
+        } else {
+            k += s.w1[n] * s.w1[n];
+            ke += s.nmass[n] * k;
+        }
     }
     ie = block_sum(ie);
     ke = block_sum(ke);
@@ -672,12 +676,12 @@ void launch_get_divergence(const DMesh& m, const DState& s, hipStream_t st)
 bool tuned_variant_ok(int kernel, int32_t bs, int32_t minw)
 {
     const bool heavy = kernel == kTunedForce || kernel == kTunedEnergy;
-#define FSGH_H(B, W) if (bs == B && minw == W) return true;
-#define FSGH_L(B) if (bs == B && minw == 1) return true;
-    if (heavy) { FSGH_HEAVY_VARIANTS(FSGH_H) }
-    else { FSGH_LIGHT_VARIANTS(FSGH_L) }
-#undef FSGH_H
-#undef FSGH_L
+#define FSGH_VH(B, W) if (bs == B && minw == W) return true;
+#define FSGH_VL(B) if (bs == B && minw == 1) return true;
+    if (heavy) { FSGH_HEAVY_VARIANTS(FSGH_VH) }
+    else { FSGH_LIGHT_VARIANTS(FSGH_VL) }
+#undef FSGH_VH
+#undef FSGH_VL
     return false;
 }
 
@@ -685,25 +689,25 @@ void launch_force(const DMesh& m, const DState& s, const InstCtl* ctl, double fu
                   hipStream_t st)
 {
     if (!m.n_eblk) return;
-#define FSGH_H(B, W)                                                                       \
+#define FSGH_VH(B, W)                                                                       \
     if (m.bs == B && m.minw == W) {                                                        \
         hipLaunchKernelGGL((k_force<B, W>), dim3(m.n_eblk), dim3(B), 0, st, m, s, ctl, fuzz, \
                            small);                                                         \
         return;                                                                            \
     }
-    FSGH_HEAVY_VARIANTS(FSGH_H)
-#undef FSGH_H
+    FSGH_HEAVY_VARIANTS(FSGH_VH)
+#undef FSGH_VH
 }
 void launch_div_extremes(const DMesh& m, const DState& s, const InstCtl* ctl, hipStream_t st)
 {
     if (!m.n_nblk) return;
-#define FSGH_L(B)                                                                          \
+#define FSGH_VL(B)                                                                          \
     if (m.bs == B) {                                                                       \
         hipLaunchKernelGGL((k_div_extremes<B>), dim3(m.n_nblk), dim3(B), 0, st, m, s, ctl);  \
         return;                                                                            \
     }
-    FSGH_LIGHT_VARIANTS(FSGH_L)
-#undef FSGH_L
+    FSGH_LIGHT_VARIANTS(FSGH_VL)
+#undef FSGH_VL
 }
 void launch_update_velocity(const DMesh& m, const DState& s, double alpha, double dt,
                             hipStream_t st)
@@ -743,14 +747,14 @@ void launch_dt_partial(const DMesh& m, const DState& s, const InstCtl* ctl, doub
                        hipStream_t st)
 {
     if (!m.n_eblk) return;
-#define FSGH_L(B)                                                                          \
+#define FSGH_VL(B)                                                                          \
     if (m.bs == B) {                                                                       \
         hipLaunchKernelGGL((k_dt_partial<B>), dim3(m.n_eblk), dim3(B), 0, st, m, s, ctl,     \
                            partial, dt_cfl, dt_max, dt_min, fuzz, use_ctl);                \
         return;                                                                            \
     }
-    FSGH_LIGHT_VARIANTS(FSGH_L)
-#undef FSGH_L
+    FSGH_LIGHT_VARIANTS(FSGH_VL)
+#undef FSGH_VL
 }
 void launch_reduce_min(const double* partial, int32_t n, double* out, hipStream_t st)
 {
@@ -766,27 +770,27 @@ void launch_vel_bc_pos(const DMesh& m, const DState& s, const InstCtl* ctl, doub
                        hipStream_t st)
 {
     if (!m.n_nblk) return;
-#define FSGH_L(B)                                                                          \
+#define FSGH_VL(B)                                                                          \
     if (m.bs == B) {                                                                       \
         hipLaunchKernelGGL((k_vel_bc_pos<B>), dim3(m.n_nblk), dim3(B), 0, st, m, s, ctl,     \
                            alpha);                                                         \
         return;                                                                            \
     }
-    FSGH_LIGHT_VARIANTS(FSGH_L)
-#undef FSGH_L
+    FSGH_LIGHT_VARIANTS(FSGH_VL)
+#undef FSGH_VL
 }
 void launch_energy_state_div(const DMesh& m, const DState& s, const InstCtl* ctl, double alpha,
                              double* dt_partial, hipStream_t st)
 {
     if (!m.n_eblk) return;
-#define FSGH_H(B, W)                                                                       \
+#define FSGH_VH(B, W)                                                                       \
     if (m.bs == B && m.minw == W) {                                                        \
         hipLaunchKernelGGL((k_energy_state_div<B, W>), dim3(m.n_eblk), dim3(B), 0, st, m, s, \
                            ctl, alpha, dt_partial);                                        \
         return;                                                                            \
     }
-    FSGH_HEAVY_VARIANTS(FSGH_H)
-#undef FSGH_H
+    FSGH_HEAVY_VARIANTS(FSGH_VH)
+#undef FSGH_VH
 }
 void launch_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin, hipStream_t st)
 {
@@ -794,12 +798,12 @@ void launch_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin, hipStr
                        newest_bin);
 }
 void launch_energy_sums(const DMesh& m, const DState& s, const int32_t* eoff, const int32_t* noff,
-                        int32_t inst, double* out2, double* scratch, int32_t nscratch,
+                        int32_t inst, double* out2, double* scratch, int32_t nscratch, int rz,
                         hipStream_t st)
 {
     (void)m;
     hipLaunchKernelGGL(k_energy_partial, dim3(nscratch), dim3(kBlock), 0, st, s, eoff[inst],
-                       eoff[inst + 1], noff[inst], noff[inst + 1], scratch, nscratch);
+                       eoff[inst + 1], noff[inst], noff[inst + 1], scratch, nscratch, rz);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, scratch, nscratch, out2);
 }
 void launch_swap_bins_range(const DState& s, int32_t n0, int32_t n1, int32_t e0, int32_t e1,

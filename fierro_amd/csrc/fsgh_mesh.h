@@ -41,13 +41,15 @@ FSGH_HD inline uint32_t morton_quantize(double c, double lo, double hi)
 
 // Morton renumbering inside each instance.  new_of_old_elem / new_of_old_node
 // map caller ids to internal ids; old_of_new_* are their inverses.
+// npe = nodes per element (8 hexes, 4 RZ quads); coords are N x 3 (z = 0 in 2D)
 void morton_renumber(const Layout& L, const int64_t* nodes_in_elem, const double* coords,
-                     std::vector<int32_t>& new_of_old_elem, std::vector<int32_t>& new_of_old_node);
+                     std::vector<int32_t>& new_of_old_elem, std::vector<int32_t>& new_of_old_node,
+                     int npe = 8);
 
 // corners_in_node of the caller's mesh, each node's list in ascending element
 // order (K/mesh.h:218-241), as CSR over caller node ids.
 void corner_csr(int64_t N, int64_t E, const int64_t* nodes_in_elem, std::vector<int64_t>& start,
-                std::vector<int64_t>& corners);
+                std::vector<int64_t>& corners, int npe = 8);
 
 // Work items of `bs` entries per instance.  With `xcd` set they are dealt to
 // XCDs so that each XCD sweeps one contiguous range of logical blocks
@@ -61,6 +63,6 @@ void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
 int boundary_sets(const Layout& L, int32_t inst, const int64_t* nodes_in_elem,
                   const double* coords, const std::vector<int64_t>& cstart,
                   const std::vector<int64_t>& corners, int32_t nsets, const fsgh_boundary* bc,
-                  std::vector<std::vector<int64_t>>& out);
+                  std::vector<std::vector<int64_t>>& out, int npe = 8);
 
 }  // namespace fsgh

@@ -9,14 +9,17 @@ namespace fsgh {
 
 namespace {
 
-// node lids of the six faces of a hex (K/mesh.h:353-358)
+// node lids of the six faces of a hex (K/mesh.h:353-358) and of the four
+// edges of a quad (K/mesh.h:366-377)
 const int kFace[6][4] = {{0, 4, 7, 3}, {1, 2, 6, 5}, {0, 1, 5, 4},
                          {2, 3, 7, 6}, {0, 3, 2, 1}, {4, 5, 6, 7}};
+const int kQuadEdge[4][4] = {{0, 3, 0, 0}, {1, 2, 0, 0}, {0, 1, 0, 0}, {3, 2, 0, 0}};
 
 }  // namespace
 
 void morton_renumber(const Layout& L, const int64_t* conn, const double* xyz,
-                     std::vector<int32_t>& new_of_old_elem, std::vector<int32_t>& new_of_old_node)
+                     std::vector<int32_t>& new_of_old_elem, std::vector<int32_t>& new_of_old_node,
+                     int npe)
 {
     const int64_t E = L.elem_off.back(), N = L.node_off.back();
     new_of_old_elem.assign(static_cast<size_t>(E), -1);
@@ -34,10 +37,11 @@ void morton_renumber(const Layout& L, const int64_t* conn, const double* xyz,
         key.resize(static_cast<size_t>(e1 - e0));
         for (int64_t e = e0; e < e1; ++e) {
             double c[3] = {0.0, 0.0, 0.0};
-            for (int a = 0; a < 8; ++a)
-                for (int d = 0; d < 3; ++d) c[d] += xyz[3 * conn[8 * e + a] + d];
+            for (int a = 0; a < npe; ++a)
+                for (int d = 0; d < 3; ++d) c[d] += xyz[3 * conn[npe * e + a] + d];
             uint64_t code = 0;
-            for (int d = 0; d < 3; ++d) code |= morton_spread3(morton_quantize(c[d] / 8.0, lo[d], hi[d])) << d;
+            for (int d = 0; d < 3; ++d)
+                code |= morton_spread3(morton_quantize(c[d] / static_cast<double>(npe), lo[d], hi[d])) << d;
             key[static_cast<size_t>(e - e0)] = {code, static_cast<int32_t>(e)};
         }
         std::sort(key.begin(), key.end());
@@ -45,8 +49,8 @@ void morton_renumber(const Layout& L, const int64_t* conn, const double* xyz,
         for (size_t k = 0; k < key.size(); ++k) {
             const int32_t e = key[k].second;
             new_of_old_elem[static_cast<size_t>(e)] = static_cast<int32_t>(e0 + static_cast<int64_t>(k));
-            for (int a = 0; a < 8; ++a) {
-                const int64_t n = conn[8 * static_cast<int64_t>(e) + a];
+            for (int a = 0; a < npe; ++a) {
+                const int64_t n = conn[npe * static_cast<int64_t>(e) + a];
                 if (new_of_old_node[static_cast<size_t>(n)] < 0)
                     new_of_old_node[static_cast<size_t>(n)] = next_node++;
             }
@@ -59,15 +63,16 @@ void morton_renumber(const Layout& L, const int64_t* conn, const double* xyz,
 }
 
 void corner_csr(int64_t N, int64_t E, const int64_t* conn, std::vector<int64_t>& start,
-                std::vector<int64_t>& corners)
+                std::vector<int64_t>& corners, int npe)
 {
+    const int64_t C = npe * E;
     start.assign(static_cast<size_t>(N + 1), 0);
-    for (int64_t c = 0; c < 8 * E; ++c) start[static_cast<size_t>(conn[c] + 1)]++;
+    for (int64_t c = 0; c < C; ++c) start[static_cast<size_t>(conn[c] + 1)]++;
     for (int64_t n = 0; n < N; ++n) start[static_cast<size_t>(n + 1)] += start[static_cast<size_t>(n)];
     std::vector<int64_t> fill(start.begin(), start.end() - 1);
-    corners.resize(static_cast<size_t>(8 * E));
+    corners.resize(static_cast<size_t>(C));
     // corners are visited in ascending id = ascending element, then node_lid
-    for (int64_t c = 0; c < 8 * E; ++c) corners[static_cast<size_t>(fill[static_cast<size_t>(conn[c])]++)] = c;
+    for (int64_t c = 0; c < C; ++c) corners[static_cast<size_t>(fill[static_cast<size_t>(conn[c])]++)] = c;
 }
 
 void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
@@ -105,8 +110,12 @@ void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
 
 int boundary_sets(const Layout& L, int32_t inst, const int64_t* conn, const double* xyz,
                   const std::vector<int64_t>& cstart, const std::vector<int64_t>& corners,
-                  int32_t nsets, const fsgh_boundary* bc, std::vector<std::vector<int64_t>>& out)
+                  int32_t nsets, const fsgh_boundary* bc, std::vector<std::vector<int64_t>>& out,
+                  int npe)
 {
+    // patches: the 6 four-node faces of a hex, or the 4 two-node edges of a quad
+    const int nface = npe == 4 ? 4 : 6, fnodes = npe == 4 ? 2 : 4;
+    const int(*face)[4] = npe == 4 ? kQuadEdge : kFace;
     const int64_t e0 = L.elem_off[inst], e1 = L.elem_off[inst + 1];
     const int64_t n0 = L.node_off[inst], n1 = L.node_off[inst + 1];
     out.assign(static_cast<size_t>(nsets), {});
@@ -126,29 +135,29 @@ int boundary_sets(const Layout& L, int32_t inst, const int64_t* conn, const doub
         }
         std::fill(mark.begin(), mark.end(), 0);
         for (int64_t e = e0; e < e1; ++e) {
-            const int64_t* en = conn + 8 * e;
-            for (int f = 0; f < 6; ++f) {
+            const int64_t* en = conn + npe * e;
+            for (int f = 0; f < nface; ++f) {
                 int64_t fn[4];
                 bool all = true;
-                for (int q = 0; q < 4; ++q) {
-                    fn[q] = en[kFace[f][q]];
+                for (int q = 0; q < fnodes; ++q) {
+                    fn[q] = en[face[f][q]];
                     all = all && on[static_cast<size_t>(fn[q] - n0)];
                 }
                 if (!all) continue;
-                // a face is on the mesh boundary when no other element holds its 4 nodes
+                // a face is on the mesh boundary when no other element holds all its nodes
                 bool shared = false;
                 for (int64_t q = cstart[static_cast<size_t>(fn[0])];
                      q < cstart[static_cast<size_t>(fn[0] + 1)] && !shared; ++q) {
-                    const int64_t e2 = corners[static_cast<size_t>(q)] >> 3;
+                    const int64_t e2 = corners[static_cast<size_t>(q)] / npe;
                     if (e2 == e) continue;
                     int hit = 0;
-                    for (int k = 1; k < 4; ++k)
-                        for (int a = 0; a < 8; ++a)
-                            if (conn[8 * e2 + a] == fn[k]) { ++hit; break; }
-                    shared = hit == 3;
+                    for (int k = 1; k < fnodes; ++k)
+                        for (int a = 0; a < npe; ++a)
+                            if (conn[npe * e2 + a] == fn[k]) { ++hit; break; }
+                    shared = hit == fnodes - 1;
                 }
                 if (shared) continue;
-                for (int q = 0; q < 4; ++q) mark[static_cast<size_t>(fn[q] - n0)] = 1;
+                for (int q = 0; q < fnodes; ++q) mark[static_cast<size_t>(fn[q] - n0)] = 1;
             }
         }
         for (int64_t n = n0; n < n1; ++n)

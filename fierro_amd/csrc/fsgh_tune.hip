@@ -209,7 +209,7 @@ DMesh fsgh::tuned_mesh(fsgh_ctx* c, int kernel) { return mesh_for(c, kernel, c->
 const std::vector<int32_t>& fsgh::dt_block_begin(fsgh_ctx* c)
 {
     const KernelCfg& g = c->tune[kTunedEnergy];
-    if (default_table(g)) return c->eblk_begin;
+    if (default_table(g) || c->dims == 2) return c->eblk_begin;
     return c->tables.at(table_key(true, g.bs, g.xcd)).inst_begin;
 }
 
@@ -219,6 +219,10 @@ int fsgh_set_tuning(fsgh_ctx* c, const fsgh_tuning* cfg, int32_t count)
 {
     CHECK_CTX(c);
     if (c->host_only) return FSGH_E_STATE;
+    if (c->dims == 2) {
+        set_error(c, "launch tuning covers the 3D kernels only");
+        return FSGH_E_UNSUPPORTED;
+    }
     if (count < 0 || (count > 0 && !cfg)) return FSGH_E_INVALID;
     for (int32_t i = 0; i < count; ++i) {
         KernelCfg g;
@@ -256,6 +260,10 @@ int fsgh_autotune(fsgh_ctx* c, int32_t reps, fsgh_tuning* out)
 {
     CHECK_CTX(c);
     if (c->host_only) return FSGH_E_STATE;
+    if (c->dims == 2) {
+        set_error(c, "launch tuning covers the 3D kernels only");
+        return FSGH_E_UNSUPPORTED;
+    }
     if (c->nmat <= 0) {
         set_error(c, "materials not set (fsgh_set_materials)");
         return FSGH_E_STATE;

@@ -5,6 +5,11 @@ fills, boundary sets, run controls) and passing a mesh file (K/driver.cpp:138).
 Here a problem is a ``ProblemSpec``: plain data that both the HIP solver
 (``fierro_amd.sgh``) and the CPU oracle consume, so both run the same inputs.
 
+2D axisymmetric (RZ) problems have ``dims == 2``: quads in the (axial x,
+radius y) half plane from ``rect_mesh`` (node (i,j) = i + (nx+1) j, element
+nodes (i,j),(i+1,j),(i+1,j+1),(i,j+1), counter-clockwise as the reference's
+2D meshes, K/mesh.h:366-373) or a 2D EnSight file (``read_ensight_2d``).
+
 Mesh generator: ``box_mesh`` numbers nodes i-fastest, then j, then k, and
 elements with nodes (i,j,k),(i+1,j,k),(i+1,j+1,k),(i,j+1,k) followed by the
 k+1 layer -- the numbering of the reference's EnSight meshes (e.g.
@@ -52,16 +57,23 @@ class ProblemSpec:
     run: dict = field(default_factory=lambda: dict(DEFAULT_RUN))
     num_state_vars: int = 6
     eos: list = None                # per material: EOS_IDEAL_GAS (default) or EOS_USER
+    dims: int = 3                   # 2: axisymmetric RZ (mesh kinds "rect", "geo2")
 
     def mesh_arrays(self, oracle_gen=None, oracle_read=None):
-        """(coords N x 3, nodes_in_elem E x 8).  The oracle passes its own
-        generator/reader so that its inputs come from its own code."""
+        """(coords N x dims, nodes_in_elem E x 2^dims).  The oracle passes its
+        own generator/reader so that its inputs come from its own code."""
         m = self.mesh
         if m["kind"] == "box":
             gen = oracle_gen or box_mesh
             return gen(m["nx"], m["ny"], m["nz"], m["lx"], m["ly"], m["lz"], m["lden"])
         if m["kind"] == "geo":
             rd = oracle_read or read_ensight
+            return rd(m["path"])
+        if m["kind"] == "rect":
+            gen = oracle_gen or rect_mesh
+            return gen(m["nx"], m["ny"], m["lx"], m["ly"], m["lden"])
+        if m["kind"] == "geo2":
+            rd = oracle_read or read_ensight_2d
             return rd(m["path"])
         raise ValueError(m["kind"])
 
@@ -73,7 +85,9 @@ class ProblemSpec:
     @property
     def num_elems(self):
         m = self.mesh
-        return m["nx"] * m["ny"] * m["nz"] if m["kind"] == "box" else None
+        if m["kind"] == "box":
+            return m["nx"] * m["ny"] * m["nz"]
+        return m["nx"] * m["ny"] if m["kind"] == "rect" else None
 
 
 def box_mesh(nx, ny, nz, lx_num=1.0, ly_num=1.0, lz_num=1.0, l_den=1.0):
@@ -93,6 +107,38 @@ def box_mesh(nx, ny, nz, lx_num=1.0, ly_num=1.0, lz_num=1.0, l_den=1.0):
     conn[:, 3] = n0 + nx1
     conn[:, 4:] = conn[:, :4] + nx1 * ny1
     return coords, conn
+
+
+def rect_mesh(nx, ny, lx_num=1.0, ly_num=1.0, l_den=1.0):
+    """Structured quads on [0,Lx]x[0,Ly] (x axial, y radius), L = L_num/L_den."""
+    nx1, ny1 = nx + 1, ny + 1
+    j, i = np.meshgrid(np.arange(ny1), np.arange(nx1), indexing="ij")
+    coords = np.empty((nx1 * ny1, 2))
+    coords[:, 0] = (i.ravel().astype(np.float64) * lx_num) / (float(nx) * l_den)
+    coords[:, 1] = (j.ravel().astype(np.float64) * ly_num) / (float(ny) * l_den)
+    je, ie = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    n0 = (ie + nx1 * je).ravel().astype(np.int64)
+    conn = np.empty((nx * ny, 4), np.int64)
+    conn[:, 0] = n0
+    conn[:, 1] = n0 + 1
+    conn[:, 2] = n0 + 1 + nx1
+    conn[:, 3] = n0 + nx1
+    return coords, conn
+
+
+def read_ensight_2d(path):
+    """EnSight Gold ASCII 2D reader following K/read_mesh.cpp:12-147 with
+    num_dims = 2: x and y blocks kept, the z block read and dropped, 4 node
+    ids per element."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    toks = " ".join(lines[8:]).split()
+    n = int(toks[0])
+    xyz = np.array(toks[1:1 + 3 * n], dtype=np.float64).reshape(3, n).T
+    rest = toks[1 + 3 * n:]
+    ne = int(rest[1])
+    conn = np.array(rest[2:2 + 4 * ne], dtype=np.int64).reshape(ne, 4) - 1
+    return np.ascontiguousarray(xyz[:, :2]), conn
 
 
 def read_ensight(path):
@@ -185,6 +231,82 @@ def sedov_user_region(n=8, cycles=100):
                                     x1=0.6, x2=1.2, y1=0.0, y2=1.2, z1=0.0, z2=1.2)]
     spec.eos = [EOS_IDEAL_GAS, EOS_USER]
     return spec
+
+PI = 3.141592653589793   # K/mesh.h:8
+
+
+def _rz(spec):
+    spec.dims = 2
+    return spec
+
+
+def sedov_rz(n=32, cycles=200, q2=1.3333, dt_cfl=0.4):
+    """2D RZ Sedov blast (the reference's SedovRZ test input,
+    tests-for-Explicit-Lagrange-Kokkos/input_for_SedovRZ_test.cpp:110-190):
+    n x n quads on [0,1.2]^2 in (axial, radius), rho=1, e=1e-10, energy
+    0.5 * 0.49339 / (rho V_s) in the sphere of radius 1.2/n, with
+    V_s = PI (r2^3 - r1^3); reflected planes x=0, y=0 (the axis), x=1.2,
+    y=1.2; gamma 5/3, q1=1, q2=1.3333, CFL 0.4 (K/driver.cpp:49)."""
+    import math
+    r2 = 1.2 / n
+    vol = PI * (math.pow(r2, 3) - math.pow(0.0, 3))
+    spec = _rz(ProblemSpec(
+        name=f"sedov_rz{n}",
+        mesh=dict(kind="rect", nx=n, ny=n, lx=12.0, ly=12.0, lden=10.0),
+        materials=[(1.0, 1.0, q2, 0.0)],
+        state_vars=_gas(5.0 / 3.0),
+        mat_num_state_vars=[3],
+        fills=[fill(GLOBAL, den=1.0, sie=1.0e-10),
+               fill(SPHERE, den=1.0, sie=(0.5 * 0.49339 / (1.0 * vol)), radius1=0.0,
+                    radius2=r2)],
+        bcs=[(X_PLANE, 0.0, REFLECTED), (Y_PLANE, 0.0, REFLECTED),
+             (X_PLANE, 1.2, REFLECTED), (Y_PLANE, 1.2, REFLECTED)],
+    ))
+    spec.run.update(time_final=1.0, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
+                    dt_cfl=dt_cfl, graphics_dt_ival=1.0e16, graphics_cyc_ival=1000000000,
+                    cycle_stop=cycles)
+    return spec
+
+
+def noh_rz(n=32, cycles=100, dt_cfl=0.4):
+    """2D RZ Noh implosion (input_for_NohRZ_test.cpp:110-160): n x n quads on
+    [0,1]^2, rho=1, e=1e-9, radial inward velocity of speed 1 in the
+    (axial, radius) plane, reflected x=0 and y=0, gamma 5/3, q2 = 1.3333."""
+    spec = _rz(ProblemSpec(
+        name=f"noh_rz{n}",
+        mesh=dict(kind="rect", nx=n, ny=n, lx=1.0, ly=1.0, lden=1.0),
+        materials=[(1.0, 1.0, 1.3333, 0.0)],
+        state_vars=_gas(5.0 / 3.0),
+        mat_num_state_vars=[3],
+        fills=[fill(GLOBAL, den=1.0, sie=1e-9, velocity=RADIAL, speed=-1.0)],
+        bcs=[(X_PLANE, 0.0, REFLECTED), (Y_PLANE, 0.0, REFLECTED)],
+    ))
+    spec.run.update(time_final=0.6, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
+                    dt_cfl=dt_cfl, graphics_dt_ival=1.0e16, graphics_cyc_ival=1000000000,
+                    cycle_stop=cycles)
+    return spec
+
+
+def sod_rz(nx=100, ny=10, cycles=200, dt_cfl=0.4):
+    """Sod along the axis in RZ coordinates (K/input.cpp:393-476, SodZ):
+    [0,1] x [0,0.1], rho=1, e=2.5 (p=1) and, in x >= 0.5, rho=0.125, e=2.5
+    (p=0.125), gamma 1.4; reflected x=0, y=0, x=1, y=0.1."""
+    spec = _rz(ProblemSpec(
+        name=f"sod_rz{nx}x{ny}",
+        mesh=dict(kind="rect", nx=nx, ny=ny, lx=1.0, ly=0.1, lden=1.0),
+        materials=[(1.0, 1.0, 1.3333, 0.0)],
+        state_vars=_gas(1.4),
+        mat_num_state_vars=[3],
+        fills=[fill(GLOBAL, den=1.0, sie=2.5),
+               fill(BOX, den=0.125, sie=2.5, x1=0.5, x2=1.0, y1=0.0, y2=1.0, z1=0.0, z2=1.0)],
+        bcs=[(X_PLANE, 0.0, REFLECTED), (Y_PLANE, 0.0, REFLECTED),
+             (X_PLANE, 1.0, REFLECTED), (Y_PLANE, 0.1, REFLECTED)],
+    ))
+    spec.run.update(time_final=0.2, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
+                    dt_cfl=dt_cfl, graphics_dt_ival=1.0e16, graphics_cyc_ival=1000000000,
+                    cycle_stop=cycles)
+    return spec
+
 
 def reference_sedov(mesh_path):
     """The reference's default compiled-in problem (K/input.cpp:69-202 with

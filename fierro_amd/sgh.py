@@ -8,7 +8,9 @@ Every method runs on the GPU through libfsgh.so; there is no CPU path.
 
 A list of ProblemSpecs makes one batched context: the instances are packed as
 a ragged mesh (nodes and elements concatenated in instance order), each with
-its own boundary sets, materials, dt and stop state.
+its own boundary sets, materials, dt and stop state.  Specs with ``dims == 2``
+make a 2D axisymmetric (RZ) context: quads, node fields with 2 components
+(axial, radius), corner forces [4E][2].
 """
 from __future__ import annotations
 
@@ -20,8 +22,8 @@ from . import _abi
 from ._abi import check
 
 FIELD_SHAPES = {
-    _abi.NODE_COORDS: ("node", (2, None, 3), np.float64),
-    _abi.NODE_VEL: ("node", (2, None, 3), np.float64),
+    _abi.NODE_COORDS: ("node", (2, None, "D"), np.float64),
+    _abi.NODE_VEL: ("node", (2, None, "D"), np.float64),
     _abi.NODE_MASS: ("node", (None,), np.float64),
     _abi.ELEM_DEN: ("elem", (None,), np.float64),
     _abi.ELEM_PRES: ("elem", (None,), np.float64),
@@ -33,7 +35,7 @@ FIELD_SHAPES = {
     _abi.ELEM_MAT_ID: ("elem", (None,), np.int64),
     _abi.ELEM_GAMMA: ("elem", (None,), np.float64),
     _abi.ELEM_CSMIN: ("elem", (None,), np.float64),
-    _abi.CORNER_FORCE: ("corner", (None, 3), np.float64),
+    _abi.CORNER_FORCE: ("corner", (None, "D"), np.float64),
 }
 FIELDS = {"node_coords": _abi.NODE_COORDS, "node_vel": _abi.NODE_VEL,
           "node_mass": _abi.NODE_MASS, "elem_den": _abi.ELEM_DEN, "elem_pres": _abi.ELEM_PRES,
@@ -45,6 +47,13 @@ FIELDS = {"node_coords": _abi.NODE_COORDS, "node_vel": _abi.NODE_VEL,
 
 def _p(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _dims(specs):
+    dims = {getattr(s, "dims", 3) for s in specs}
+    if len(dims) != 1:
+        raise ValueError("a batch holds problems of one dimension")
+    return dims.pop()
 
 
 def pack(meshes):
@@ -68,6 +77,7 @@ class SGHSolver:
         if not isinstance(specs, (list, tuple)):
             specs = [specs]
         self.specs = list(specs)
+        self.dims = _dims(self.specs)
         if meshes is None:
             meshes = [s.mesh_arrays() for s in self.specs]
         coords, conn, noff, eoff = pack(meshes)
@@ -77,7 +87,7 @@ class SGHSolver:
         self.ninst = len(self.specs)
         desc = _abi.MeshDesc(self.N, self.E, self.ninst, 1 if renumber else 0,
                              _p(noff).value if self.ninst > 1 else None,
-                             _p(eoff).value if self.ninst > 1 else None)
+                             _p(eoff).value if self.ninst > 1 else None, self.dims, 0)
         ctx = C.c_void_p()
         rc = self.L.fsgh_create(C.byref(desc), _p(conn), _p(coords), device, C.byref(ctx))
         self.ctx = ctx
@@ -131,8 +141,9 @@ class SGHSolver:
     # -- fields --
     def _shape(self, fid):
         kind, shp, dt = FIELD_SHAPES[fid]
-        n = {"node": self.N, "elem": self.E, "corner": 8 * self.E}[kind]
-        return tuple(n if s is None else s for s in shp), dt
+        npe = 4 if self.dims == 2 else 8
+        n = {"node": self.N, "elem": self.E, "corner": npe * self.E}[kind]
+        return tuple(n if s is None else (self.dims if s == "D" else s) for s in shp), dt
 
     def download(self, name):
         fid = FIELDS[name]
@@ -151,7 +162,7 @@ class SGHSolver:
 
     def corners_in_node(self):
         start = np.zeros(self.N + 1, np.int64)
-        corners = np.zeros(8 * self.E, np.int64)
+        corners = np.zeros((4 if self.dims == 2 else 8) * self.E, np.int64)
         check(self.L.fsgh_get_corners_in_node(self.ctx, _p(start), _p(corners)), self.ctx, "cin")
         return start, corners
 
@@ -164,7 +175,7 @@ class SGHSolver:
     def device_csr(self):
         """The device's corner CSR in internal numbering (diagnostic)."""
         start = np.zeros(self.N + 1, np.int32)
-        corners = np.zeros(8 * self.E, np.int32)
+        corners = np.zeros((4 if self.dims == 2 else 8) * self.E, np.int32)
         check(self.L.fsgh_get_device_csr(self.ctx, _p(start), _p(corners)), self.ctx, "device_csr")
         return start, corners
 
@@ -238,6 +249,15 @@ class SGHSolver:
         self.update_position_sgh(rk_alpha, dt)
         self.get_vol()
         self.update_state(dt, rk_alpha)
+        if self.dims == 2:
+            self.rz_update_node_mass(r["tiny"])
+
+    # -- 2D RZ node mass (K/sgh_solve.cpp:107-116, 495-581) --
+    def rz_extensive_mass(self):
+        check(self.L.fsgh_rz_extensive_mass(self.ctx), self.ctx, "rz_extensive_mass")
+
+    def rz_update_node_mass(self, tiny=1.0e-12):
+        check(self.L.fsgh_rz_update_node_mass(self.ctx, float(tiny)), self.ctx, "rz_node_mass")
 
     # -- fused cycle loop --
     def _run_params(self):
@@ -247,7 +267,7 @@ class SGHSolver:
             ps.append(_abi.RunParams(r["time_final"], r["dt_start"], r["dt_max"], r["dt_min"],
                                      r["dt_cfl"], r["graphics_dt_ival"],
                                      int(r["graphics_cyc_ival"]), int(r["cycle_stop"]),
-                                     r["fuzz"], r["small"]))
+                                     r["fuzz"], r["small"], r.get("tiny", 1.0e-12)))
         return (_abi.RunParams * len(ps))(*ps)
 
     def run_begin(self):
@@ -394,6 +414,7 @@ class MeshMaps:
         if not isinstance(specs, (list, tuple)):
             specs = [specs]
         self.specs = list(specs)
+        self.dims = _dims(self.specs)
         if meshes is None:
             meshes = [s.mesh_arrays() for s in self.specs]
         coords, conn, noff, eoff = pack(meshes)
@@ -401,7 +422,8 @@ class MeshMaps:
         self.N, self.E = coords.shape[0], conn.shape[0]
         n = len(self.specs)
         desc = _abi.MeshDesc(self.N, self.E, n, 1 if renumber else 0,
-                             _p(noff).value if n > 1 else None, _p(eoff).value if n > 1 else None)
+                             _p(noff).value if n > 1 else None, _p(eoff).value if n > 1 else None,
+                             self.dims, 0)
         ctx = C.c_void_p()
         rc = self.L.fsgh_create(C.byref(desc), _p(conn), _p(coords), -1, C.byref(ctx))
         self.ctx = ctx

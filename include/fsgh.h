@@ -1,9 +1,10 @@
 /*
  * fsgh.h -- C ABI of the MI355X-native SGH (staggered-grid hydrodynamics) cycle.
  *
- * The library replaces the per-stage operators of Fierro's 3D explicit
+ * The library replaces the per-stage operators of Fierro's explicit
  * Lagrangian solver (reference: Explicit-Lagrange-Kokkos/SGH_solver/, cited
- * below as K/) with hand-written HIP kernels for gfx950, behind plain C entry
+ * below as K/) -- 3D hexahedra and the 2D axisymmetric (RZ) quadrilateral
+ * path -- with hand-written HIP kernels for gfx950, behind plain C entry
  * points: no C++ or torch types cross this boundary.
  *
  * Model of use (mirrors K/driver.cpp:94-313):
@@ -22,6 +23,9 @@
  *   - Field layouts at the ABI are the reference's (row-major MATAR):
  *     coordinates/velocity [2 bins][N][3], sie [2 bins][E], corner force
  *     [8E][3] with corner = 8e+i.  Indices are int64 (reference: size_t).
+ *     A 2D RZ context (fsgh_mesh_desc.num_dims = 2) has num_dims = 2 in
+ *     these shapes: coordinates/velocity [2 bins][N][2] with (x, y) =
+ *     (axial, radius), corner force [4E][2], nodes_in_elem E x 4.
  *     Internally the library keeps SoA FP64 fields and int32 connectivity,
  *     optionally renumbered along a Morton curve; uploads and downloads
  *     convert, so callers always see their own numbering.
@@ -49,7 +53,7 @@ extern "C" {
 #define FSGH_E_NOMEM (-3)       /* device or host allocation failed */
 #define FSGH_E_NONFINITE (-4)   /* non-finite or non-physical state found by fsgh_check */
 #define FSGH_E_STATE (-5)       /* call out of order (e.g. run before setup) */
-#define FSGH_E_UNSUPPORTED (-6) /* feature outside this build (strength models, 2D-RZ) */
+#define FSGH_E_UNSUPPORTED (-6) /* feature outside this build (e.g. strength models) */
 #define FSGH_E_NOTBUILT (-7)    /* library built without GPU code (never in a product build) */
 
 typedef struct fsgh_ctx fsgh_ctx;
@@ -62,14 +66,18 @@ typedef struct fsgh_mesh_desc {
     int32_t renumber;                  /* 0: keep caller numbering, 1: Morton order */
     const int64_t *instance_node_off;  /* num_instances+1 entries, or NULL if 1 instance */
     const int64_t *instance_elem_off;  /* num_instances+1 entries, or NULL if 1 instance */
+    int32_t num_dims;                  /* 3 (or 0): hexahedra; 2: axisymmetric RZ quads */
+    int32_t reserved_;
 } fsgh_mesh_desc;
 
 /* Creates a context on HIP device `device`: builds the corner connectivity
  * (corners_in_node in ascending element order, K/mesh.h:180-245), copies the
  * coordinates into both RK bins (K/read_mesh.cpp:129-137), allocates device
- * state.  nodes_in_elem: E x 8, 0-based, node ids global over the batch.
- * Replaces: read_mesh_ensight + mesh.build_corner_connectivity
- * (K/driver.cpp:138-139). */
+ * state.  nodes_in_elem: E x 8 (E x 4 counter-clockwise for num_dims = 2),
+ * 0-based, node ids global over the batch; coords N x num_dims.  Replaces:
+ * read_mesh_ensight + mesh.build_corner_connectivity (K/driver.cpp:138-139).
+ * A 2D context builds its maps, boundary sets and fills on the host (the
+ * reference's set-up, untimed); its cycle runs on the GPU. */
 int fsgh_create(const fsgh_mesh_desc *desc, const int64_t *nodes_in_elem,
                 const double *coords, int device, fsgh_ctx **out);
 /* device < 0 creates a host-only context: connectivity, renumbering and
@@ -118,9 +126,11 @@ typedef struct fsgh_boundary {
     int32_t hydro_bc;                  /* 0 fixed, 1 reflected; other values: no-op */
     double value;                      /* plane position or radius */
 } fsgh_boundary;
-/* Tags the boundary patches of `instance` whose 4 nodes lie within 1e-7 of
- * the surface (K/setup.cpp:487-571) and stores each set's nodes.  Replaces
- * mesh.init_bdy_sets + tag_bdys + build_boundry_node_sets. */
+/* Tags the boundary patches of `instance` whose 4 nodes (2 in 2D) lie
+ * within 1e-7 of the surface (K/setup.cpp:487-571) and stores each set's
+ * nodes.  Replaces mesh.init_bdy_sets + tag_bdys + build_boundry_node_sets.
+ * 2D contexts accept surfaces 0, 1 and 3 (the reference's check_bdy reads an
+ * unset third coordinate for 2 and 4). */
 int fsgh_set_bcs(fsgh_ctx *ctx, int32_t instance, int32_t num_sets,
                  const fsgh_boundary *sets);
 /* number of nodes in boundary set `set` of `instance`, and the nodes (caller
@@ -144,7 +154,10 @@ typedef struct fsgh_fill {
 /* Paints fills in order over `instance` by element centroid, sets M = rho V,
  * e, statev, EOS, node velocity, applies the boundary velocity and the nodal
  * mass sum_{elems of n} M_e/8 (K/setup.cpp:106-424).  state_vars is
- * num_materials x num_state_vars; mat_num_state_vars[m] of them are copied. */
+ * num_materials x num_state_vars; mat_num_state_vars[m] of them are copied.
+ * 2D: V is the RZ volume per radian and the node mass is the areal mass
+ * sum over the node's corners of (Barlow corner area x density)
+ * (K/setup.cpp:365-420); velocity kinds 0 and 1 (2: FSGH_E_UNSUPPORTED). */
 int fsgh_setup_fills(fsgh_ctx *ctx, int32_t instance, int32_t num_fills,
                      const fsgh_fill *fills, int32_t num_materials,
                      int32_t num_state_vars, const double *state_vars,
@@ -188,8 +201,15 @@ int fsgh_update_position_sgh(fsgh_ctx *ctx, double rk_alpha, double dt); /* K/ge
 int fsgh_get_vol(fsgh_ctx *ctx);                                    /* K/geometry.cpp:238-310 */
 int fsgh_update_state(fsgh_ctx *ctx, double dt, double rk_alpha);   /* K/properties.cpp:9-114 */
 /* extensive energies IE = sum M e, KE = 1/2 sum m |v|^2 of one instance
- * (K/sgh_solve.cpp:66-104); synchronises. */
+ * (K/sgh_solve.cpp:66-104; 2D: KE = 1/2 sum m R |v|^2); synchronises. */
 int fsgh_energy(fsgh_ctx *ctx, int32_t instance, double *ie, double *ke);
+/* 2D RZ only.  The node extensive mass = areal node mass x radius, which
+ * sgh_solve takes once at its start (K/sgh_solve.cpp:107-116), and the
+ * per-stage areal mass update from it with the on-axis fix
+ * (K/sgh_solve.cpp:495-581; tiny = 1e-12 in K/driver.cpp:61).  3D
+ * contexts: FSGH_E_STATE. */
+int fsgh_rz_extensive_mass(fsgh_ctx *ctx);
+int fsgh_rz_update_node_mass(fsgh_ctx *ctx, double tiny);
 
 /* -------------------------------------------------- fused cycle loop --- */
 typedef struct fsgh_run_params {
@@ -200,6 +220,7 @@ typedef struct fsgh_run_params {
     int64_t graphics_cyc_ival;         /* K/driver.cpp:36 */
     int64_t cycle_stop;                /* K/driver.cpp:56 */
     double fuzz, small;                /* K/driver.cpp:60-62 */
+    double tiny;                       /* K/driver.cpp:61 (2D RZ node mass) */
 } fsgh_run_params;
 
 typedef struct fsgh_instance_status {

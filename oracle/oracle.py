@@ -329,3 +329,262 @@ def vol_hex(x, y, z):
     return lib().orc_get_vol_hex(np.ascontiguousarray(x, np.float64),
                                  np.ascontiguousarray(y, np.float64),
                                  np.ascontiguousarray(z, np.float64))
+
+
+# ---------------------------------------------------------------------------
+# 2D axisymmetric (RZ) path: rz_oracle.c
+# ---------------------------------------------------------------------------
+class OrzMesh(C.Structure):
+    _fields_ = [("num_nodes", C.c_int64), ("num_elems", C.c_int64),
+                ("nodes_in_elem", P), ("corners_in_node_start", P), ("corners_in_node", P),
+                ("elems_in_elem_start", P), ("elems_in_elem", P),
+                ("num_bdy_sets", C.c_int64), ("bdy_nodes_in_set_start", P),
+                ("bdy_nodes_in_set", P), ("num_bdy_nodes", C.c_int64), ("bdy_nodes", P),
+                ("nodes_in_node_start", P), ("nodes_in_node", P)]
+
+
+def _rz_lib(fast=False):
+    L = lib(fast)
+    if not getattr(L, "_rz_ready", False):
+        L.orz_get_vol_quad.restype = C.c_double
+        L.orz_get_area_quad.restype = C.c_double
+        L.orz_heron.restype = C.c_double
+        for f in ("orz_get_vol_quad", "orz_get_area_quad"):
+            getattr(L, f).argtypes = [f64p, f64p]
+        L.orz_heron.argtypes = [C.c_double] * 6
+        L.orz_get_bmatrix2D.argtypes = [f64p, f64p, f64p]
+        L.orz_get_area_weights2D.argtypes = [f64p, f64p, f64p]
+        L.orz_build_elem_elem_connectivity.restype = C.c_int64
+        L.orz_build_patch_connectivity.restype = C.c_int64
+        L.orz_build_bdy_sets.restype = C.c_int64
+        L.orz_read_ensight.restype = C.c_int
+        L._rz_ready = True
+    return L
+
+
+def gen_rect(nx, ny, lx_num, ly_num, l_den):
+    N, E = (nx + 1) * (ny + 1), nx * ny
+    coords = np.zeros((N, 2))
+    conn = np.zeros((E, 4), np.int64)
+    _rz_lib().orz_gen_rect(C.c_int64(nx), C.c_int64(ny), C.c_double(lx_num), C.c_double(ly_num),
+                           C.c_double(l_den), _ptr(coords), _ptr(conn))
+    return coords, conn
+
+
+def read_ensight_2d(path: str):
+    L = _rz_lib()
+    nn, ne = C.c_int64(), C.c_int64()
+    rc = L.orz_read_ensight(path.encode(), C.byref(nn), C.byref(ne), None, None)
+    if rc != 0:
+        raise IOError(f"orz_read_ensight({path}) -> {rc}")
+    coords = np.zeros((nn.value, 2))
+    conn = np.zeros((ne.value, 4), np.int64)
+    rc = L.orz_read_ensight(path.encode(), C.byref(nn), C.byref(ne), _ptr(coords), _ptr(conn))
+    if rc != 0:
+        raise IOError(f"orz_read_ensight({path}) -> {rc}")
+    return coords, conn
+
+
+def connectivity_rz(coords: np.ndarray, conn: np.ndarray):
+    """The reference's 2D connectivity (K/mesh.h:180-796, num_dims = 2)."""
+    L = _rz_lib()
+    N, E = coords.shape[0], conn.shape[0]
+    conn = np.ascontiguousarray(conn, np.int64)
+    cin_start = np.zeros(N + 1, np.int64)
+    cin = np.zeros(4 * E, np.int64)
+    ein = np.zeros(4 * E, np.int64)
+    L.orz_build_corner_connectivity(C.c_int64(N), C.c_int64(E), _ptr(conn), _ptr(cin_start),
+                                    _ptr(cin), _ptr(ein))
+    ee_start = np.zeros(E + 1, np.int64)
+    nb = L.orz_build_elem_elem_connectivity(C.c_int64(N), C.c_int64(E), _ptr(conn),
+                                            _ptr(cin_start), _ptr(ein), _ptr(ee_start), None)
+    ee = np.zeros(max(nb, 1), np.int64)
+    L.orz_build_elem_elem_connectivity(C.c_int64(N), C.c_int64(E), _ptr(conn), _ptr(cin_start),
+                                       _ptr(ein), _ptr(ee_start), _ptr(ee))
+    counts = np.zeros(3, np.int64)
+    L.orz_build_patch_connectivity(C.c_int64(N), C.c_int64(E), _ptr(conn), _ptr(ee_start),
+                                   _ptr(ee), None, None, None, _ptr(counts))
+    P_, BP, BN = (int(c) for c in counts)
+    nip = np.zeros((max(P_, 1), 2), np.int64)
+    bp = np.zeros(max(BP, 1), np.int64)
+    bn = np.zeros(max(BN, 1), np.int64)
+    L.orz_build_patch_connectivity(C.c_int64(N), C.c_int64(E), _ptr(conn), _ptr(ee_start),
+                                   _ptr(ee), _ptr(nip), _ptr(bp), _ptr(bn), _ptr(counts))
+    nn_start = np.zeros(N + 1, np.int64)
+    nn = np.zeros(max(2 * P_, 1), np.int64)
+    L.orz_build_node_node_connectivity(C.c_int64(N), C.c_int64(P_), _ptr(nip), _ptr(nn_start),
+                                       _ptr(nn))
+    return dict(corners_in_node_start=cin_start, corners_in_node=cin, elems_in_node=ein,
+                elems_in_elem_start=ee_start, elems_in_elem=ee[:nb], nodes_in_patch=nip[:P_],
+                bdy_patches=bp[:BP], bdy_nodes=bn[:BN], num_patches=P_,
+                nodes_in_node_start=nn_start, nodes_in_node=nn[:2 * P_])
+
+
+def bdy_sets_rz(coords, cx, bcs):
+    L = _rz_lib()
+    ns = len(bcs)
+    arr = (OrcBoundary * max(ns, 1))(*[OrcBoundary(int(b[0]), float(b[1]), int(b[2])) for b in bcs])
+    start = np.zeros(ns + 1, np.int64)
+    cap = max(ns * max(len(cx["bdy_patches"]), 1) * 2, 1)
+    out = np.zeros(cap, np.int64)
+    xy = np.ascontiguousarray(coords, np.float64)
+    tot = L.orz_build_bdy_sets(_ptr(xy), C.c_int64(len(cx["bdy_patches"])),
+                               _ptr(cx["bdy_patches"]), _ptr(cx["nodes_in_patch"]),
+                               C.c_int64(ns), arr, C.c_int64(xy.shape[0]), _ptr(start), _ptr(out))
+    return start, out[:tot].copy(), arr
+
+
+class OracleRZ:
+    """One 2D RZ problem in the reference layout (num_dims = 2), set up and
+    run as K/driver.cpp and K/sgh_solve.cpp do on that branch."""
+
+    def __init__(self, spec, fast: bool = False, coords=None, conn=None):
+        self.L = _rz_lib(fast)
+        self.spec = spec
+        if coords is None:
+            coords, conn = spec.mesh_arrays(oracle_gen=gen_rect, oracle_read=read_ensight_2d)
+        self.coords0 = np.ascontiguousarray(coords, np.float64)
+        self.conn = np.ascontiguousarray(conn, np.int64)
+        N, E = self.coords0.shape[0], self.conn.shape[0]
+        self.N, self.E = N, E
+        self.cx = connectivity_rz(self.coords0, self.conn)
+        self.bset_start, self.bset_nodes, self._bc = bdy_sets_rz(self.coords0, self.cx, spec.bcs)
+        nsv = spec.num_state_vars
+        self.node_coords = np.zeros((2, N, 2))
+        self.node_coords[0] = self.coords0
+        self.node_coords[1] = self.coords0
+        self.node_vel = np.zeros((2, N, 2))
+        self.node_mass = np.zeros(N)
+        self.node_ext = np.zeros(N)
+        z = lambda: np.zeros(E)  # noqa: E731
+        self.elem_den, self.elem_pres, self.elem_sspd = z(), z(), z()
+        self.elem_vol, self.elem_div, self.elem_mass = z(), z(), z()
+        self.elem_sie = np.zeros((2, E))
+        self.elem_mat_id = np.zeros(E, np.int64)
+        self.elem_statev = np.zeros((E, nsv))
+        self.corner_force = np.zeros((4 * E, 2))
+        cx = self.cx
+        self._mesh = OrzMesh(N, E, _ptr(self.conn), _ptr(cx["corners_in_node_start"]),
+                             _ptr(cx["corners_in_node"]), _ptr(cx["elems_in_elem_start"]),
+                             _ptr(cx["elems_in_elem"]), len(spec.bcs), _ptr(self.bset_start),
+                             _ptr(self.bset_nodes), len(cx["bdy_nodes"]), _ptr(cx["bdy_nodes"]),
+                             _ptr(cx["nodes_in_node_start"]), _ptr(cx["nodes_in_node"]))
+        self._state = OrcState(_ptr(self.node_coords), _ptr(self.node_vel), _ptr(self.node_mass),
+                               _ptr(self.elem_den), _ptr(self.elem_pres), _ptr(self.elem_sspd),
+                               _ptr(self.elem_sie), _ptr(self.elem_vol), _ptr(self.elem_div),
+                               _ptr(self.elem_mass), _ptr(self.elem_mat_id),
+                               _ptr(self.elem_statev), nsv, _ptr(self.corner_force), None)
+        if spec.eos:
+            self.mat_eos = np.ascontiguousarray(spec.eos, np.int64)
+            self._state.mat_eos = _ptr(self.mat_eos)
+        mats = spec.materials
+        self._mat = (OrcMaterial * len(mats))(*[OrcMaterial(*m) for m in mats])
+        self.setup()
+
+    def _ms(self):
+        return C.byref(self._mesh), C.byref(self._state)
+
+    def setup(self):
+        L, spec = self.L, self.spec
+        m, s = self._ms()
+        L.orz_get_vol(m, s)
+        fills = (OrcFill * len(spec.fills))(*[OrcFill(**f) for f in spec.fills])
+        sv = np.ascontiguousarray(spec.state_vars, np.float64)
+        nsv = np.ascontiguousarray(spec.mat_num_state_vars, np.int64)
+        L.orz_setup_fills(m, s, C.c_int64(len(spec.fills)), fills, _ptr(sv), _ptr(nsv))
+        L.orz_boundary_velocity(m, s, self._bc)
+        L.orz_node_mass(m, s)
+
+    def get_vol(self):
+        self.L.orz_get_vol(*self._ms())
+
+    def get_divergence(self):
+        self.L.orz_get_divergence(*self._ms())
+
+    def get_force_sgh(self, dt, rk_alpha):
+        r = self.spec.run
+        self.L.orz_get_force(*self._ms(), self._mat, C.c_double(r["fuzz"]), C.c_double(r["small"]))
+
+    def update_velocity_sgh(self, rk_alpha, dt):
+        self.L.orz_update_velocity(*self._ms(), C.c_double(rk_alpha), C.c_double(dt))
+
+    def boundary_velocity(self):
+        self.L.orz_boundary_velocity(*self._ms(), self._bc)
+
+    def update_energy_sgh(self, rk_alpha, dt):
+        self.L.orz_update_energy(*self._ms(), C.c_double(rk_alpha), C.c_double(dt))
+
+    def update_position_sgh(self, rk_alpha, dt):
+        self.L.orz_update_position(*self._ms(), C.c_double(rk_alpha), C.c_double(dt))
+
+    def update_state(self):
+        self.L.orz_update_state(*self._ms())
+
+    def rk_init(self):
+        self.L.orz_rk_init(*self._ms())
+
+    def extensive_mass(self):
+        m, s = self._ms()
+        self.L.orz_extensive_mass(m, s, _ptr(self.node_ext))
+
+    def update_node_mass(self, tiny=None):
+        tiny = self.spec.run["tiny"] if tiny is None else tiny
+        m, s = self._ms()
+        self.L.orz_update_node_mass(m, s, _ptr(self.node_ext), C.c_double(tiny))
+
+    def get_timestep(self, time_value, graphics_time, dt):
+        r = self.spec.run
+        d = C.c_double(dt)
+        self.L.orz_get_timestep(*self._ms(), C.c_double(time_value), C.c_double(graphics_time),
+                                C.c_double(r["time_final"]), C.c_double(r["dt_max"]),
+                                C.c_double(r["dt_min"]), C.c_double(r["dt_cfl"]), C.byref(d),
+                                C.c_double(r["fuzz"]))
+        return d.value
+
+    def energy(self):
+        ie, ke = C.c_double(), C.c_double()
+        self.L.orz_energy_tallies(*self._ms(), C.byref(ie), C.byref(ke))
+        return ie.value, ke.value
+
+    def solve(self, cycle_stop=None, dt_replay=None, record_dt=False):
+        r = dict(self.spec.run)
+        if cycle_stop is not None:
+            r["cycle_stop"] = cycle_stop
+        ncyc = int(r["cycle_stop"])
+        rep = None if dt_replay is None else np.ascontiguousarray(dt_replay, np.float64)
+        rec = np.zeros(ncyc) if record_dt else None
+        p = OrcRunParams(0.0, r["time_final"], r["dt_start"], r["dt_max"], r["dt_min"],
+                         r["dt_cfl"], r["graphics_dt_ival"], r["graphics_dt_ival"],
+                         int(r["graphics_cyc_ival"]), ncyc, 2, r["fuzz"], r["small"],
+                         _ptr(rep), _ptr(rec))
+        res = OrcRunResult()
+        self.L.orz_sgh_solve(*self._ms(), self._mat, self._bc, C.byref(p),
+                             C.c_double(r["tiny"]), C.byref(res))
+        out = {k: getattr(res, k) for k, _ in OrcRunResult._fields_}
+        if record_dt:
+            out["dt_history"] = rec[:res.cycles]
+        return out
+
+    def stage(self, dt, rk_alpha):
+        """One RK stage of the 2D branch (K/sgh_solve.cpp:355-581)."""
+        self.get_divergence()
+        self.get_force_sgh(dt, rk_alpha)
+        self.update_velocity_sgh(rk_alpha, dt)
+        self.boundary_velocity()
+        self.update_energy_sgh(rk_alpha, dt)
+        self.update_position_sgh(rk_alpha, dt)
+        self.get_vol()
+        self.update_state()
+        self.update_node_mass()
+
+
+def rz_geometry(x, y):
+    """(B [4][2], area, RZ volume, corner areas [4]) of one quad."""
+    L = _rz_lib()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    B = np.zeros(8)
+    A = np.zeros(4)
+    L.orz_get_bmatrix2D(B, x, y)
+    L.orz_get_area_weights2D(A, x, y)
+    return B.reshape(4, 2), L.orz_get_area_quad(x, y), L.orz_get_vol_quad(x, y), A
