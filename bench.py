@@ -48,6 +48,25 @@ def cpu_baseline(n, cycles, threads):
                       f"{r['seconds']:.2f} s, oracle/liboracle_sgh_fast.so (gcc -O3, OpenMP)"}
 
 
+def cpu_baseline_rz(n, cycles, threads):
+    """The RZ oracle on the host cores over a bounded sample of the 2D RZ
+    blast."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "threads")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # noqa: E402  (test/baseline infrastructure)
+    from fierro_amd import problems as P
+
+    spec = P.sedov_rz(n, cycles=cycles)
+    pr = O.OracleRZ(spec, fast=True)
+    r = pr.solve()
+    zc = pr.E * r["cycles"] / r["seconds"]
+    return {"value": zc, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "sample": f"sedov_rz {n}^2 ({pr.E} zones), {r['cycles']} RK2 cycles, "
+                      f"{r['seconds']:.2f} s, oracle/liboracle_sgh_fast.so (gcc -O3, OpenMP)"}
+
+
 def measured_traffic(workload, E, N, kernel):
     """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
     WRITE_SIZE (scaled by a plain copy's counters) for this workload, from
@@ -73,7 +92,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-cycles", type=int, default=100)
     ap.add_argument("--workload",
-                    choices=["sedov", "sod-ensemble", "anvil-adjoint", "calibration"],
+                    choices=["sedov", "sedov-rz", "sod-ensemble", "anvil-adjoint",
+                             "calibration"],
                     default="sedov")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--tune", choices=["auto", "default"], default="auto",
@@ -96,6 +116,8 @@ def main():
         return adjoint(args)
     if args.workload == "calibration":
         return calibration(args)
+    if args.workload == "sedov-rz":
+        return sedov_rz(args)
 
     from fierro_amd import bytemodel, problems as P
     from fierro_amd.sgh import SGHSolver
@@ -168,6 +190,67 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_cycles, threads)
+        except Exception as exc:  # the baseline is reported, never the product
+            out["cpu_baseline"] = {"value": None, "error": repr(exc)}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def sedov_rz(args):
+    """2D axisymmetric blast (the reference's SedovRZ input) on n x n quads
+    of [0,1.2]^2 (default 4096^2 = 16.8M zones, the zone count of the 3D
+    256^3 workload), Morton renumbered; the fused RZ cycle loop."""
+    from fierro_amd import bytemodel, problems as P
+    from fierro_amd.sgh import SGHSolver
+
+    n = args.n if args.n != 256 else 4096
+    spec = P.sedov_rz(n, cycles=10 ** 9)
+    t0 = time.perf_counter()
+    g = SGHSolver(spec, renumber=True)
+    log(f"setup rz {n}^2: E={g.E} N={g.N} in {time.perf_counter() - t0:.1f} s")
+    g.run_begin()
+    g.run(args.warmup, use_graph=True)
+    t = time.perf_counter()
+    g.run(args.steps, use_graph=True)
+    graph_s = time.perf_counter() - t
+    g.profile(True)
+    t = time.perf_counter()
+    g.run(args.steps, use_graph=False)
+    wall = time.perf_counter() - t
+    prof = g.profile_read()
+    g.profile(False)
+    st = g.status(0)
+    log(f"timed loop: {args.steps} cycles in {wall:.3f} s (graph loop {graph_s:.3f} s); {st}")
+    E, N = g.E, g.N
+    kernels = {}
+    for name, fn in bytemodel.RZ_KERNELS.items():
+        ms, calls = prof.get(name, (0.0, 0))
+        avg = ms / max(calls, 1)
+        b = fn(E, N)
+        kernels[name] = {"avg_ms": avg, "calls": calls, "alg_bytes": b,
+                         "GBs": b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["calls"])
+    out = {
+        "metric": "FP64 SGH zone-cycles/s (2D RZ Sedov blast)",
+        "value": E * args.steps / wall, "unit": "zone-cycles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (generated RZ Sedov mesh and initial state)",
+        "config": {"workload": f"sedov_rz{n}", "elements": E, "nodes": N, "rk_stages": 2,
+                   "renumber": "morton", "cfl": 0.4, "q2": 1.3333},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kernels[dom]["GBs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"sedov_rz{n}", E, N, dom)},
+        "kernels": kernels,
+        "value_graph": E * args.steps / graph_s,
+    }
+    del g
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        try:
+            out["cpu_baseline"] = cpu_baseline_rz(1024, 100, threads)
         except Exception as exc:  # the baseline is reported, never the product
             out["cpu_baseline"] = {"value": None, "error": repr(exc)}
     else:

@@ -33,6 +33,33 @@ KERNELS = {"div_extremes": div_extremes, "force": force, "vel_bc_pos": vel_bc_po
            "energy_state_div": energy_state_div}
 
 
+# ---- 2D axisymmetric (RZ) path: quads, 4 corners per element, 2 components ----
+def rz_div_extremes(E, N):
+    # corner CSR (16E + 4(N+1)) + div 8E + the two extreme pairs 32N (w)
+    return (16 * E + 4 * (N + 1)) + 8 * E + 32 * N
+
+
+def rz_force(E, N):
+    # conn 16E + x,y,u,v 32N + den,pres,sspd,div 32E + one extreme pair 16N + F 64E (w)
+    return 16 * E + 32 * N + 32 * E + 16 * N + 64 * E
+
+
+def rz_node(E, N):
+    # F 64E + corner CSR + m 8N (r) + m 8N (w) + BC mask 1N + u0,v0,x0,y0 32N
+    # + stage-input radius 8N + u1,v1,x1,y1 32N (w) + radius copy 8N (w) + extensive mass 8N
+    return 64 * E + (16 * E + 4 * (N + 1)) + 16 * N + N + 32 * N + 8 * N + 32 * N + 8 * N + 8 * N
+
+
+def rz_energy_state_div(E, N):
+    # F 64E + conn 16E + u1,v1,u0,v0 32N + radius 8N + x1,y1 16N + M,e0,gamma,csmin 32E
+    # + e1,V,rho,p,c,div 48E (w)
+    return 64 * E + 16 * E + 32 * N + 8 * N + 16 * N + 32 * E + 48 * E
+
+
+RZ_KERNELS = {"rz_div_extremes": rz_div_extremes, "rz_force": rz_force, "rz_node": rz_node,
+              "rz_energy_state_div": rz_energy_state_div}
+
+
 def cycle(E, N):
     """Bytes of one fused RK2 cycle (2 stages; dt and control kernels are O(1)
     per block)."""

@@ -1093,6 +1093,43 @@ void fsgh::issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt
     timed(c, "time_advance", [&] { launch_time_advance(c->d_ctl, c->L.ninst, nw, c->stream); });
 }
 
+// one fused RZ cycle (the 2D branches of K/sgh_solve.cpp:125-634): divergence
+// extremes -> force -> node (velocity, boundary, position, stage-input
+// radius, RZ mass) + on-axis mass -> energy + volume + EOS + next divergence
+void fsgh::issue_cycle_rz(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record)
+{
+    const int nw = 1 - old;
+    const DMesh m = make_mesh(c);
+    DState st0 = make_state(c, old, old);
+    DState st = make_state(c, old, nw);
+    DState sd[2] = {st, st};
+    for (int k = 0; k < 2; ++k) {
+        sd[k].div = c->d_div[k];
+        sd[k].div_out = c->d_div[1 - k];
+    }
+    st0.div = c->d_div[0];
+    timed(c, "cycle_dt", [&] {
+        launch_cycle_dt(c->d_ctl, c->L.ninst, c->d_partial, dt_replay, dt_record, c->stream);
+    });
+    for (int stage = 0; stage < 2; ++stage) {
+        const double alpha = 1.0 / (2.0 - static_cast<double>(stage));   // K/sgh_solve.cpp:353
+        const DState& sf = stage == 0 ? st0 : sd[1];
+        timed(c, "rz_div_extremes", [&] { launch_div_extremes(m, sf, c->d_ctl, c->stream); });
+        timed(c, "rz_force", [&] { launch_rz_force(m, sf, c->d_ctl, 0.0, 0.0, c->stream); });
+        // the stage-input radius: t_n (bin old) on stage 0, the stage-0 result on stage 1
+        const double* rsrc = stage == 0 ? st.y0 : st.y1;
+        timed(c, "rz_node", [&] {
+            launch_rz_node(m, st, c->d_ctl, alpha, rsrc, c->d_rin, c->d_ext, c->d_axis, c->n_axis,
+                           c->stream);
+        });
+        timed(c, "rz_energy_state_div", [&] {
+            launch_rz_energy_state_div(m, sd[stage], c->d_ctl, alpha, c->d_rin,
+                                       stage == 1 ? c->d_partial : nullptr, c->stream);
+        });
+    }
+    timed(c, "time_advance", [&] { launch_time_advance(c->d_ctl, c->L.ninst, nw, c->stream); });
+}
+
 extern "C" {
 
 int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t flags)

@@ -738,33 +738,4 @@ int rz_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill* f
     return rc ? FSGH_E_HIP : FSGH_OK;
 }
 
-// one fused RZ cycle: old = c->cur holds t_n; results go to the other bin
-void issue_cycle_rz(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record)
-{
-    const int nw = 1 - old;
-    const DMesh m = make_mesh(c);
-    DState st0 = make_state(c, old, old);
-    DState st = make_state(c, old, nw);
-    DState sd[2] = {st, st};
-    for (int k = 0; k < 2; ++k) {
-        sd[k].div = c->d_div[k];
-        sd[k].div_out = c->d_div[1 - k];
-    }
-    st0.div = c->d_div[0];
-    launch_cycle_dt(c->d_ctl, c->L.ninst, c->d_partial, dt_replay, dt_record, c->stream);
-    const int2* axis = reinterpret_cast<const int2*>(c->d_axis);
-    for (int stage = 0; stage < 2; ++stage) {
-        const double alpha = 1.0 / (2.0 - static_cast<double>(stage));   // K/sgh_solve.cpp:353
-        const DState& sf = stage == 0 ? st0 : sd[1];
-        launch_div_extremes(m, sf, c->d_ctl, c->stream);
-        launch_rz_force(m, sf, c->d_ctl, 0.0, 0.0, c->stream);
-        // the stage-input radius: t_n (bin old) on stage 0, the stage-0 result on stage 1
-        const double* rsrc = stage == 0 ? st.y0 : st.y1;
-        launch_rz_node(m, st, c->d_ctl, alpha, rsrc, c->d_rin, c->d_ext, axis, c->n_axis, c->stream);
-        launch_rz_energy_state_div(m, sd[stage], c->d_ctl, alpha, c->d_rin,
-                                   stage == 1 ? c->d_partial : nullptr, c->stream);
-    }
-    launch_time_advance(c->d_ctl, c->L.ninst, nw, c->stream);
-}
-
 }  // namespace fsgh
