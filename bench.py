@@ -96,8 +96,9 @@ def main():
                              "calibration"],
                     default="sedov")
     ap.add_argument("--instances", type=int, default=512)
-    ap.add_argument("--tune", choices=["auto", "default"], default="auto",
-                    help="auto: pick the kernels' launch variants with fsgh_autotune first")
+    ap.add_argument("--tune", choices=["table", "auto", "default"], default="table",
+                    help="launch variants of the fused kernels: the measured table "
+                         "(fierro_amd/tuning.py), a fresh fsgh_autotune, or the defaults")
     ap.add_argument("--loop-sample", type=int, default=8,
                     help="time the per-instance loop on every k-th instance")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -262,6 +263,10 @@ def tune(g, args):
     """Autotune the fused-cycle kernels' launch variants (outside the timed
     region; the state is unchanged) and return the choice per kernel."""
     if args.tune == "default":
+        return g.get_tuning()
+    if args.tune == "table":
+        from fierro_amd.tuning import TABLE
+        g.set_tuning(TABLE)
         return g.get_tuning()
     t = time.perf_counter()
     res = g.autotune(reps=5)

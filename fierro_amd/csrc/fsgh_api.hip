@@ -38,6 +38,8 @@ DState fsgh::make_state(fsgh_ctx* c, int b0, int b1)
     s.gamma = c->d_gamma; s.csmin = c->d_csmin;
     s.mat = c->d_mat; s.F = c->d_F;
     s.mats = c->d_mats; s.nmat = c->nmat;
+    s.user_eos = 0;
+    for (const Material& m : c->mats) s.user_eos |= m.eos == FSGH_EOS_USER ? 1 : 0;
     s.ext_lo = c->d_ext_lo; s.ext_hi = c->d_ext_hi;
     return s;
 }

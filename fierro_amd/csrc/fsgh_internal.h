@@ -84,6 +84,7 @@ struct DState {
     double* F;                 // 8E x 3 corner forces
     const Material* mats;
     int32_t nmat;
+    int32_t user_eos;          // some material uses the user EOS (else all are ideal gas)
 };
 
 void set_error(fsgh_ctx* ctx, const std::string& msg);

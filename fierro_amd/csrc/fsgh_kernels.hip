@@ -510,7 +510,8 @@ __global__ __launch_bounds__(BS) void k_vel_bc_pos(DMesh m, DState s, const Inst
 // energy + volume + EOS + next divergence (+ next cycle's dt candidates),
 // element-centric (K/energy_sgh.cpp:16-52, K/geometry.cpp:255-306,
 // K/properties.cpp:30-106, K/momentum.cpp:229-287, K/time_integration.cpp:74-140)
-template <int BS, int MINW>
+// USER = false: every material is ideal gas, the EOS selector folds away
+template <int BS, int MINW, bool USER>
 __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s,
                                                                const InstCtl* ctl, double alpha,
                                                                double* dt_partial)
@@ -533,7 +534,7 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
         s.sie1[e] = sie;
         gather3(s.x1, s.y1, s.z1, nd, x, y, z);
         double vol, den, p, cs, div;
-        const int32_t kind = s.mats[s.nmat > 1 ? s.mat[e] : 0].eos;
+        const int32_t kind = USER ? s.mats[s.nmat > 1 ? s.mat[e] : 0].eos : kEosIdealGas;
         derived_state(x, y, z, u1, v1, w1, emass, sie, s.gamma[e], s.csmin[e], kind, vol, den, p, cs,
                       div);
         s.vol[e] = vol;
@@ -785,8 +786,12 @@ void launch_energy_state_div(const DMesh& m, const DState& s, const InstCtl* ctl
     if (!m.n_eblk) return;
 #define FSGH_VH(B, W)                                                                       \
     if (m.bs == B && m.minw == W) {                                                        \
-        hipLaunchKernelGGL((k_energy_state_div<B, W>), dim3(m.n_eblk), dim3(B), 0, st, m, s, \
-                           ctl, alpha, dt_partial);                                        \
+        if (s.user_eos)                                                                    \
+            hipLaunchKernelGGL((k_energy_state_div<B, W, true>), dim3(m.n_eblk), dim3(B), 0, \
+                               st, m, s, ctl, alpha, dt_partial);                          \
+        else                                                                               \
+            hipLaunchKernelGGL((k_energy_state_div<B, W, false>), dim3(m.n_eblk), dim3(B), 0, \
+                               st, m, s, ctl, alpha, dt_partial);                          \
         return;                                                                            \
     }
     FSGH_HEAVY_VARIANTS(FSGH_VH)

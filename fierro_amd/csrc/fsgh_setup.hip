@@ -97,6 +97,22 @@ __global__ void k_morton(const int32_t* conn, const double* xyz, const int64_t* 
 }
 
 // rank scatter: map[sorted[k]] = k
+// instance of every id in `ids` (off: ninst+1 ascending offsets)
+__global__ void k_inst_of(const int32_t* ids, const int64_t* off, int ninst, uint32_t* out,
+                          int64_t n)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+    if (k >= n) return;
+    const int64_t id = ids[k];
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= id) lo = mid;
+        else hi = mid - 1;
+    }
+    out[k] = static_cast<uint32_t>(lo);
+}
+
 __global__ void k_rank(const int32_t* sorted, int32_t* map, int64_t n)
 {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
@@ -348,6 +364,51 @@ __global__ void k_fill_node(const fsgh_fill* fills, const int32_t* cin_start, co
 
 // Morton renumbering (or identity), internal connectivity and corner CSR on
 // the device; fills c->d_conn, d_cin_start, d_cin and the host maps.
+// Stable sort of (key, id) pairs inside the instance ranges `off` (the ids
+// of range i are off[i]..off[i+1]-1 and arrive in ascending order): one
+// global stable radix sort by key, then, for several instances, a stable
+// sort by instance.  Same result as a segmented sort, but every pass is a
+// device-wide sort (a segmented sort gives a large segment to one block).
+template <typename K>
+hipError_t sort_in_instances(K* key, K* key2, int32_t* id, int32_t* id2, int64_t n,
+                             const int64_t* off_dev, int ninst, int end_bit, hipStream_t st)
+{
+    size_t tmp_bytes = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, key2, id, id2,
+                                             static_cast<size_t>(n), 0, end_bit, st);
+    if (e != hipSuccess) return e;
+    {
+        DBuf<char> tmp;
+        if ((e = tmp.alloc(tmp_bytes)) != hipSuccess) return e;
+        e = rocprim::radix_sort_pairs(tmp.p, tmp_bytes, key, key2, id, id2,
+                                      static_cast<size_t>(n), 0, end_bit, st);
+        if (e != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    }
+    if (ninst == 1) return hipSuccess;
+    DBuf<uint32_t> ik, ik2;
+    if ((e = ik.alloc(static_cast<size_t>(n))) != hipSuccess) return e;
+    if ((e = ik2.alloc(static_cast<size_t>(n))) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_inst_of, dim3(grid(static_cast<size_t>(n))), dim3(kB), 0, st, id2, off_dev,
+                       ninst, ik.p, n);
+    int bits = 1;
+    while ((1 << bits) < ninst) ++bits;
+    // the result goes back to id2: sort (ik, id2) -> (ik2, id)
+    tmp_bytes = 0;
+    e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, ik.p, ik2.p, id2, id, static_cast<size_t>(n),
+                                  0, bits, st);
+    if (e != hipSuccess) return e;
+    DBuf<char> tmp;
+    if ((e = tmp.alloc(tmp_bytes)) != hipSuccess) return e;
+    e = rocprim::radix_sort_pairs(tmp.p, tmp_bytes, ik.p, ik2.p, id2, id, static_cast<size_t>(n), 0,
+                                  bits, st);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(id2, id, static_cast<size_t>(n) * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                       st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
+
 int gpu_build_maps(fsgh_ctx* c)
 {
     const int64_t N = c->N, E = c->E, C8 = 8 * E;
@@ -405,18 +466,7 @@ int gpu_build_maps(fsgh_ctx* c)
         SETUP_HIP(id2.alloc(static_cast<size_t>(E)));
         hipLaunchKernelGGL(k_morton, dim3(grid(static_cast<size_t>(E))), dim3(kB), 0, st, conn.p,
                            xyz.p, eoff.p, ninst, dbox.p, code.p, id.p, E);
-        size_t tmp_bytes = 0;
-        SETUP_HIP(rocprim::segmented_radix_sort_pairs(nullptr, tmp_bytes, code.p, code2.p, id.p,
-                                                      id2.p, static_cast<unsigned>(E), ninst,
-                                                      eoff.p, eoff.p + 1, 0, 63, st));
-        {
-            DBuf<char> tmp;
-            SETUP_HIP(tmp.alloc(tmp_bytes));
-            SETUP_HIP(rocprim::segmented_radix_sort_pairs(tmp.p, tmp_bytes, code.p, code2.p, id.p,
-                                                          id2.p, static_cast<unsigned>(E), ninst,
-                                                          eoff.p, eoff.p + 1, 0, 63, st));
-            SETUP_HIP(hipStreamSynchronize(st));
-        }
+        SETUP_HIP(sort_in_instances(code.p, code2.p, id.p, id2.p, E, eoff.p, ninst, 63, st));
         hipLaunchKernelGGL(k_rank, dim3(grid(static_cast<size_t>(E))), dim3(kB), 0, st, id2.p, ne.p,
                            E);
         // nodes: first touch in the new element order, within each instance
@@ -431,18 +481,7 @@ int gpu_build_maps(fsgh_ctx* c)
         hipLaunchKernelGGL(k_iota, dim3(grid(static_cast<size_t>(N))), dim3(kB), 0, st, nid.p, N);
         hipLaunchKernelGGL(k_first_touch, dim3(grid(static_cast<size_t>(E))), dim3(kB), 0, st, conn.p,
                            ne.p, ft.p, E);
-        tmp_bytes = 0;
-        SETUP_HIP(rocprim::segmented_radix_sort_pairs(nullptr, tmp_bytes, ft.p, ft2.p, nid.p,
-                                                      nid2.p, static_cast<unsigned>(N), ninst,
-                                                      noff.p, noff.p + 1, 0, 64, st));
-        {
-            DBuf<char> tmp;
-            SETUP_HIP(tmp.alloc(tmp_bytes));
-            SETUP_HIP(rocprim::segmented_radix_sort_pairs(tmp.p, tmp_bytes, ft.p, ft2.p, nid.p,
-                                                          nid2.p, static_cast<unsigned>(N), ninst,
-                                                          noff.p, noff.p + 1, 0, 64, st));
-            SETUP_HIP(hipStreamSynchronize(st));
-        }
+        SETUP_HIP(sort_in_instances(ft.p, ft2.p, nid.p, nid2.p, N, noff.p, ninst, 64, st));
         hipLaunchKernelGGL(k_rank, dim3(grid(static_cast<size_t>(N))), dim3(kB), 0, st, nid2.p, nn.p,
                            N);
         SETUP_HIP(hipStreamSynchronize(st));   // before the sort buffers go out of scope

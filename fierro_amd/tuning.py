@@ -1,0 +1,20 @@
+"""Launch configurations of the fused 3D cycle kernels found by
+``fsgh_autotune`` on an MI355X (Sedov blast 256^3, Morton order): per
+kernel the workgroup size, the waves-per-SIMD floor and whether work
+ranges are dealt per XCD.  ``SGHSolver.set_tuning(TABLE)`` applies them
+without re-timing; ``SGHSolver.autotune()`` measures again on the caller's
+own mesh.  Every configuration gives bit-identical results.
+
+Measured (mean per launch, default = 256 threads, no floor, XCD dealing):
+div_extremes 0.326 ms with 512 threads in launch order (default 0.338 ms);
+vel_bc_pos 1.482 ms in launch order (default 1.522 ms); force and
+energy_state_div fastest at the default (the 3-wave variants spill and run
+1.7x slower).
+"""
+
+TABLE = {
+    "div_extremes": {"block": 512, "min_waves": 1, "xcd_order": 0},
+    "force": {"block": 256, "min_waves": 1, "xcd_order": 1},
+    "vel_bc_pos": {"block": 256, "min_waves": 1, "xcd_order": 0},
+    "energy_state_div": {"block": 256, "min_waves": 1, "xcd_order": 1},
+}
