@@ -19,10 +19,11 @@ ERRORS = {-1: "FSGH_E_INVALID", -2: "FSGH_E_HIP", -3: "FSGH_E_NOMEM", -4: "FSGH_
 # enum fsgh_field
 NODE_COORDS, NODE_VEL, NODE_MASS = 0, 1, 2
 ELEM_DEN, ELEM_PRES, ELEM_SSPD, ELEM_SIE, ELEM_VOL, ELEM_DIV, ELEM_MASS = 3, 4, 5, 6, 7, 8, 9
-ELEM_MAT_ID, ELEM_GAMMA, ELEM_CSMIN, CORNER_FORCE = 10, 11, 12, 13
+ELEM_MAT_ID, ELEM_GAMMA, ELEM_CSMIN, CORNER_FORCE, ELEM_STRESS = 10, 11, 12, 13, 14
 
 EOS_IDEAL_GAS, EOS_USER = 0, 1
-STRENGTH_NONE = 0
+STRENGTH_NONE, STRENGTH_HYPO, STRENGTH_HYPER = 0, 1, 2
+STRENGTH_PARAMS = 8
 
 # every symbol include/fsgh.h declares
 EXPORTS = [
@@ -37,6 +38,7 @@ EXPORTS = [
     "fsgh_forward", "fsgh_dt_history", "fsgh_backward", "fsgh_tangent",
     "fsgh_autotune", "fsgh_set_tuning", "fsgh_get_tuning",
     "fsgh_rz_extensive_mass", "fsgh_rz_update_node_mass",
+    "fsgh_set_strength_params", "fsgh_get_velgrad",
 ]
 
 KERNEL_DIV_EXTREMES, KERNEL_FORCE, KERNEL_VEL_BC_POS, KERNEL_ENERGY_STATE_DIV = 0, 1, 2, 3
@@ -141,6 +143,8 @@ def load() -> C.CDLL:
         "fsgh_set_tuning": [vp, C.POINTER(Tuning), i32],
         "fsgh_get_tuning": [vp, C.POINTER(Tuning), i32],
         "fsgh_rz_extensive_mass": [vp],
+        "fsgh_set_strength_params": [vp, i32, i32, vp],
+        "fsgh_get_velgrad": [vp, vp],
         "fsgh_rz_update_node_mass": [vp, dbl],
     }
     for name, args in sig.items():

@@ -1483,8 +1483,8 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
                  int64_t checkpoint_every, const double* dt_replay)
 {
     if (!c) return FSGH_E_INVALID;
-    if (c->dims == 2) {
-        set_error(c, "the adjoint covers the 3D cycle only");
+    if (c->dims == 2 || c->d_stress[0]) {
+        set_error(c, "the adjoint covers the 3D strength-free cycle only");
         return FSGH_E_UNSUPPORTED;
     }
     if (c->host_only) {

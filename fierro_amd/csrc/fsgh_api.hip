@@ -40,6 +40,10 @@ DState fsgh::make_state(fsgh_ctx* c, int b0, int b1)
     s.mats = c->d_mats; s.nmat = c->nmat;
     s.user_eos = 0;
     for (const Material& m : c->mats) s.user_eos |= m.eos == FSGH_EOS_USER ? 1 : 0;
+    s.sig0 = c->d_stress[b0];
+    s.sig1 = c->d_stress[b1];
+    s.sig1w = c->d_stress[b1];
+    s.sparams = c->d_sparams;
     s.ext_lo = c->d_ext_lo; s.ext_hi = c->d_ext_hi;
     return s;
 }
@@ -196,7 +200,8 @@ int normalize_bins(fsgh_ctx* c)
             launch_swap_bins_range(s, static_cast<int32_t>(c->L.node_off[i]),
                                    static_cast<int32_t>(c->L.node_off[i + 1]),
                                    static_cast<int32_t>(c->L.elem_off[i]),
-                                   static_cast<int32_t>(c->L.elem_off[i + 1]), c->stream);
+                                   static_cast<int32_t>(c->L.elem_off[i + 1]),
+                                   static_cast<int32_t>(c->E), c->stream);
             h[i].final_bin = c->cur;
             changed = true;
         }
@@ -404,6 +409,8 @@ int fsgh_destroy(fsgh_ctx* c)
         if (p) hipFree(p);
     for (auto& kv : c->tables)
         if (kv.second.d) hipFree(kv.second.d);
+    for (double* p : {c->d_stress[0], c->d_stress[1], c->d_sparams})
+        if (p) hipFree(p);
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 6; ++k)
             if (c->d_node[b][k]) hipFree(c->d_node[b][k]);
@@ -466,6 +473,32 @@ int fsgh_set_materials(fsgh_ctx* c, int32_t nmat, const fsgh_material* mats)
     return FSGH_OK;
 }
 
+// the stress field (both bins, zero as K/setup.cpp:214-220 sets it) and the
+// strength parameter table, on first need
+static int ensure_stress(fsgh_ctx* c)
+{
+    if (c->d_stress[0]) return FSGH_OK;
+    const size_t n = 9 * static_cast<size_t>(c->E);
+    for (int b = 0; b < 2; ++b) {
+        if (hipMalloc(reinterpret_cast<void**>(&c->d_stress[b]), n * sizeof(double)) != hipSuccess) {
+            set_error(c, "hipMalloc: stress field");
+            return FSGH_E_NOMEM;
+        }
+        HIPCHK(c, hipMemsetAsync(c->d_stress[b], 0, n * sizeof(double), c->stream));
+    }
+    const size_t np = static_cast<size_t>(kStrengthParams) * std::max<int32_t>(c->nmat, 1);
+    if (hipMalloc(reinterpret_cast<void**>(&c->d_sparams), np * sizeof(double)) != hipSuccess) {
+        set_error(c, "hipMalloc: strength parameters");
+        return FSGH_E_NOMEM;
+    }
+    HIPCHK(c, hipMemsetAsync(c->d_sparams, 0, np * sizeof(double), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    // the stress kernels run on the default element table: a running loop's
+    // dt partial minima follow it from now on
+    return c->run_active ? resync_dt_ranges(c) : FSGH_OK;
+}
+
 int fsgh_set_material_models(fsgh_ctx* c, int32_t mat, int32_t eos_kind, int32_t strength)
 {
     CHECK_CTX(c);
@@ -475,14 +508,41 @@ int fsgh_set_material_models(fsgh_ctx* c, int32_t mat, int32_t eos_kind, int32_t
         return FSGH_E_STATE;
     }
     if (mat < 0 || mat >= c->nmat) return FSGH_E_INVALID;
-    if (strength != FSGH_STRENGTH_NONE) {
-        set_error(c, "strength models (hypo/hyper-elastic) are not in this build");
+    if (strength != FSGH_STRENGTH_NONE && strength != FSGH_STRENGTH_HYPO &&
+        strength != FSGH_STRENGTH_HYPER)
+        return FSGH_E_INVALID;
+    if (c->dims == 2 && strength == FSGH_STRENGTH_HYPER) {
+        // the reference's 2D update_state evaluates the 3D B matrix on a quad
+        set_error(c, "hyper-elastic strength in 2D RZ is not supported");
         return FSGH_E_UNSUPPORTED;
     }
     if (eos_kind != FSGH_EOS_IDEAL_GAS && eos_kind != FSGH_EOS_USER) return FSGH_E_INVALID;
+    if (strength != FSGH_STRENGTH_NONE)
+        if (int rc = ensure_stress(c)) return rc;
     c->mats[static_cast<size_t>(mat)].eos = eos_kind;
+    c->mats[static_cast<size_t>(mat)].strength = strength;
+    drop_graphs(c);
     HIPCHK(c, hipMemcpyAsync(c->d_mats + mat, &c->mats[static_cast<size_t>(mat)], sizeof(Material),
                              hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FSGH_OK;
+}
+
+int fsgh_set_strength_params(fsgh_ctx* c, int32_t mat, int32_t n, const double* params)
+{
+    CHECK_CTX(c);
+    if (c->host_only) return FSGH_E_STATE;
+    if (c->nmat <= 0) {
+        set_error(c, "materials not set (fsgh_set_materials)");
+        return FSGH_E_STATE;
+    }
+    if (mat < 0 || mat >= c->nmat || n < 0 || n > kStrengthParams || (n > 0 && !params))
+        return FSGH_E_INVALID;
+    if (int rc = ensure_stress(c)) return rc;
+    double p[kStrengthParams] = {};
+    for (int32_t i = 0; i < n; ++i) p[i] = params[i];
+    HIPCHK(c, hipMemcpyAsync(c->d_sparams + static_cast<size_t>(kStrengthParams) * mat, p,
+                             sizeof(p), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FSGH_OK;
 }
@@ -770,6 +830,37 @@ static int field_io(fsgh_ctx* c, int32_t field, void* host, size_t bytes, bool u
         if (bytes != E * 8) return FSGH_E_INVALID;
         return up ? put_elems(c, c->d_mat, static_cast<int64_t*>(host), 1, 0)
                   : get_elems(c, c->d_mat, static_cast<int64_t*>(host), 1, 0);
+    case FSGH_ELEM_STRESS: {
+        // [2 bins][E][3][3] at the ABI, [bin][9][E] inside
+        if (bytes != 2 * E * 9 * 8) return FSGH_E_INVALID;
+        if (up) {
+            if ((rc = ensure_stress(c))) return rc;
+        } else if (!c->d_stress[0]) {
+            std::memset(host, 0, bytes);
+            return FSGH_OK;
+        }
+        std::vector<double> t(9 * E);
+        for (int b = 0; b < 2; ++b) {
+            double* h = static_cast<double*>(host) + b * 9 * E;
+            if (up) {
+                for (size_t e = 0; e < E; ++e)
+                    for (int k = 0; k < 9; ++k)
+                        t[k * E + static_cast<size_t>(c->ne_of_oe[e])] = h[9 * e + k];
+                HIPCHK(c, hipMemcpyAsync(c->d_stress[phys[b]], t.data(), t.size() * 8,
+                                         hipMemcpyHostToDevice, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+            } else {
+                HIPCHK(c, hipMemcpyAsync(t.data(), c->d_stress[phys[b]], t.size() * 8,
+                                         hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                for (size_t e = 0; e < E; ++e)
+                    for (int k = 0; k < 9; ++k)
+                        h[9 * e + k] = t[k * E + static_cast<size_t>(c->ne_of_oe[e])];
+            }
+        }
+        if (up) drop_graphs(c);
+        return FSGH_OK;
+    }
     case FSGH_CORNER_FORCE: {
         // per element: npe corners x num_dims components, contiguous
         const size_t K = static_cast<size_t>(c->npe * c->dims);
@@ -885,15 +976,14 @@ int fsgh_get_divergence(fsgh_ctx* c)
 
 int fsgh_get_force_sgh(fsgh_ctx* c, double fuzz, double small, double dt, double rk_alpha)
 {
-    (void)dt;
-    (void)rk_alpha;
     OP_PROLOGUE(c);
     if (int rc = ready(c)) return rc;
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
     timed(c, "div_extremes", [&] { launch_div_extremes(m, s, nullptr, c->stream); });
     timed(c, "force", [&] {
-        if (c->dims == 2) launch_rz_force(m, s, nullptr, fuzz, small, c->stream);   // K/force_sgh.cpp:407-808
+        if (c->dims == 2) launch_rz_force(m, s, nullptr, fuzz, small, dt, rk_alpha, c->stream);   // K/force_sgh.cpp:407-808
+        else if (s.sig1) launch_force_stress(m, s, nullptr, fuzz, small, dt, rk_alpha, c->stream);
         else launch_force(m, s, nullptr, fuzz, small, c->stream);
     });
     OP_EPILOGUE(c);
@@ -955,12 +1045,35 @@ int fsgh_get_vol(fsgh_ctx* c)
 
 int fsgh_update_state(fsgh_ctx* c, double dt, double rk_alpha)
 {
-    (void)dt;
-    (void)rk_alpha;
     OP_PROLOGUE(c);
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
-    timed(c, "update_state", [&] { launch_update_state(m, s, c->stream); });
+    timed(c, "update_state", [&] {
+        if (s.sig1) launch_update_state_stress(m, s, dt, rk_alpha, c->stream);
+        else launch_update_state(m, s, c->stream);
+    });
+    OP_EPILOGUE(c);
+}
+
+int fsgh_get_velgrad(fsgh_ctx* c, double* out)
+{
+    OP_PROLOGUE(c);
+    if (!out) return FSGH_E_INVALID;
+    DMesh m = make_mesh(c);
+    DState s = ref_state(c);
+    const size_t E = static_cast<size_t>(c->E);
+    double* d = nullptr;
+    HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&d), 9 * E * sizeof(double)));
+    if (c->dims == 2) launch_rz_velgrad(m, s, d, c->stream);
+    else launch_velgrad(m, s, d, c->stream);
+    std::vector<double> t(9 * E);
+    const hipError_t e1 = hipMemcpyAsync(t.data(), d, t.size() * 8, hipMemcpyDeviceToHost, c->stream);
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    HIPCHK(c, e1);
+    HIPCHK(c, e2);
+    for (size_t e = 0; e < E; ++e)
+        std::memcpy(out + 9 * e, &t[9 * static_cast<size_t>(c->ne_of_oe[e])], 9 * 8);
     OP_EPILOGUE(c);
 }
 
@@ -1052,8 +1165,7 @@ int fsgh_run_begin(fsgh_ctx* c, const fsgh_run_params* p)
     } else {
         timed(c, "get_divergence", [&] { launch_get_divergence(m, s, c->stream); });
         timed(c, "dt_partial", [&] {
-            launch_dt_partial(tuned_mesh(c, kTunedEnergy), s, c->d_ctl, c->d_partial, 0, 0, 0, 0,
-                              1, c->stream);
+            launch_dt_partial(dt_mesh(c), s, c->d_ctl, c->d_partial, 0, 0, 0, 0, 1, c->stream);
         });
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1077,6 +1189,11 @@ void fsgh::issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt
         sd[k].div_out = c->d_div[1 - k];
     }
     st0.div = c->d_div[0];
+    // with a stress field the first stage reads the t_n stress and its force
+    // kernel writes the stage's stress into the new bin
+    const bool stress = c->d_stress[0] != nullptr;
+    st0.sig1w = c->d_stress[nw];
+    const DMesh m0 = make_mesh(c);
     timed(c, "cycle_dt", [&] {
         launch_cycle_dt(c->d_ctl, c->L.ninst, c->d_partial, dt_replay, dt_record, c->stream);
     });
@@ -1084,12 +1201,16 @@ void fsgh::issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt
         const double alpha = 1.0 / (2.0 - static_cast<double>(stage));   // K/sgh_solve.cpp:353
         const InstCtl* ctl = c->d_ctl;
         const DState& sf = stage == 0 ? st0 : sd[1];
+        double* partial = stage == 1 ? c->d_partial : nullptr;
         timed(c, "div_extremes", [&] { launch_div_extremes(mx, sf, ctl, c->stream); });
-        timed(c, "force", [&] { launch_force(mf, sf, ctl, 0.0, 0.0, c->stream); });
+        timed(c, "force", [&] {
+            if (stress) launch_force_stress(m0, sf, ctl, 0.0, 0.0, 0.0, alpha, c->stream);
+            else launch_force(mf, sf, ctl, 0.0, 0.0, c->stream);
+        });
         timed(c, "vel_bc_pos", [&] { launch_vel_bc_pos(mv, st, ctl, alpha, c->stream); });
         timed(c, "energy_state_div", [&] {
-            launch_energy_state_div(me, sd[stage], ctl, alpha, stage == 1 ? c->d_partial : nullptr,
-                                    c->stream);
+            if (stress) launch_energy_state_div_stress(m0, sd[stage], ctl, alpha, partial, c->stream);
+            else launch_energy_state_div(me, sd[stage], ctl, alpha, partial, c->stream);
         });
     }
     timed(c, "time_advance", [&] { launch_time_advance(c->d_ctl, c->L.ninst, nw, c->stream); });
@@ -1110,6 +1231,7 @@ void fsgh::issue_cycle_rz(fsgh_ctx* c, int old, const double* dt_replay, double*
         sd[k].div_out = c->d_div[1 - k];
     }
     st0.div = c->d_div[0];
+    st0.sig1w = c->d_stress[nw];
     timed(c, "cycle_dt", [&] {
         launch_cycle_dt(c->d_ctl, c->L.ninst, c->d_partial, dt_replay, dt_record, c->stream);
     });
@@ -1117,7 +1239,7 @@ void fsgh::issue_cycle_rz(fsgh_ctx* c, int old, const double* dt_replay, double*
         const double alpha = 1.0 / (2.0 - static_cast<double>(stage));   // K/sgh_solve.cpp:353
         const DState& sf = stage == 0 ? st0 : sd[1];
         timed(c, "rz_div_extremes", [&] { launch_div_extremes(m, sf, c->d_ctl, c->stream); });
-        timed(c, "rz_force", [&] { launch_rz_force(m, sf, c->d_ctl, 0.0, 0.0, c->stream); });
+        timed(c, "rz_force", [&] { launch_rz_force(m, sf, c->d_ctl, 0.0, 0.0, 0.0, alpha, c->stream); });
         // the stage-input radius: t_n (bin old) on stage 0, the stage-0 result on stage 1
         const double* rsrc = stage == 0 ? st.y0 : st.y1;
         timed(c, "rz_node", [&] {

@@ -45,6 +45,11 @@ struct fsgh_ctx {
     int32_t *d_on_of_nn = nullptr, *d_oe_of_ne = nullptr;   // internal -> caller ids
     fsgh::Material* d_mats = nullptr;
     std::vector<fsgh::Material> mats;   // host copy of the material table
+    // Cauchy stress per physical bin ([9][E] each), allocated when a material
+    // gets a strength model or a stress field is uploaded; strength
+    // parameters per material
+    double* d_stress[2] = {nullptr, nullptr};
+    double* d_sparams = nullptr;
     int32_t nmat = 0;
     fsgh::InstCtl* d_ctl = nullptr;
     double* d_partial = nullptr;
@@ -102,6 +107,11 @@ DMesh tuned_mesh(fsgh_ctx* c, int kernel);
 // per-instance logical block ranges of the element table that the fused
 // energy kernel (and the dt partial minima) use
 const std::vector<int32_t>& dt_block_begin(fsgh_ctx* c);
+// the mesh view whose element table the dt partial minima follow
+DMesh dt_mesh(fsgh_ctx* c);
+// point a running loop's dt block ranges at dt_block_begin and recompute the
+// partial minima from the current state
+int resync_dt_ranges(fsgh_ctx* c);
 // build (once) the work table of a configuration; FSGH_OK or an error code
 int ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd);
 void issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record);

@@ -44,8 +44,9 @@ struct InstCtl {
 struct Material {
     double q1, q1ex, q2, q2ex;
     int32_t eos;               // FSGH_EOS_IDEAL_GAS or FSGH_EOS_USER
-    int32_t pad_;
+    int32_t strength;          // FSGH_STRENGTH_NONE / _HYPO / _HYPER
 };
+constexpr int kStrengthParams = FSGH_STRENGTH_PARAMS;   // parameters per material for the user strength model
 
 // Device view of the mesh (new numbering, int32 indices).
 struct DMesh {
@@ -85,6 +86,12 @@ struct DState {
     const Material* mats;
     int32_t nmat;
     int32_t user_eos;          // some material uses the user EOS (else all are ideal gas)
+    // Cauchy stress, SoA [9][E] row-major components, or all null (zero
+    // stress): sig0 = t_n, sig1 = the current stage's input, sig1w = where the
+    // force kernel writes the stress after the hypo-elastic model (sig1 except
+    // on the first stage of a fused cycle, whose input is t_n)
+    double *sig0, *sig1, *sig1w;
+    const double* sparams;     // kStrengthParams per material
 };
 
 void set_error(fsgh_ctx* ctx, const std::string& msg);
@@ -99,6 +106,17 @@ void launch_get_divergence(const DMesh&, const DState&, hipStream_t);
 bool tuned_variant_ok(int kernel, int32_t bs, int32_t minw);
 void launch_force(const DMesh&, const DState&, const InstCtl*, double fuzz, double small,
                   hipStream_t);
+// with a stress field (hypo-elastic hook after the forces); dt / alpha are
+// taken from ctl when it is given
+void launch_force_stress(const DMesh&, const DState&, const InstCtl*, double fuzz, double small,
+                         double dt, double alpha, hipStream_t);
+void launch_energy_state_div_stress(const DMesh&, const DState&, const InstCtl*, double alpha,
+                                    double* dt_partial, hipStream_t);
+// velocity gradient of the current state, E x 9 (row-major)
+void launch_velgrad(const DMesh&, const DState&, double* out, hipStream_t);
+// update_state with stress (hyper-elastic hook before the EOS)
+void launch_update_state_stress(const DMesh&, const DState&, double dt, double alpha,
+                                hipStream_t);
 void launch_div_extremes(const DMesh&, const DState&, const InstCtl*, hipStream_t);
 void launch_update_velocity(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
 void launch_boundary_velocity(const int32_t* nodes, const uint8_t* masks, int32_t count,
@@ -127,8 +145,11 @@ void launch_energy_sums(const DMesh&, const DState&, const int32_t* eoff, const 
 // ---- 2D axisymmetric (RZ) path (fsgh_rz.hip) ----
 void launch_rz_get_vol(const DMesh&, const DState&, hipStream_t);
 void launch_rz_get_divergence(const DMesh&, const DState&, hipStream_t);
+// with a stress field in s: tau = stress - p I, hoop terms of the stress
+// and the hypo-elastic hook; dt / alpha from ctl when given
 void launch_rz_force(const DMesh&, const DState&, const InstCtl*, double fuzz, double small,
-                     hipStream_t);
+                     double dt, double alpha, hipStream_t);
+void launch_rz_velgrad(const DMesh&, const DState&, double* out, hipStream_t);
 void launch_rz_update_velocity(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
 void launch_rz_update_energy(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
 void launch_rz_dt_partial(const DMesh&, const DState&, const InstCtl*, double* partial,
@@ -142,6 +163,6 @@ void launch_rz_node(const DMesh&, const DState&, const InstCtl*, double alpha, c
 void launch_rz_energy_state_div(const DMesh&, const DState&, const InstCtl*, double alpha,
                                 const double* rin, double* dt_partial, hipStream_t);
 void launch_swap_bins_range(const DState&, int32_t n0, int32_t n1, int32_t e0, int32_t e1,
-                            hipStream_t);
+                            int32_t Etot, hipStream_t);
 void launch_check(const DMesh&, const DState&, int32_t* flag, hipStream_t);
 }  // namespace fsgh

@@ -386,6 +386,12 @@ __global__ __launch_bounds__(kBlock) void k_rk_init_elems(DMesh m, DState s)
     const int e = bi.start + static_cast<int>(threadIdx.x);
     if (e >= bi.end) return;
     s.sie0[e] = s.sie1[e];
+    if (s.sig0 != nullptr) {
+        // stress is always 3 x 3 (K/time_integration.cpp:19-24)
+        const size_t E = static_cast<size_t>(m.E);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s.sig0[k * E + e] = s.sig1[k * E + e];
+    }
 }
 
 // per-element CFL dt, block minimum into partial[logical block]
@@ -551,6 +557,181 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
     }
 }
 
+// --------------------------------------------------------------------------
+// kernels with a stress field (strength-model materials)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void load_stress(const double* sig, size_t E, int e, double st[9])
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) st[k] = sig[static_cast<size_t>(k) * E + e];
+}
+__device__ __forceinline__ void store_stress(double* sig, size_t E, int e, const double st[9])
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sig[static_cast<size_t>(k) * E + e] = st[k];
+}
+
+// get_force_sgh with tau = stress - p I and the hypo-elastic hook after the
+// forces (K/force_sgh.cpp:84-99, 127-137, 338-391)
+__global__ __launch_bounds__(kBlock) void k_force_stress(DMesh m, DState s, const InstCtl* ctl,
+                                                         double fuzz, double small, double dt,
+                                                         double alpha)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    if (ctl != nullptr) {
+        if (ctl[bi.inst].stopped) return;
+        fuzz = ctl[bi.inst].fuzz;
+        small = ctl[bi.inst].small;
+        dt = ctl[bi.inst].dt;
+    }
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    const size_t E = static_cast<size_t>(m.E);
+    int32_t nd[8];
+    load_conn(m.conn, e, nd);
+    double x[8], y[8], z[8], u[8], v[8], w[8];
+    gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+    gather3(s.u1, s.v1, s.w1, nd, u, v, w);
+    const double vol = s.vol[e];
+    const double div = s.div[e];
+    const double den = s.den[e];
+    const double sspd = s.sspd[e];
+    const double pres = s.pres[e];
+    const int32_t mat = s.nmat > 1 ? s.mat[e] : 0;
+    const Material mt = s.mats[mat];
+    const bool lo = div + small >= 0.0;
+    const double2* __restrict__ xs = lo ? s.ext_lo : s.ext_hi;
+    double ext = lo ? HUGE_VAL : -HUGE_VAL;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const double2 t = xs[nd[a]];
+        const double other = (t.x == div) ? t.y : t.x;
+        ext = lo ? fmin(ext, other) : fmax(ext, other);
+    }
+    double st[9], B[24], f[24];
+    load_stress(s.sig1, E, e, st);
+    force_elem_stressed(x, y, z, u, v, w, den, pres, sspd, vol, div, ext, mt.q1, mt.q1ex, mt.q2,
+                        mt.q2ex, fuzz, small, st, B, f);
+    double2* out = reinterpret_cast<double2*>(s.F + 24 * static_cast<size_t>(e));
+#pragma unroll
+    for (int i = 0; i < 12; ++i) out[i] = make_double2(f[2 * i], f[2 * i + 1]);
+    if (mt.strength == kStrengthHypo) {
+        double L[9];
+        velgrad(B, u, v, w, vol, L);
+        double p = pres, c = sspd;
+        user_strength(mat, s.sparams + kStrengthParams * mat, den, s.sie0[e], L, vol, dt, alpha,
+                      st, p, c);
+        s.pres[e] = p;
+        s.sspd[e] = c;
+    }
+    store_stress(s.sig1w, E, e, st);
+}
+
+// the fused energy/state kernel with a stress field: the hyper-elastic hook
+// before the EOS (K/properties.cpp:39-106), the user EOS seeing the stress
+__global__ __launch_bounds__(kBlock) void k_energy_state_div_stress(DMesh m, DState s,
+                                                                    const InstCtl* ctl,
+                                                                    double alpha,
+                                                                    double* dt_partial)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const InstCtl& c = ctl[bi.inst];
+    if (c.stopped) return;
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    double dtl = DBL_MAX;
+    if (e < bi.end) {
+        const size_t E = static_cast<size_t>(m.E);
+        int32_t nd[8];
+        load_conn(m.conn, e, nd);
+        double f[24], x[8], y[8], z[8], u1[8], v1[8], w1[8], u0[8], v0[8], w0[8];
+        load_forces(s, e, f);
+        gather3(s.u1, s.v1, s.w1, nd, u1, v1, w1);
+        gather3(s.u0, s.v0, s.w0, nd, u0, v0, w0);
+        const double emass = s.emass[e];
+        const double sie =
+            energy_update(f, u1, v1, w1, u0, v0, w0, s.sie0[e], emass, alpha, c.dt);
+        s.sie1[e] = sie;
+        gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+        double B[24];
+        const double vol = hex_bmatrix_volume(x, y, z, B);
+        const double den = emass / vol;
+        const double div = divergence(B, u1, v1, w1, vol);
+        const int32_t mat = s.nmat > 1 ? s.mat[e] : 0;
+        const Material mt = s.mats[mat];
+        double st[9];
+        load_stress(s.sig1, E, e, st);
+        double p = s.pres[e], cs = s.sspd[e];
+        if (mt.strength == kStrengthHyper) {
+            double L[9];
+            velgrad(B, u1, v1, w1, vol, L);
+            user_strength(mat, s.sparams + kStrengthParams * mat, den, s.sie0[e], L, vol, c.dt,
+                          alpha, st, p, cs);
+            store_stress(s.sig1, E, e, st);
+        }
+        eos(mt.eos, s.gamma[e], s.csmin[e], den, sie, p, cs, st);
+        s.vol[e] = vol;
+        s.den[e] = den;
+        s.pres[e] = p;
+        s.sspd[e] = cs;
+        s.div_out[e] = div;
+        if (dt_partial != nullptr)
+            dtl = elem_dt(min_pair_distance(x, y, z), cs, c.dt_cfl, c.dt_max, c.dt_min, c.fuzz);
+    }
+    if (dt_partial != nullptr) {
+        dtl = block_min<kBlock>(dtl);
+        if (threadIdx.x == 0) dt_partial[bi.lid] = dtl;
+    }
+}
+
+// update_state with a stress field (K/properties.cpp:9-114)
+__global__ __launch_bounds__(kBlock) void k_update_state_stress(DMesh m, DState s, double dt,
+                                                                double alpha)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    const size_t E = static_cast<size_t>(m.E);
+    const double den = s.emass[e] / s.vol[e];
+    s.den[e] = den;
+    const int32_t mat = s.nmat > 1 ? s.mat[e] : 0;
+    const Material mt = s.mats[mat];
+    double st[9];
+    load_stress(s.sig1, E, e, st);
+    double p = s.pres[e], c = s.sspd[e];
+    if (mt.strength == kStrengthHyper) {
+        int32_t nd[8];
+        load_conn(m.conn, e, nd);
+        double x[8], y[8], z[8], u[8], v[8], w[8], B[24], L[9];
+        gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+        gather3(s.u1, s.v1, s.w1, nd, u, v, w);
+        hex_bmatrix(x, y, z, B);
+        velgrad(B, u, v, w, s.vol[e], L);
+        user_strength(mat, s.sparams + kStrengthParams * mat, den, s.sie0[e], L, s.vol[e], dt,
+                      alpha, st, p, c);
+        store_stress(s.sig1, E, e, st);
+    }
+    eos(mt.eos, s.gamma[e], s.csmin[e], den, s.sie1[e], p, c, st);
+    s.pres[e] = p;
+    s.sspd[e] = c;
+}
+
+// velocity gradient of the current state (K/momentum.cpp:60-148), E x 9
+__global__ __launch_bounds__(kBlock) void k_velgrad(DMesh m, DState s, double* out)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    int32_t nd[8];
+    load_conn(m.conn, e, nd);
+    double x[8], y[8], z[8], u[8], v[8], w[8], B[24], L[9];
+    gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+    gather3(s.u1, s.v1, s.w1, nd, u, v, w);
+    hex_bmatrix(x, y, z, B);
+    velgrad(B, u, v, w, s.vol[e], L);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[9 * static_cast<size_t>(e) + k] = L[k];
+}
+
 // end of a cycle (K/sgh_solve.cpp:590-632)
 __global__ void k_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin)
 {
@@ -618,7 +799,7 @@ __global__ __launch_bounds__(kBlock) void k_energy_final(const double* scratch, 
 }
 
 __global__ __launch_bounds__(kBlock) void k_swap_bins(DState s, int32_t n0, int32_t n1, int32_t e0,
-                                                      int32_t e1)
+                                                      int32_t e1, int32_t Etot)
 {
     const int i = blockIdx.x * kBlock + static_cast<int>(threadIdx.x);
     const int n = n0 + i;
@@ -636,6 +817,13 @@ __global__ __launch_bounds__(kBlock) void k_swap_bins(DState s, int32_t n0, int3
         const double t = s.sie0[e];
         s.sie0[e] = s.sie1[e];
         s.sie1[e] = t;
+        if (s.sig0 != nullptr)
+            for (int k = 0; k < 9; ++k) {
+                const size_t q = static_cast<size_t>(k) * Etot + e;
+                const double u = s.sig0[q];
+                s.sig0[q] = s.sig1[q];
+                s.sig1[q] = u;
+            }
     }
 }
 
@@ -797,6 +985,31 @@ void launch_energy_state_div(const DMesh& m, const DState& s, const InstCtl* ctl
     FSGH_HEAVY_VARIANTS(FSGH_VH)
 #undef FSGH_VH
 }
+void launch_force_stress(const DMesh& m, const DState& s, const InstCtl* ctl, double fuzz,
+                         double small, double dt, double alpha, hipStream_t st)
+{
+    if (m.n_eblk)
+        hipLaunchKernelGGL(k_force_stress, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, ctl, fuzz,
+                           small, dt, alpha);
+}
+void launch_energy_state_div_stress(const DMesh& m, const DState& s, const InstCtl* ctl,
+                                    double alpha, double* dt_partial, hipStream_t st)
+{
+    if (m.n_eblk)
+        hipLaunchKernelGGL(k_energy_state_div_stress, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s,
+                           ctl, alpha, dt_partial);
+}
+void launch_update_state_stress(const DMesh& m, const DState& s, double dt, double alpha,
+                                hipStream_t st)
+{
+    if (m.n_eblk)
+        hipLaunchKernelGGL(k_update_state_stress, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, dt,
+                           alpha);
+}
+void launch_velgrad(const DMesh& m, const DState& s, double* out, hipStream_t st)
+{
+    if (m.n_eblk) hipLaunchKernelGGL(k_velgrad, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, out);
+}
 void launch_time_advance(InstCtl* ctl, int32_t ninst, int32_t newest_bin, hipStream_t st)
 {
     hipLaunchKernelGGL(k_time_advance, dim3(grid_for(ninst)), dim3(kBlock), 0, st, ctl, ninst,
@@ -812,11 +1025,12 @@ void launch_energy_sums(const DMesh& m, const DState& s, const int32_t* eoff, co
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, scratch, nscratch, out2);
 }
 void launch_swap_bins_range(const DState& s, int32_t n0, int32_t n1, int32_t e0, int32_t e1,
-                            hipStream_t st)
+                            int32_t Etot, hipStream_t st)
 {
     const int32_t cnt = (n1 - n0) > (e1 - e0) ? (n1 - n0) : (e1 - e0);
     if (cnt > 0)
-        hipLaunchKernelGGL(k_swap_bins, dim3(grid_for(cnt)), dim3(kBlock), 0, st, s, n0, n1, e0, e1);
+        hipLaunchKernelGGL(k_swap_bins, dim3(grid_for(cnt)), dim3(kBlock), 0, st, s, n0, n1, e0, e1,
+                           Etot);
 }
 void launch_check(const DMesh& m, const DState& s, int32_t* flag, hipStream_t st)
 {

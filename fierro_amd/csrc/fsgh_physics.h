@@ -13,6 +13,7 @@
 #endif
 #include "fsgh_dual.h"
 #include "fsgh_user_eos.h"
+#include "fsgh_user_strength.h"
 #include "hex_geometry.inc"
 
 namespace fsgh {
@@ -50,13 +51,38 @@ FSGH_HD inline void ideal_gas(T gamma, double csmin, T den, T sie, T& pres, T& s
 // the material's equation of state (the eos_model pointer of K/state.h:140-147)
 constexpr int32_t kEosIdealGas = 0;
 constexpr int32_t kEosUser = 1;
+// stress: the element's Cauchy stress (row-major 3 x 3) or nullptr (zero)
 template <typename T>
-FSGH_HD inline void eos(int32_t kind, T gamma, double csmin, T den, T sie, T& pres, T& sspd)
+FSGH_HD inline void eos(int32_t kind, T gamma, double csmin, T den, T sie, T& pres, T& sspd,
+                        const double* stress = nullptr)
 {
     if (kind == kEosUser)
-        user_eos(gamma, csmin, den, sie, pres, sspd);
+        user_eos(gamma, csmin, den, sie, stress, pres, sspd);
     else
         ideal_gas(gamma, csmin, den, sie, pres, sspd);
+}
+
+// strength models of a material (the strength_type of K/state.h:166-167)
+constexpr int32_t kStrengthNone = 0;
+constexpr int32_t kStrengthHypo = 1;
+constexpr int32_t kStrengthHyper = 2;
+
+// velocity gradient L_ij = sum_p u_p^i B_pj / V with the outward B
+// (K/momentum.cpp:60-148)
+template <typename T>
+FSGH_HD inline void velgrad(const T* B, const T* u, const T* v, const T* w, T vol, T* L)
+{
+    const T inverse_vol = 1.0 / vol;
+    const T* c[3] = {u, v, w};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            T acc = c[i][0] * B[j];
+#pragma unroll
+            for (int p = 1; p < 8; ++p) acc = acc + c[i][p] * B[3 * p + j];
+            L[3 * i + j] = acc * inverse_vol;
+        }
 }
 
 // Corner forces of one element, 3D, strength-free (K/force_sgh.cpp:35-359).
@@ -122,6 +148,75 @@ FSGH_HD inline void force_elem(const T* x, const T* y, const T* z, const T* u, c
         f[3 * a + 0] = B[3 * a + 0] * pres + pm * (vsx - u[a]);
         f[3 * a + 1] = B[3 * a + 1] * pres + pm * (vsy - v[a]);
         f[3 * a + 2] = B[3 * a + 2] * pres + pm * (vsz - w[a]);
+    }
+}
+
+// Corner forces of one element, 3D, with a stress field: tau = stress - p I
+// (K/force_sgh.cpp:127-137) and F_i,d = a_i . tau_:,d + phi mu_i (v* - v_i)
+// with the inward normals a = -B (:103-107, :338-359), term by term as the
+// reference.  B (outward) is returned for the velocity gradient.
+FSGH_HD inline void force_elem_stressed(const double* x, const double* y, const double* z,
+                                        const double* u, const double* v, const double* w,
+                                        double den, double pres, double sspd, double vol,
+                                        double div, double ext, double q1, double q1ex,
+                                        double q2, double q2ex, double fuzz, double small,
+                                        const double* stress, double* B, double* f)
+{
+    hex_bmatrix(x, y, z, B);
+    double vsx = 0.0, vsy = 0.0, vsz = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        vsx += 0.125 * u[a];
+        vsy += 0.125 * v[a];
+        vsz += 0.125 * w[a];
+    }
+    const double qa = div < 0.0 ? q1 : q1ex;
+    const double qb = div < 0.0 ? q2 : q2ex;
+    double sum0 = 0.0, sum1 = 0.0, sum2 = 0.0, sum3 = 0.0;
+    double mu[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const double dx = u[a] - vsx, dy = v[a] - vsy, dz = w[a] - vsz;
+        const double mag_vel = sqrt(dx * dx + dy * dy + dz * dz);
+        const double muc = den * (qa * sspd + qb * mag_vel);
+        const double bx = B[3 * a], by = B[3 * a + 1], bz = B[3 * a + 2];
+        const double mu_term = muc * sqrt(bx * bx + by * by + bz * bz);
+        sum0 += mu_term * u[a];
+        sum1 += mu_term * v[a];
+        sum2 += mu_term * w[a];
+        sum3 += mu_term;
+        mu[a] = mu_term;
+    }
+    if (sum3 > fuzz) {
+        vsx = sum0 / sum3;
+        vsy = sum1 / sum3;
+        vsz = sum2 / sum3;
+    } else {
+        vsx = 0.0;
+        vsy = 0.0;
+        vsz = 0.0;
+    }
+    const double r_face = 0.9 * (ext + small) / (div + small);
+    const double r_min = fmin(r_face, 1.0);
+    double phi = 1.0 - fmax(0.0, r_min);
+    const double c_length = pow(vol, 1.0 / 3.0);
+    const double alpha = fmin(1.0, 20.0 * (c_length * fabs(div)) / (sspd + fuzz));
+    phi = alpha * phi;
+    double tau[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tau[k] = stress[k];
+    tau[0] -= pres;
+    tau[4] -= pres;
+    tau[8] -= pres;
+    const double vs[3] = {vsx, vsy, vsz};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const double ax = (-1.0) * B[3 * a], ay = (-1.0) * B[3 * a + 1], az = (-1.0) * B[3 * a + 2];
+        const double vel[3] = {u[a], v[a], w[a]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            f[3 * a + d] = ax * tau[d] + ay * tau[3 + d] + az * tau[6 + d] +
+                           phi * mu[a] * (vs[d] - vel[d]);
     }
 }
 

@@ -74,18 +74,23 @@ __device__ __forceinline__ double rz_block_min(double v)
     return r;
 }
 
-// corner forces of one quad (K/force_sgh.cpp:407-808), strength-free: tau =
-// -p I, so a_i . tau_d = B_id p; hoop terms tau_10 A_i / R and
-// (tau_11 - tau_22) A_i / R at nodes off the axis (:755-769)
+// corner forces of one quad (K/force_sgh.cpp:407-808).  Strength-free
+// (stress == nullptr): tau = -p I, so a_i . tau_d = B_id p.  With a stress
+// field: tau = stress - p I and a_i . tau_d = a_i0 tau_0d + a_i1 tau_1d with
+// a = -B, term by term.  Hoop terms tau_10 A_i / R and (tau_11 - tau_22)
+// A_i / R at nodes off the axis (:755-769).  B (outward) and the area are
+// returned for the velocity gradient.
 __device__ __forceinline__ void rz_force_elem(const double x[4], const double y[4],
                                               const double u[4], const double v[4], double den,
                                               double pres, double sspd, double div, double ext,
                                               const Material& mt, double fuzz, double small,
+                                              const double* stress, double B[8], double& area,
                                               double f[8])
 {
-    double B[8], A[4];
+    double A[4];
     rz::bmatrix(x, y, B);
     const double elem_area = rz::area(x, y);
+    area = elem_area;
     rz::corner_areas(x, y, A);
     double vs0 = 0.0, vs1 = 0.0;
 #pragma unroll
@@ -122,13 +127,27 @@ __device__ __forceinline__ void rz_force_elem(const double x[4], const double y[
     const double c_length = sqrt(elem_area);
     const double alpha = fmin(1.0, 20.0 * (c_length * fabs(div)) / (sspd + fuzz));
     phi = alpha * phi;
-    const double tau10 = 0.0;
-    const double tau11 = 0.0 - pres, tau22 = 0.0 - pres;
+    double tau00 = 0.0 - pres, tau01 = 0.0, tau10 = 0.0;
+    double tau11 = 0.0 - pres, tau22 = 0.0 - pres;
+    if (stress != nullptr) {
+        tau00 = stress[0] - pres;
+        tau01 = stress[1];
+        tau10 = stress[3];
+        tau11 = stress[4] - pres;
+        tau22 = stress[8] - pres;
+    }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         const double pm = phi * mu[a];
-        double f0 = B[2 * a + 0] * pres + pm * (vs0 - u[a]);
-        double f1 = B[2 * a + 1] * pres + pm * (vs1 - v[a]);
+        double f0, f1;
+        if (stress != nullptr) {
+            const double ax = (-1.0) * B[2 * a], ay = (-1.0) * B[2 * a + 1];
+            f0 = ax * tau00 + ay * tau10 + pm * (vs0 - u[a]);
+            f1 = ax * tau01 + ay * tau11 + pm * (vs1 - v[a]);
+        } else {
+            f0 = B[2 * a + 0] * pres + pm * (vs0 - u[a]);
+            f1 = B[2 * a + 1] * pres + pm * (vs1 - v[a]);
+        }
         const double node_radius = y[a];
         if (node_radius > 1e-14) {
             f0 += tau10 * A[a] / node_radius;
@@ -156,14 +175,33 @@ __device__ __forceinline__ double rz_neighbour_extreme(const DState& s, const in
     return ext;
 }
 
+// velocity gradient in RZ with the outward B (K/momentum.cpp:151-213)
+__device__ __forceinline__ void rz_velgrad(const double B[8], const double u[4], const double v[4],
+                                           double vol, double area, double L[9])
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) L[k] = 0.0;
+    const double mean_radius = vol / area;
+    const double elem_vel_r = 0.25 * (v[0] + v[1] + v[2] + v[3]);
+    const double inverse_area = 1.0 / area;
+    L[0] = (u[0] * B[0] + u[1] * B[2] + u[2] * B[4] + u[3] * B[6]) * inverse_area;
+    L[1] = (u[0] * B[1] + u[1] * B[3] + u[2] * B[5] + u[3] * B[7]) * inverse_area;
+    L[3] = (v[0] * B[0] + v[1] * B[2] + v[2] * B[4] + v[3] * B[6]) * inverse_area;
+    L[4] = (v[0] * B[1] + v[1] * B[3] + v[2] * B[5] + v[3] * B[7]) * inverse_area;
+    L[8] = elem_vel_r / mean_radius;
+}
+
+template <bool STRESS>
 __global__ __launch_bounds__(kB) void k_rz_force(DMesh m, DState s, const InstCtl* ctl,
-                                                 double fuzz, double small)
+                                                 double fuzz, double small, double dt,
+                                                 double alpha)
 {
     const BlockItem bi = m.eblk[blockIdx.x];
     if (ctl != nullptr) {
         if (ctl[bi.inst].stopped) return;
         fuzz = ctl[bi.inst].fuzz;
         small = ctl[bi.inst].small;
+        dt = ctl[bi.inst].dt;
     }
     const int e = bi.start + static_cast<int>(threadIdx.x);
     if (e >= bi.end) return;
@@ -174,11 +212,47 @@ __global__ __launch_bounds__(kB) void k_rz_force(DMesh m, DState s, const InstCt
     gather2(s.u1, s.v1, nd, u, v);
     const double div = s.div[e];
     const double ext = rz_neighbour_extreme(s, nd, div, small);
-    const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
-    rz_force_elem(x, y, u, v, s.den[e], s.pres[e], s.sspd[e], div, ext, mt, fuzz, small, f);
+    const int32_t mat = s.nmat > 1 ? s.mat[e] : 0;
+    const Material mt = s.mats[mat];
+    double B[8], area, st[9];
+    const size_t E = static_cast<size_t>(m.E);
+    if (STRESS)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) st[k] = s.sig1[static_cast<size_t>(k) * E + e];
+    rz_force_elem(x, y, u, v, s.den[e], s.pres[e], s.sspd[e], div, ext, mt, fuzz, small,
+                  STRESS ? st : nullptr, B, area, f);
     double2* out = reinterpret_cast<double2*>(s.F + 8 * static_cast<size_t>(e));
 #pragma unroll
     for (int a = 0; a < 4; ++a) out[a] = make_double2(f[2 * a], f[2 * a + 1]);
+    if (STRESS) {
+        if (mt.strength == kStrengthHypo) {
+            double L[9];
+            rz_velgrad(B, u, v, s.vol[e], area, L);
+            double p = s.pres[e], c = s.sspd[e];
+            user_strength(mat, s.sparams + kStrengthParams * mat, s.den[e], s.sie0[e], L, s.vol[e],
+                          dt, alpha, st, p, c);
+            s.pres[e] = p;
+            s.sspd[e] = c;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s.sig1w[static_cast<size_t>(k) * E + e] = st[k];
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_rz_velgrad(DMesh m, DState s, double* out)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    if (e >= bi.end) return;
+    int32_t nd[4];
+    load_quad(m, e, nd);
+    double x[4], y[4], u[4], v[4], B[8], L[9];
+    gather2(s.x1, s.y1, nd, x, y);
+    gather2(s.u1, s.v1, nd, u, v);
+    rz::bmatrix(x, y, B);
+    rz_velgrad(B, u, v, s.vol[e], rz::area(x, y), L);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[9 * static_cast<size_t>(e) + k] = L[k];
 }
 
 // corner -> node force sum in corners_in_node order (K/momentum.cpp:31-41)
@@ -420,8 +494,15 @@ __global__ __launch_bounds__(kB) void k_rz_energy_state_div(DMesh m, DState s, c
         gather2(s.x1, s.y1, nd, x, y);
         const double vol = rz::volume(x, y);
         const double den = emass / vol;
-        double p, cs;
-        eos(s.mats[s.nmat > 1 ? s.mat[e] : 0].eos, s.gamma[e], s.csmin[e], den, sie, p, cs);
+        double p, cs, st[9];
+        const double* stp = nullptr;
+        if (s.sig1 != nullptr) {
+            const size_t E = static_cast<size_t>(m.E);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) st[k] = s.sig1[static_cast<size_t>(k) * E + e];
+            stp = st;
+        }
+        eos(s.mats[s.nmat > 1 ? s.mat[e] : 0].eos, s.gamma[e], s.csmin[e], den, sie, p, cs, stp);
         rz::bmatrix(x, y, B);
         const double div = rz::divergence(B, u1, v1, rz::area(x, y), vol);
         s.vol[e] = vol;
@@ -452,10 +533,19 @@ void launch_rz_get_divergence(const DMesh& m, const DState& s, hipStream_t st)
     if (m.n_eblk) hipLaunchKernelGGL(k_rz_get_divergence, dim3(m.n_eblk), dim3(kB), 0, st, m, s);
 }
 void launch_rz_force(const DMesh& m, const DState& s, const InstCtl* ctl, double fuzz,
-                     double small, hipStream_t st)
+                     double small, double dt, double alpha, hipStream_t st)
 {
-    if (m.n_eblk)
-        hipLaunchKernelGGL(k_rz_force, dim3(m.n_eblk), dim3(kB), 0, st, m, s, ctl, fuzz, small);
+    if (!m.n_eblk) return;
+    if (s.sig1 != nullptr)
+        hipLaunchKernelGGL(k_rz_force<true>, dim3(m.n_eblk), dim3(kB), 0, st, m, s, ctl, fuzz, small,
+                           dt, alpha);
+    else
+        hipLaunchKernelGGL(k_rz_force<false>, dim3(m.n_eblk), dim3(kB), 0, st, m, s, ctl, fuzz,
+                           small, dt, alpha);
+}
+void launch_rz_velgrad(const DMesh& m, const DState& s, double* out, hipStream_t st)
+{
+    if (m.n_eblk) hipLaunchKernelGGL(k_rz_velgrad, dim3(m.n_eblk), dim3(kB), 0, st, m, s, out);
 }
 void launch_rz_update_velocity(const DMesh& m, const DState& s, double alpha, double dt,
                                hipStream_t st)

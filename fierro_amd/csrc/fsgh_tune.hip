@@ -63,23 +63,9 @@ DMesh mesh_for(fsgh_ctx* c, int k, const KernelCfg& g)
 // kernel wrote: element dt of the current coordinates and sound speed).
 int resync_dt(fsgh_ctx* c)
 {
-    if (!c->run_active) return FSGH_OK;
-    std::vector<InstCtl> h(static_cast<size_t>(c->L.ninst));
-    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_ctl, h.size() * sizeof(InstCtl), hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const std::vector<int32_t>& beg = dt_block_begin(c);
-    for (size_t i = 0; i < h.size(); ++i) {
-        h[i].eblk_begin = beg[i];
-        h[i].eblk_end = beg[i + 1];
-    }
-    HIPCHK(c, hipMemcpyAsync(c->d_ctl, h.data(), h.size() * sizeof(InstCtl), hipMemcpyHostToDevice,
-                             c->stream));
-    launch_dt_partial(tuned_mesh(c, kTunedEnergy), make_state(c, 1 - c->cur, c->cur), c->d_ctl,
-                      c->d_partial, 0, 0, 0, 0, 1, c->stream);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return FSGH_OK;
+    return resync_dt_ranges(c);
 }
+
 
 int apply(fsgh_ctx* c, int k, const KernelCfg& g)
 {
@@ -206,10 +192,40 @@ int fsgh::ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd)
 
 DMesh fsgh::tuned_mesh(fsgh_ctx* c, int kernel) { return mesh_for(c, kernel, c->tune[kernel]); }
 
+DMesh fsgh::dt_mesh(fsgh_ctx* c)
+{
+    if (c->dims == 2 || c->d_stress[0]) return make_mesh(c);
+    return tuned_mesh(c, kTunedEnergy);
+}
+
+int fsgh::resync_dt_ranges(fsgh_ctx* c)
+{
+    if (!c->run_active) return FSGH_OK;
+    std::vector<InstCtl> h(static_cast<size_t>(c->L.ninst));
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_ctl, h.size() * sizeof(InstCtl), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const std::vector<int32_t>& beg = dt_block_begin(c);
+    for (size_t i = 0; i < h.size(); ++i) {
+        h[i].eblk_begin = beg[i];
+        h[i].eblk_end = beg[i + 1];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_ctl, h.data(), h.size() * sizeof(InstCtl), hipMemcpyHostToDevice,
+                             c->stream));
+    const DState s = make_state(c, 1 - c->cur, c->cur);
+    if (c->dims == 2)
+        launch_rz_dt_partial(dt_mesh(c), s, c->d_ctl, c->d_partial, 0, 0, 0, 0, 1, c->stream);
+    else
+        launch_dt_partial(dt_mesh(c), s, c->d_ctl, c->d_partial, 0, 0, 0, 0, 1, c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FSGH_OK;
+}
+
+
 const std::vector<int32_t>& fsgh::dt_block_begin(fsgh_ctx* c)
 {
     const KernelCfg& g = c->tune[kTunedEnergy];
-    if (default_table(g) || c->dims == 2) return c->eblk_begin;
+    if (default_table(g) || c->dims == 2 || c->d_stress[0]) return c->eblk_begin;
     return c->tables.at(table_key(true, g.bs, g.xcd)).inst_begin;
 }
 
@@ -219,8 +235,8 @@ int fsgh_set_tuning(fsgh_ctx* c, const fsgh_tuning* cfg, int32_t count)
 {
     CHECK_CTX(c);
     if (c->host_only) return FSGH_E_STATE;
-    if (c->dims == 2) {
-        set_error(c, "launch tuning covers the 3D kernels only");
+    if (c->dims == 2 || c->d_stress[0]) {
+        set_error(c, "launch tuning covers the 3D strength-free kernels only");
         return FSGH_E_UNSUPPORTED;
     }
     if (count < 0 || (count > 0 && !cfg)) return FSGH_E_INVALID;
@@ -260,8 +276,8 @@ int fsgh_autotune(fsgh_ctx* c, int32_t reps, fsgh_tuning* out)
 {
     CHECK_CTX(c);
     if (c->host_only) return FSGH_E_STATE;
-    if (c->dims == 2) {
-        set_error(c, "launch tuning covers the 3D kernels only");
+    if (c->dims == 2 || c->d_stress[0]) {
+        set_error(c, "launch tuning covers the 3D strength-free kernels only");
         return FSGH_E_UNSUPPORTED;
     }
     if (c->nmat <= 0) {

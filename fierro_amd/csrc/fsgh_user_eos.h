@@ -8,11 +8,15 @@
 // and rebuilds the library.
 //
 // Inputs: gamma = statev(e, 0), csmin = statev(e, 1), density, specific
-// internal energy.  Outputs: pressure and sound speed.
+// internal energy, and the element's Cauchy stress (row-major 3 x 3), or
+// nullptr where no stress field exists (the stress is then zero).  Outputs:
+// pressure and sound speed.
 //
-// The shipped body is the reference's: pressure = -tr(stress)/3, which is 0
-// on the strength-free path (the stress is identically zero), and sound
-// speed 1e-8.
+// The shipped body is the reference's: pressure = -tr(stress)/3 and sound
+// speed 1e-8.  (The reference reads elem_stress(i,i) with two indices of its
+// four-index stress view; here the model sees the element's own stress,
+// which gives the same result whenever the stress is zero -- always with the
+// reference's empty strength model.)
 #pragma once
 
 #ifndef FSGH_HD
@@ -22,13 +26,18 @@
 namespace fsgh {
 
 template <typename T>
-FSGH_HD inline void user_eos(T gamma, double csmin, T den, T sie, T& pres, T& sspd)
+FSGH_HD inline void user_eos(T gamma, double csmin, T den, T sie, const double* stress, T& pres,
+                             T& sspd)
 {
     (void)gamma;
     (void)csmin;
     (void)den;
     (void)sie;
-    pres = 0.0;
+    T p = 0.0;
+    if (stress != nullptr)
+        for (int i = 0; i < 3; ++i) p = p - stress[4 * i];
+    p = p * (1.0 / 3.0);
+    pres = p;
     sspd = 1.0e-8;
 }
 

@@ -30,6 +30,7 @@ CARTESIAN, RADIAL, SPHERICAL = 0, 1, 2
 X_PLANE, Y_PLANE, Z_PLANE = 0, 1, 2
 FIXED, REFLECTED = 0, 1
 EOS_IDEAL_GAS, EOS_USER = 0, 1      # the material models of K/state.h:140-170
+STRENGTH_NONE, STRENGTH_HYPO, STRENGTH_HYPER = 0, 1, 2   # strength_tag, K/state.h:107-112
 
 # K/driver.cpp:35-62 defaults
 DEFAULT_RUN = dict(time_final=1.0e16, dt_start=1.0e-8, dt_max=1.0e-2, dt_min=1.0e-8,
@@ -58,6 +59,7 @@ class ProblemSpec:
     num_state_vars: int = 6
     eos: list = None                # per material: EOS_IDEAL_GAS (default) or EOS_USER
     dims: int = 3                   # 2: axisymmetric RZ (mesh kinds "rect", "geo2")
+    strength: list = None           # per material: STRENGTH_NONE (default), _HYPO or _HYPER
 
     def mesh_arrays(self, oracle_gen=None, oracle_read=None):
         """(coords N x dims, nodes_in_elem E x 2^dims).  The oracle passes its

@@ -36,13 +36,15 @@ FIELD_SHAPES = {
     _abi.ELEM_GAMMA: ("elem", (None,), np.float64),
     _abi.ELEM_CSMIN: ("elem", (None,), np.float64),
     _abi.CORNER_FORCE: ("corner", (None, "D"), np.float64),
+    _abi.ELEM_STRESS: ("elem", (2, None, 3, 3), np.float64),
 }
 FIELDS = {"node_coords": _abi.NODE_COORDS, "node_vel": _abi.NODE_VEL,
           "node_mass": _abi.NODE_MASS, "elem_den": _abi.ELEM_DEN, "elem_pres": _abi.ELEM_PRES,
           "elem_sspd": _abi.ELEM_SSPD, "elem_sie": _abi.ELEM_SIE, "elem_vol": _abi.ELEM_VOL,
           "elem_div": _abi.ELEM_DIV, "elem_mass": _abi.ELEM_MASS,
           "elem_mat_id": _abi.ELEM_MAT_ID, "elem_gamma": _abi.ELEM_GAMMA,
-          "elem_csmin": _abi.ELEM_CSMIN, "corner_force": _abi.CORNER_FORCE}
+          "elem_csmin": _abi.ELEM_CSMIN, "corner_force": _abi.CORNER_FORCE,
+          "elem_stress": _abi.ELEM_STRESS}
 
 
 def _p(a: np.ndarray):
@@ -103,10 +105,14 @@ class SGHSolver:
         arr = (_abi.Material * len(mats))(*[_abi.Material(*m) for m in mats])
         check(self.L.fsgh_set_materials(self.ctx, len(mats), arr), self.ctx, "set_materials")
         for i, s in enumerate(self.specs):
-            for k, kind in enumerate(s.eos or []):
-                if kind != _abi.EOS_IDEAL_GAS:
+            eos = list(s.eos or [])
+            strength = list(getattr(s, "strength", None) or [])
+            for k in range(len(s.materials)):
+                kind = eos[k] if k < len(eos) else _abi.EOS_IDEAL_GAS
+                st = strength[k] if k < len(strength) else _abi.STRENGTH_NONE
+                if kind != _abi.EOS_IDEAL_GAS or st != _abi.STRENGTH_NONE:
                     check(self.L.fsgh_set_material_models(self.ctx, index[i] + k, int(kind),
-                                                          _abi.STRENGTH_NONE),
+                                                          int(st)),
                           self.ctx, "set_material_models")
         for i, s in enumerate(self.specs):
             bcs = (_abi.Boundary * max(len(s.bcs), 1))(
@@ -251,6 +257,17 @@ class SGHSolver:
         self.update_state(dt, rk_alpha)
         if self.dims == 2:
             self.rz_update_node_mass(r["tiny"])
+
+    def get_velgrad(self):
+        """Velocity gradient of every element, E x 3 x 3 (caller numbering)."""
+        out = np.zeros((self.E, 3, 3))
+        check(self.L.fsgh_get_velgrad(self.ctx, _p(out)), self.ctx, "get_velgrad")
+        return out
+
+    def set_strength_params(self, material, params):
+        p = np.ascontiguousarray(params, np.float64)
+        check(self.L.fsgh_set_strength_params(self.ctx, int(material), p.size, _p(p)), self.ctx,
+              "set_strength_params")
 
     # -- 2D RZ node mass (K/sgh_solve.cpp:107-116, 495-581) --
     def rz_extensive_mass(self):

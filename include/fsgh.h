@@ -112,13 +112,27 @@ int fsgh_set_materials(fsgh_ctx *ctx, int32_t num_materials, const fsgh_material
  * eos: FSGH_EOS_IDEAL_GAS (K/eos.cpp:12-46) or FSGH_EOS_USER, the device
  * function fsgh::user_eos of fierro_amd/csrc/fsgh_user_eos.h compiled into
  * the library (the counterpart of user_eos_model, K/user_mat.cpp:14-40).
- * strength: FSGH_STRENGTH_NONE only (hypo/hyper-elastic models:
- * FSGH_E_UNSUPPORTED).  Call after fsgh_set_materials and before
- * fsgh_setup_fills. */
+ * strength: FSGH_STRENGTH_NONE, or FSGH_STRENGTH_HYPO / _HYPER, which call the
+ * device function fsgh::user_strength (fierro_amd/csrc/fsgh_user_strength.h,
+ * the counterpart of user_strength_model, K/user_mat.cpp:47-90) from the
+ * force kernel after the corner forces (K/force_sgh.cpp:369-391) or from the
+ * state update before the EOS (K/properties.cpp:46-95), with the element's
+ * velocity gradient (K/momentum.cpp:60-148).  Any strength model allocates a
+ * stress field (zero, K/setup.cpp:214-220) that the force kernels then use
+ * (tau = stress - p I, K/force_sgh.cpp:127-137) and rk_init carries.  2D:
+ * HYPO only.  The adjoint and launch tuning cover the strength-free cycle.
+ * Call after fsgh_set_materials and before fsgh_setup_fills. */
 #define FSGH_EOS_IDEAL_GAS 0
 #define FSGH_EOS_USER 1
 #define FSGH_STRENGTH_NONE 0
+#define FSGH_STRENGTH_HYPO 1
+#define FSGH_STRENGTH_HYPER 2
 int fsgh_set_material_models(fsgh_ctx *ctx, int32_t material, int32_t eos, int32_t strength);
+/* Parameters (up to FSGH_STRENGTH_PARAMS) passed to the user strength model
+ * of `material` (zero until set). */
+#define FSGH_STRENGTH_PARAMS 8
+int fsgh_set_strength_params(fsgh_ctx *ctx, int32_t material, int32_t num_params,
+                             const double *params);
 
 /* ----------------------------------------------------- boundary sets --- */
 typedef struct fsgh_boundary {
@@ -178,7 +192,9 @@ typedef enum fsgh_field {
     FSGH_ELEM_MAT_ID = 10, /* int64  [E] */
     FSGH_ELEM_GAMMA = 11,  /* double [E]  = statev(e,0) */
     FSGH_ELEM_CSMIN = 12,  /* double [E]  = statev(e,1) */
-    FSGH_CORNER_FORCE = 13 /* double [8E][3] */
+    FSGH_CORNER_FORCE = 13, /* double [8E][3] */
+    FSGH_ELEM_STRESS = 14   /* double [2][E][3][3] Cauchy stress; an upload allocates the
+                               stress field (it is zero and absent otherwise) */
 } fsgh_field;
 int fsgh_upload(fsgh_ctx *ctx, int32_t field, const void *host, size_t bytes);
 int fsgh_download(fsgh_ctx *ctx, int32_t field, void *host, size_t bytes);
@@ -209,6 +225,11 @@ int fsgh_energy(fsgh_ctx *ctx, int32_t instance, double *ie, double *ke);
  * (K/sgh_solve.cpp:495-581; tiny = 1e-12 in K/driver.cpp:61).  3D
  * contexts: FSGH_E_STATE. */
 int fsgh_rz_extensive_mass(fsgh_ctx *ctx);
+/* The velocity gradient L_ij = du_i/dx_j of every element from the current
+ * coordinates, velocities and volume (get_velgrad, K/momentum.cpp:60-148;
+ * 2D: get_velgrad2D, K/momentum.cpp:151-213, with L_22 = v_r / r), as
+ * E x 3 x 3 in caller numbering: the input the strength models see. */
+int fsgh_get_velgrad(fsgh_ctx *ctx, double *vel_grad);
 int fsgh_rz_update_node_mass(fsgh_ctx *ctx, double tiny);
 
 /* -------------------------------------------------- fused cycle loop --- */

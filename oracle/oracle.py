@@ -37,7 +37,8 @@ class OrcState(C.Structure):
                 ("elem_den", P), ("elem_pres", P), ("elem_sspd", P),
                 ("elem_sie", P), ("elem_vol", P), ("elem_div", P),
                 ("elem_mass", P), ("elem_mat_id", P), ("elem_statev", P),
-                ("num_state_vars", C.c_int64), ("corner_force", P), ("mat_eos", P)]
+                ("num_state_vars", C.c_int64), ("corner_force", P), ("mat_eos", P),
+                ("elem_stress", P), ("mat_strength", P)]
 
 
 class OrcMaterial(C.Structure):
@@ -215,13 +216,27 @@ class OracleProblem:
                                _ptr(self.elem_den), _ptr(self.elem_pres), _ptr(self.elem_sspd),
                                _ptr(self.elem_sie), _ptr(self.elem_vol), _ptr(self.elem_div),
                                _ptr(self.elem_mass), _ptr(self.elem_mat_id),
-                               _ptr(self.elem_statev), nsv, _ptr(self.corner_force), None)
+                               _ptr(self.elem_statev), nsv, _ptr(self.corner_force), None, None,
+                               None)
         mats = spec.materials
         if spec.eos:
             self.mat_eos = np.ascontiguousarray(spec.eos, np.int64)
             self._state.mat_eos = _ptr(self.mat_eos)
+        self.elem_stress = None
+        if getattr(spec, "strength", None):
+            self.enable_stress()
         self._mat = (OrcMaterial * len(mats))(*[OrcMaterial(*m) for m in mats])
         self.setup()
+
+    def enable_stress(self):
+        """Carry a stress field [2][E][3][3] (zero, as K/setup.cpp:214-220
+        sets it) and the materials' strength models."""
+        if self.elem_stress is None:
+            self.elem_stress = np.zeros((2, self.E, 3, 3))
+            self._state.elem_stress = _ptr(self.elem_stress)
+        st = getattr(self.spec, "strength", None) or [0] * len(self.spec.materials)
+        self.mat_strength = np.ascontiguousarray(st, np.int64)
+        self._state.mat_strength = _ptr(self.mat_strength)
 
     # -- setup: K/driver.cpp:239 get_vol, then K/setup.cpp --
     def setup(self):
@@ -263,8 +278,9 @@ class OracleProblem:
         self.L.orc_update_position_sgh(C.byref(self._mesh), C.byref(self._state),
                                        C.c_double(rk_alpha), C.c_double(dt))
 
-    def update_state(self):
-        self.L.orc_update_state(C.byref(self._mesh), C.byref(self._state))
+    def update_state(self, dt=0.0, rk_alpha=1.0):
+        self.L.orc_update_state(C.byref(self._mesh), C.byref(self._state), C.c_double(dt),
+                                C.c_double(rk_alpha))
 
     def rk_init(self):
         self.L.orc_rk_init(C.byref(self._mesh), C.byref(self._state))
@@ -315,7 +331,7 @@ class OracleProblem:
         self.update_energy_sgh(rk_alpha, dt)
         self.update_position_sgh(rk_alpha, dt)
         self.get_vol()
-        self.update_state()
+        self.update_state(dt, rk_alpha)
 
 
 def bmatrix(x, y, z):
@@ -473,16 +489,22 @@ class OracleRZ:
                                _ptr(self.elem_den), _ptr(self.elem_pres), _ptr(self.elem_sspd),
                                _ptr(self.elem_sie), _ptr(self.elem_vol), _ptr(self.elem_div),
                                _ptr(self.elem_mass), _ptr(self.elem_mat_id),
-                               _ptr(self.elem_statev), nsv, _ptr(self.corner_force), None)
+                               _ptr(self.elem_statev), nsv, _ptr(self.corner_force), None, None,
+                               None)
         if spec.eos:
             self.mat_eos = np.ascontiguousarray(spec.eos, np.int64)
             self._state.mat_eos = _ptr(self.mat_eos)
+        self.elem_stress = None
+        if getattr(spec, "strength", None):
+            OracleProblem.enable_stress(self)
         mats = spec.materials
         self._mat = (OrcMaterial * len(mats))(*[OrcMaterial(*m) for m in mats])
         self.setup()
 
     def _ms(self):
         return C.byref(self._mesh), C.byref(self._state)
+
+    enable_stress = OracleProblem.enable_stress
 
     def setup(self):
         L, spec = self.L, self.spec
@@ -503,7 +525,8 @@ class OracleRZ:
 
     def get_force_sgh(self, dt, rk_alpha):
         r = self.spec.run
-        self.L.orz_get_force(*self._ms(), self._mat, C.c_double(r["fuzz"]), C.c_double(r["small"]))
+        self.L.orz_get_force(*self._ms(), self._mat, C.c_double(r["fuzz"]), C.c_double(r["small"]),
+                             C.c_double(dt), C.c_double(rk_alpha))
 
     def update_velocity_sgh(self, rk_alpha, dt):
         self.L.orz_update_velocity(*self._ms(), C.c_double(rk_alpha), C.c_double(dt))
@@ -517,7 +540,7 @@ class OracleRZ:
     def update_position_sgh(self, rk_alpha, dt):
         self.L.orz_update_position(*self._ms(), C.c_double(rk_alpha), C.c_double(dt))
 
-    def update_state(self):
+    def update_state(self, dt=0.0, rk_alpha=1.0):
         self.L.orz_update_state(*self._ms())
 
     def rk_init(self):
@@ -588,3 +611,37 @@ def rz_geometry(x, y):
     L.orz_get_bmatrix2D(B, x, y)
     L.orz_get_area_weights2D(A, x, y)
     return B.reshape(4, 2), L.orz_get_area_quad(x, y), L.orz_get_vol_quad(x, y), A
+
+
+def velgrad_3d(pr):
+    """Velocity gradient of every element of a 3D OracleProblem (the
+    reference's get_velgrad with the outward B), E x 3 x 3."""
+    L = lib()
+    out = np.zeros((pr.E, 3, 3))
+    X, V = pr.node_coords[1], pr.node_vel[1]
+    for e in range(pr.E):
+        nd = pr.conn[e]
+        B = bmatrix(X[nd, 0], X[nd, 1], X[nd, 2]).ravel()
+        u, v, w = (np.ascontiguousarray(V[nd, d]) for d in range(3))
+        o = np.zeros(9)
+        L.orc_get_velgrad(_ptr(o), _ptr(u), _ptr(v), _ptr(w), _ptr(np.ascontiguousarray(B)),
+                          C.c_double(pr.elem_vol[e]))
+        out[e] = o.reshape(3, 3)
+    return out
+
+
+def velgrad_rz(pr):
+    """Velocity gradient of every element of an OracleRZ (get_velgrad2D)."""
+    L = _rz_lib()
+    out = np.zeros((pr.E, 3, 3))
+    X, V = pr.node_coords[1], pr.node_vel[1]
+    for e in range(pr.E):
+        nd = pr.conn[e]
+        x, y = np.ascontiguousarray(X[nd, 0]), np.ascontiguousarray(X[nd, 1])
+        B, area, _, _ = rz_geometry(x, y)
+        u, v = np.ascontiguousarray(V[nd, 0]), np.ascontiguousarray(V[nd, 1])
+        o = np.zeros(9)
+        L.orz_get_velgrad2D(_ptr(o), _ptr(u), _ptr(v), _ptr(np.ascontiguousarray(B.ravel())),
+                            C.c_double(pr.elem_vol[e]), C.c_double(area))
+        out[e] = o.reshape(3, 3)
+    return out

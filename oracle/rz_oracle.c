@@ -79,6 +79,21 @@ void orz_get_area_weights2D(double A[4], const double x[4], const double y[4])
     A[3] = (5. * A41 + A12 + A23 + 5. * A34) / 12.;
 }
 
+/* get_velgrad2D (K/momentum.cpp:151-213) with the outward B */
+void orz_get_velgrad2D(double L[9], const double u[4], const double v[4], const double B[8],
+                       double elem_vol, double elem_area)
+{
+    for (int i = 0; i < 9; ++i) L[i] = 0.0;
+    double mean_radius = elem_vol / elem_area;
+    double elem_vel_r = 0.25 * (v[0] + v[1] + v[2] + v[3]);
+    double inverse_area = 1.0 / elem_area;
+    L[0] = (u[0] * B[0] + u[1] * B[2] + u[2] * B[4] + u[3] * B[6]) * inverse_area;
+    L[1] = (u[0] * B[1] + u[1] * B[3] + u[2] * B[5] + u[3] * B[7]) * inverse_area;
+    L[3] = (v[0] * B[0] + v[1] * B[2] + v[2] * B[4] + v[3] * B[6]) * inverse_area;
+    L[4] = (v[0] * B[1] + v[1] * B[3] + v[2] * B[5] + v[3] * B[7]) * inverse_area;
+    L[8] = elem_vel_r / mean_radius;
+}
+
 static void elem_xy(const orz_mesh *m, const orc_state *s, int64_t e, double *x, double *y)
 {
     const int64_t N = m->num_nodes;
@@ -130,7 +145,7 @@ void orz_get_divergence(const orz_mesh *m, orc_state *s)
  * zero, tau = -p I; the velocity gradient (curl) and the shock direction do
  * not reach the corner force and are omitted. */
 void orz_get_force(const orz_mesh *m, orc_state *s, const orc_material *mat, double fuzz,
-                   double small)
+                   double small, double dt, double rk_alpha)
 {
     const int64_t N = m->num_nodes;
 #pragma omp parallel for schedule(static)
@@ -151,9 +166,10 @@ void orz_get_force(const orz_mesh *m, orc_state *s, const orc_material *mat, dou
         const double den = s->elem_den[e];
         const double sspd = s->elem_sspd[e];
         const double pres = s->elem_pres[e];
+        double *stress = s->elem_stress ? s->elem_stress + (m->num_elems + e) * 9 : NULL;
         double tau[3][3];
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) tau[i][j] = 0.0;
+            for (int j = 0; j < 3; ++j) tau[i][j] = stress ? stress[i * 3 + j] : 0.0;
         for (int i = 0; i < 3; ++i) tau[i][i] -= pres;
 
         double vel_star[2] = {0.0, 0.0};
@@ -205,6 +221,19 @@ void orz_get_force(const orz_mesh *m, orc_state *s, const orc_material *mat, dou
                 F[0] += tau[1][0] * corner_areas[a] / node_radius;
                 F[1] += (tau[1][1] - tau[2][2]) * corner_areas[a] / node_radius;
             }
+        }
+        /* hypo-elastic strength (:774-801) with the velocity gradient of the
+         * outward B (:488-501) */
+        if (stress && s->mat_strength && s->mat_strength[s->elem_mat_id[e]] == 1) {
+            double u[4], v[4], L[9];
+            for (int a = 0; a < 4; ++a) {
+                u[a] = vel[a][0];
+                v[a] = vel[a][1];
+            }
+            orz_get_velgrad2D(L, u, v, B, s->elem_vol[e], elem_area);
+            orc_user_strength(s->elem_mat_id[e], den, s->elem_sie[m->num_elems + e], L,
+                              s->elem_vol[e], dt, rk_alpha, stress, &s->elem_pres[e],
+                              &s->elem_sspd[e]);
         }
     }
 }
@@ -276,10 +305,14 @@ void orz_update_position(const orz_mesh *m, orc_state *s, double rk_alpha, doubl
 
 /* the element's EOS (ideal_gas, K/eos.cpp:12-46, or the user model of
  * K/user_mat.cpp:14-40 with zero stress) */
-static void eos(orc_state *s, int64_t e, double den, double sie)
+static void eos(orc_state *s, int64_t E, int64_t e, double den, double sie)
 {
     if (s->mat_eos && s->mat_eos[s->elem_mat_id[e]] == 1) {
-        s->elem_pres[e] = 0.0;
+        double p = 0.0;
+        if (s->elem_stress)
+            for (int i = 0; i < 3; ++i) p -= s->elem_stress[(E + e) * 9 + i * 3 + i];
+        p *= 1.0 / 3.0;
+        s->elem_pres[e] = p;
         s->elem_sspd[e] = 1.0e-8;
         return;
     }
@@ -297,7 +330,7 @@ void orz_update_state(const orz_mesh *m, orc_state *s)
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < E; ++e) {
         s->elem_den[e] = s->elem_mass[e] / s->elem_vol[e];
-        eos(s, e, s->elem_den[e], s->elem_sie[E + e]);
+        eos(s, E, e, s->elem_den[e], s->elem_sie[E + e]);
     }
 }
 
@@ -306,6 +339,7 @@ void orz_rk_init(const orz_mesh *m, orc_state *s)
 {
     const int64_t N = m->num_nodes, E = m->num_elems;
     memcpy(s->elem_sie, s->elem_sie + E, (size_t)E * sizeof(double));
+    if (s->elem_stress) memcpy(s->elem_stress, s->elem_stress + 9 * E, (size_t)(9 * E) * sizeof(double));
     memcpy(s->node_coords, s->node_coords + 2 * N, (size_t)(2 * N) * sizeof(double));
     memcpy(s->node_vel, s->node_vel + 2 * N, (size_t)(2 * N) * sizeof(double));
 }
@@ -410,7 +444,7 @@ void orz_sgh_solve(const orz_mesh *m, orc_state *s, const orc_material *mat,
         for (int64_t stage = 0; stage < p->rk_num_stages; ++stage) {
             double rk_alpha = 1.0 / ((double)p->rk_num_stages - (double)stage);
             orz_get_divergence(m, s);
-            orz_get_force(m, s, mat, p->fuzz, p->small);
+            orz_get_force(m, s, mat, p->fuzz, p->small, dt, rk_alpha);
             orz_update_velocity(m, s, rk_alpha, dt);
             orz_boundary_velocity(m, s, bc);
             orz_update_energy(m, s, rk_alpha, dt);
@@ -738,7 +772,7 @@ void orz_setup_fills(const orz_mesh *m, orc_state *s, int64_t num_fills, const o
             int64_t mid = F->mat_id;
             for (int64_t v = 0; v < mat_num_state_vars[mid]; ++v)
                 s->elem_statev[e * s->num_state_vars + v] = state_vars[mid * s->num_state_vars + v];
-            eos(s, e, s->elem_den[e], s->elem_sie[E + e]);
+            eos(s, E, e, s->elem_den[e], s->elem_sie[E + e]);
             for (int a = 0; a < 4; ++a) {
                 int64_t n = m->nodes_in_elem[e * 4 + a];
                 if (F->velocity == 0) {

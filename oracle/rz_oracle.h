@@ -55,7 +55,9 @@ void   orz_get_area_weights2D(double A[4], const double x[4], const double y[4])
 void orz_get_vol(const orz_mesh *m, orc_state *s);
 void orz_get_divergence(const orz_mesh *m, orc_state *s);
 void orz_get_force(const orz_mesh *m, orc_state *s, const orc_material *mat, double fuzz,
-                   double small);
+                   double small, double dt, double rk_alpha);
+void orz_get_velgrad2D(double L[9], const double u[4], const double v[4], const double B[8],
+                       double elem_vol, double elem_area);
 void orz_update_velocity(const orz_mesh *m, orc_state *s, double rk_alpha, double dt);
 void orz_boundary_velocity(const orz_mesh *m, orc_state *s, const orc_boundary *bc);
 void orz_update_energy(const orz_mesh *m, orc_state *s, double rk_alpha, double dt);

@@ -178,15 +178,40 @@ void orc_get_divergence(const orc_mesh *m, orc_state *s)
     }
 }
 
-/* get_force_sgh, 3D (K/force_sgh.cpp:12-399); strength-free, so the stress
- * tensor is zero and tau = -p I (K/force_sgh.cpp:127-137).  The velocity
- * gradient / curl (:94-123) and the shock direction (:185-212) do not reach
- * the corner force and are omitted. */
+void orc_user_strength(int64_t mat_id, double den, double sie, const double vel_grad[9],
+                       double vol, double dt, double rk_alpha, double stress[9], double *pres,
+                       double *sspd)
+{
+    (void)mat_id; (void)den; (void)sie; (void)vel_grad; (void)vol; (void)dt; (void)rk_alpha;
+    (void)stress; (void)pres; (void)sspd;
+}
+
+void orc_get_velgrad(double L[9], const double u[8], const double v[8], const double w[8],
+                     const double B[24], double vol)
+{
+    const double *c[3] = {u, v, w};
+    double inverse_vol = 1.0 / vol;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double acc = c[i][0] * B[0 * 3 + j];
+            for (int p = 1; p < 8; ++p) acc = acc + c[i][p] * B[p * 3 + j];
+            L[i * 3 + j] = acc * inverse_vol;
+        }
+}
+
+static int strength_of(const orc_state *s, int64_t e)
+{
+    return s->mat_strength ? (int)s->mat_strength[s->elem_mat_id[e]] : 0;
+}
+
+/* get_force_sgh, 3D (K/force_sgh.cpp:12-399): tau = stress - p I
+ * (K/force_sgh.cpp:127-137), the stress being zero unless a stress field is
+ * present; a hypo-elastic material's strength model then updates the
+ * element's stress (:369-391).  The curl (:111-123) and the shock direction
+ * (:185-212) do not reach the corner force and are omitted. */
 void orc_get_force_sgh(const orc_mesh *m, orc_state *s, const orc_material *mat, double fuzz,
                        double small, double dt, double rk_alpha)
 {
-    (void)dt;
-    (void)rk_alpha;
     const int64_t N = m->num_nodes;
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < m->num_elems; ++e) {
@@ -205,10 +230,10 @@ void orc_get_force_sgh(const orc_mesh *m, orc_state *s, const orc_material *mat,
         const double sspd = s->elem_sspd[e];
         const double pres = s->elem_pres[e];
 
-        /* tau = stress - p I with stress == 0 */
+        double *stress = s->elem_stress ? s->elem_stress + (m->num_elems + e) * 9 : NULL;
         double tau[3][3];
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) tau[i][j] = 0.0;
+            for (int j = 0; j < 3; ++j) tau[i][j] = stress ? stress[i * 3 + j] : 0.0;
         for (int i = 0; i < 3; ++i) tau[i][i] -= pres;
 
         /* MARS: cell-average velocity (:147-164) */
@@ -264,6 +289,19 @@ void orc_get_force_sgh(const orc_mesh *m, orc_state *s, const orc_material *mat,
                     area[a][0] * tau[0][d] + area[a][1] * tau[1][d] + area[a][2] * tau[2][d] +
                     phi * muc[a] * (vel_star[d] - s->node_vel[(N + n) * 3 + d]);
             }
+        }
+        /* hypo-elastic strength: the model sees the velocity gradient of the
+         * outward B (:92-99) and updates the stress in place */
+        if (stress && strength_of(s, e) == 1) {
+            double u[8], v[8], w[8], L[9];
+            for (int a = 0; a < 8; ++a) {
+                u[a] = vel[a][0];
+                v[a] = vel[a][1];
+                w[a] = vel[a][2];
+            }
+            orc_get_velgrad(L, u, v, w, B, vol);
+            orc_user_strength(s->elem_mat_id[e], den, s->elem_sie[m->num_elems + e], L, vol, dt,
+                              rk_alpha, stress, &s->elem_pres[e], &s->elem_sspd[e]);
         }
     }
 }
@@ -339,10 +377,18 @@ void orc_update_position_sgh(const orc_mesh *m, orc_state *s, double rk_alpha, d
  * K/state.h:140-147): ideal_gas (K/eos.cpp:12-46) or the user model
  * user_eos_model (K/user_mat.cpp:14-40), which with the strength-free stress
  * gives p = -tr(stress)/3 = 0 and c = 1e-8. */
-static void ideal_gas(orc_state *s, int64_t e, double den, double sie)
+static void ideal_gas(orc_state *s, int64_t E, int64_t e, double den, double sie)
 {
     if (s->mat_eos && s->mat_eos[s->elem_mat_id[e]] == 1) {
-        s->elem_pres[e] = 0.0; /* -(stress_00 + stress_11 + stress_22) / 3 with zero stress */
+        /* p = -tr(stress)/3 of the element's current stress (zero without a
+         * stress field) */
+        double p = 0.0;
+        if (s->elem_stress) {
+            const double *st = s->elem_stress + (E + e) * 9;
+            for (int i = 0; i < 3; ++i) p -= st[i * 3 + i];
+        }
+        p *= 1.0 / 3.0;
+        s->elem_pres[e] = p;
         s->elem_sspd[e] = 1.0e-8;
         return;
     }
@@ -353,22 +399,40 @@ static void ideal_gas(orc_state *s, int64_t e, double den, double sie)
     if (s->elem_sspd[e] < csmin) s->elem_sspd[e] = csmin;
 }
 
-/* update_state, 3D, no strength (K/properties.cpp:9-114) */
-void orc_update_state(const orc_mesh *m, orc_state *s)
+/* update_state, 3D (K/properties.cpp:9-114): density, then for a
+ * hyper-elastic material the strength model with the velocity gradient of
+ * the current state (:46-95), then the EOS */
+void orc_update_state(const orc_mesh *m, orc_state *s, double dt, double rk_alpha)
 {
-    const int64_t E = m->num_elems;
+    const int64_t E = m->num_elems, N = m->num_nodes;
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < E; ++e) {
         s->elem_den[e] = s->elem_mass[e] / s->elem_vol[e];
-        ideal_gas(s, e, s->elem_den[e], s->elem_sie[E + e]);
+        if (s->elem_stress && strength_of(s, e) == 2) {
+            double x[8], y[8], z[8], B[24], u[8], v[8], w[8], L[9];
+            elem_xyz(m, s, e, x, y, z);
+            orc_get_bmatrix(B, x, y, z);
+            for (int a = 0; a < 8; ++a) {
+                int64_t n = m->nodes_in_elem[e * 8 + a];
+                u[a] = s->node_vel[(N + n) * 3 + 0];
+                v[a] = s->node_vel[(N + n) * 3 + 1];
+                w[a] = s->node_vel[(N + n) * 3 + 2];
+            }
+            orc_get_velgrad(L, u, v, w, B, s->elem_vol[e]);
+            orc_user_strength(s->elem_mat_id[e], s->elem_den[e], s->elem_sie[E + e], L,
+                              s->elem_vol[e], dt, rk_alpha, s->elem_stress + (E + e) * 9,
+                              &s->elem_pres[e], &s->elem_sspd[e]);
+        }
+        ideal_gas(s, E, e, s->elem_den[e], s->elem_sie[E + e]);
     }
 }
 
-/* rk_init (K/time_integration.cpp:9-44); stress is identically zero here */
+/* rk_init (K/time_integration.cpp:9-44) */
 void orc_rk_init(const orc_mesh *m, orc_state *s)
 {
     const int64_t N = m->num_nodes, E = m->num_elems;
     memcpy(s->elem_sie, s->elem_sie + E, (size_t)E * sizeof(double));
+    if (s->elem_stress) memcpy(s->elem_stress, s->elem_stress + 9 * E, (size_t)(9 * E) * sizeof(double));
     memcpy(s->node_coords, s->node_coords + 3 * N, (size_t)(3 * N) * sizeof(double));
     memcpy(s->node_vel, s->node_vel + 3 * N, (size_t)(3 * N) * sizeof(double));
 }
@@ -452,7 +516,7 @@ void orc_sgh_solve(const orc_mesh *m, orc_state *s, const orc_material *mat,
             orc_update_energy_sgh(m, s, rk_alpha, dt);
             orc_update_position_sgh(m, s, rk_alpha, dt);
             orc_get_vol(m, s);
-            orc_update_state(m, s);
+            orc_update_state(m, s, dt, rk_alpha);
         }
         time_value += dt;
         int write = 0;                                     /* :594-606 */
@@ -789,7 +853,7 @@ void orc_setup_fills(const orc_mesh *m, orc_state *s, int64_t num_fills, const o
             int64_t mid = F->mat_id;
             for (int64_t v = 0; v < mat_num_state_vars[mid]; ++v)
                 s->elem_statev[e * s->num_state_vars + v] = state_vars[mid * s->num_state_vars + v];
-            ideal_gas(s, e, s->elem_den[e], s->elem_sie[E + e]);
+            ideal_gas(s, E, e, s->elem_den[e], s->elem_sie[E + e]);
             for (int a = 0; a < 8; ++a) {
                 int64_t n = m->nodes_in_elem[e * 8 + a];
                 double *v = &s->node_vel[(N + n) * 3];

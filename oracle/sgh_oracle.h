@@ -58,6 +58,8 @@ typedef struct orc_state {
     int64_t num_state_vars;
     double *corner_force;  /* 8E x 3 */
     const int64_t *mat_eos; /* per material: 0 ideal gas, 1 user model; NULL = all ideal gas */
+    double *elem_stress;   /* 2 x E x 3 x 3 Cauchy stress (K/state.h:64), or NULL = zero */
+    const int64_t *mat_strength; /* per material: 0 none, 1 hypo, 2 hyper; NULL = none */
 } orc_state;
 
 /* material_t (K/state.h:130-178), ideal-gas EOS, no strength model */
@@ -114,6 +116,16 @@ typedef struct orc_run_result {
     double seconds;         /* wall time of the cycle loop */
 } orc_run_result;
 
+/* ---- strength model hook (K/user_mat.cpp:47-90) ---- */
+/* user_strength_model restated: the reference ships an empty body. */
+void orc_user_strength(int64_t mat_id, double den, double sie, const double vel_grad[9],
+                       double vol, double dt, double rk_alpha, double stress[9], double *pres,
+                       double *sspd);
+/* get_velgrad (K/momentum.cpp:59-144): L_ij = sum_p u_p^i B_pj / V with the
+ * outward B */
+void orc_get_velgrad(double L[9], const double u[8], const double v[8], const double w[8],
+                     const double B[24], double vol);
+
 /* ---- geometry (K/geometry.cpp) ---- */
 void   orc_get_bmatrix(double B[24], const double x[8], const double y[8], const double z[8]);
 double orc_get_vol_hex(const double x[8], const double y[8], const double z[8]);
@@ -127,7 +139,7 @@ void orc_update_velocity_sgh(const orc_mesh *m, orc_state *s, double rk_alpha, d
 void orc_boundary_velocity(const orc_mesh *m, orc_state *s, const orc_boundary *bc);
 void orc_update_energy_sgh(const orc_mesh *m, orc_state *s, double rk_alpha, double dt);
 void orc_update_position_sgh(const orc_mesh *m, orc_state *s, double rk_alpha, double dt);
-void orc_update_state(const orc_mesh *m, orc_state *s);
+void orc_update_state(const orc_mesh *m, orc_state *s, double dt, double rk_alpha);
 void orc_rk_init(const orc_mesh *m, orc_state *s);
 void orc_get_timestep(const orc_mesh *m, orc_state *s, double time_value, double graphics_time,
                       double time_final, double dt_max, double dt_min, double dt_cfl,

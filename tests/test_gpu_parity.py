@@ -159,10 +159,10 @@ def test_fused_cycles_user_eos_region(oracle, Solver):
 
 
 def test_material_model_errors(Solver):
-    """Strength models are outside this build; bad ids are rejected."""
+    """Unknown EOS / strength ids and materials are rejected."""
     from fierro_amd import _abi
     g = Solver(P.sedov(4, cycles=1))
-    assert g.L.fsgh_set_material_models(g.ctx, 0, _abi.EOS_IDEAL_GAS, 1) == -6
+    assert g.L.fsgh_set_material_models(g.ctx, 0, _abi.EOS_IDEAL_GAS, 9) == -1
     assert g.L.fsgh_set_material_models(g.ctx, 5, _abi.EOS_USER, 0) == -1
     assert g.L.fsgh_set_material_models(g.ctx, 0, 7, 0) == -1
     assert g.L.fsgh_set_material_models(g.ctx, 0, _abi.EOS_USER, 0) == 0

@@ -1,0 +1,101 @@
+"""Pins of the oracle's stress path (the strength-model plumbing): the
+velocity gradient is exact on linear velocity fields (3D and RZ), a uniform
+stress gives the closed-form corner forces and interior equilibrium, the
+user EOS returns -tr(stress)/3, rk_init carries the stress, and a stressed
+run still conserves total energy to round-off (the stress does its work
+through the corner forces that the energy update uses)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fierro_amd import problems as P
+
+RNG = np.random.default_rng(5)
+
+
+def sym(scale=1.0):
+    a = RNG.standard_normal((3, 3)) * scale
+    return 0.5 * (a + a.T)
+
+
+def test_velgrad_exact_3d():
+    spec = P.noh(4, cycles=1)
+    spec.strength = [P.STRENGTH_HYPO]
+    pr = O.OracleProblem(spec)
+    A = RNG.standard_normal((3, 3))
+    b = RNG.standard_normal(3)
+    X = pr.node_coords[1] + 0.05 * RNG.standard_normal(pr.node_coords[1].shape)
+    pr.node_coords[1] = X
+    pr.node_vel[1] = X @ A.T + b
+    pr.get_vol()
+    L = O.velgrad_3d(pr)
+    assert np.allclose(L, A[None], rtol=0, atol=1e-12)
+
+
+def test_velgrad_exact_rz():
+    spec = P.sod_rz(6, 3, cycles=1)
+    pr = O.OracleRZ(spec)
+    a, b, c = 0.7, -0.3, 0.4
+    X = pr.node_coords[1]
+    pr.node_vel[1, :, 0] = a * X[:, 0] + b
+    pr.node_vel[1, :, 1] = c * X[:, 1]
+    pr.get_vol()
+    L = O.velgrad_rz(pr)
+    exp = np.zeros((3, 3))
+    exp[0, 0], exp[1, 1], exp[2, 2] = a, c, c
+    assert np.allclose(L, exp[None], rtol=0, atol=1e-12)
+
+
+def test_uniform_stress_forces_3d():
+    spec = P.anvil(3, 3, 5, cycles=1)
+    spec.strength = [P.STRENGTH_HYPO]
+    pr = O.OracleProblem(spec)
+    sig = sym()
+    pr.elem_stress[1] = sig
+    pr.node_vel[:] = 0.0
+    pr.elem_pres[:] = 0.75
+    pr.elem_div[:] = 0.0
+    pr.get_force_sgh(1e-3, 0.5)
+    tau = sig - 0.75 * np.eye(3)
+    F = pr.corner_force.reshape(pr.E, 8, 3)
+    for e in range(pr.E):
+        nd = pr.conn[e]
+        B = O.bmatrix(pr.node_coords[1, nd, 0], pr.node_coords[1, nd, 1], pr.node_coords[1, nd, 2])
+        assert np.allclose(F[e], -B @ tau, rtol=0, atol=1e-15)
+    cs, ci = pr.cx["corners_in_node_start"], pr.cx["corners_in_node"]
+    Fn = np.array([pr.corner_force[ci[cs[n]:cs[n + 1]]].sum(axis=0) for n in range(pr.N)])
+    inner = [n for n in range(pr.N) if cs[n + 1] - cs[n] == 8]
+    assert inner and np.allclose(Fn[inner], 0.0, atol=1e-15)
+    # the (empty) hypo model leaves the stress as it was
+    assert np.array_equal(pr.elem_stress[1], np.broadcast_to(sig, pr.elem_stress[1].shape))
+
+
+def test_user_eos_reads_stress_and_rk_init_copies():
+    spec = P.sedov_user_region(4, cycles=1)
+    spec.strength = [P.STRENGTH_NONE, P.STRENGTH_HYPO]
+    pr = O.OracleProblem(spec)
+    sig = np.array([[1.0, 0.2, 0.0], [0.2, -2.0, 0.1], [0.0, 0.1, 4.0]])
+    pr.elem_stress[1] = sig
+    pr.update_state(1e-3, 1.0)
+    user = pr.elem_mat_id == 1
+    assert user.any()
+    assert np.all(pr.elem_pres[user] == ((0.0 - 1.0 - (-2.0) - 4.0) * (1.0 / 3.0)))
+    assert np.all(pr.elem_sspd[user] == 1.0e-8)
+    pr.rk_init()
+    assert np.array_equal(pr.elem_stress[0], pr.elem_stress[1])
+
+
+@pytest.mark.parametrize("make", [lambda: P.anvil(3, 3, 10, cycles=40),
+                                  lambda: P.sod_rz(20, 3, cycles=40)])
+def test_stressed_run_conserves_energy(make):
+    spec = make()
+    spec.strength = [P.STRENGTH_HYPO]
+    pr = O.OracleRZ(spec) if spec.dims == 2 else O.OracleProblem(spec)
+    pr.elem_stress[1] = 0.05 * sym()
+    r = pr.solve()
+    assert r["cycles"] == 40
+    assert abs(r["te1"] - r["te0"]) <= 1e-13 * abs(r["te0"])
+    # the stress moved the mesh differently from the strength-free run
+    q = O.OracleRZ(make()) if spec.dims == 2 else O.OracleProblem(make())
+    q.solve()
+    assert not np.allclose(q.node_coords[1], pr.node_coords[1])
