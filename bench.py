@@ -151,16 +151,10 @@ def main():
     ie, ke = g.energy(0)
 
     E, N = g.E, g.N
-    kernels = {}
-    for name, fn in bytemodel.KERNELS.items():
-        ms, calls = prof.get(name, (0.0, 0))
-        avg = ms / max(calls, 1)
-        b = fn(E, N)
-        kernels[name] = {"avg_ms": avg, "calls": calls, "alg_bytes": b,
-                         "GBs": b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0}
+    kernels, lists = kernel_table(g, prof, tuning)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["calls"])
     d = kernels[dom]
-    cyc_bytes = bytemodel.cycle(E, N)
+    cyc_bytes = bytemodel.cycle(E, N, lists)
     value = E * args.steps / wall
     out = {
         "metric": "FP64 SGH zone-cycles/s (Sedov 256^3 blast, plain mesh)",
@@ -259,6 +253,26 @@ def sedov_rz(args):
     print(json.dumps(out), flush=True)
 
 
+def kernel_table(g, prof, tuning):
+    """Per fused 3D kernel: mean launch time, launches, algorithmic bytes and
+    GB/s.  The energy kernel's bytes count the node lists of its element
+    partition (the library reports their size); the velocity kernel's count
+    the staged corner lists (every node block of these hex meshes, at most 8
+    corners per node, fits the kernel's LDS capacity of 9 per node)."""
+    from fierro_amd import bytemodel
+
+    eb = int(tuning.get("energy_state_div", {}).get("block", 256))
+    lists = g.stage_info(eb)[0] or None
+    kernels = {}
+    for name in bytemodel.KERNELS:
+        ms, calls = prof.get(name, (0.0, 0))
+        avg = ms / max(calls, 1)
+        b = bytemodel.kernel_bytes(name, g.E, g.N, lists)
+        kernels[name] = {"avg_ms": avg, "calls": calls, "alg_bytes": b,
+                         "GBs": b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0}
+    return kernels, lists
+
+
 def tune(g, args):
     """Autotune the fused-cycle kernels' launch variants (outside the timed
     region; the state is unchanged) and return the choice per kernel."""
@@ -322,13 +336,7 @@ def ensemble(args):
     cyc = [g.status(i)["cycles"] for i in range(len(specs))]
     assert min(cyc) == max(cyc) == args.warmup + steps, (min(cyc), max(cyc))
     E, N = g.E, g.N
-    kernels = {}
-    for name, fn in bytemodel.KERNELS.items():
-        ms, calls = prof.get(name, (0.0, 0))
-        avg = ms / max(calls, 1)
-        b = fn(E, N)
-        kernels[name] = {"avg_ms": avg, "calls": calls, "alg_bytes": b,
-                         "GBs": b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0}
+    kernels, _ = kernel_table(g, prof, tuning)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["calls"])
     value = E * steps / wall
     del g

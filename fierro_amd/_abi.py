@@ -38,7 +38,7 @@ EXPORTS = [
     "fsgh_forward", "fsgh_dt_history", "fsgh_backward", "fsgh_tangent",
     "fsgh_autotune", "fsgh_set_tuning", "fsgh_get_tuning",
     "fsgh_rz_extensive_mass", "fsgh_rz_update_node_mass",
-    "fsgh_set_strength_params", "fsgh_get_velgrad",
+    "fsgh_set_strength_params", "fsgh_get_velgrad", "fsgh_get_stage_info",
 ]
 
 KERNEL_DIV_EXTREMES, KERNEL_FORCE, KERNEL_VEL_BC_POS, KERNEL_ENERGY_STATE_DIV = 0, 1, 2, 3
@@ -144,6 +144,7 @@ def load() -> C.CDLL:
         "fsgh_get_tuning": [vp, C.POINTER(Tuning), i32],
         "fsgh_rz_extensive_mass": [vp],
         "fsgh_set_strength_params": [vp, i32, i32, vp],
+        "fsgh_get_stage_info": [vp, i32, vp, vp],
         "fsgh_get_velgrad": [vp, vp],
         "fsgh_rz_update_node_mass": [vp, dbl],
     }

@@ -1,8 +1,10 @@
 """Algorithmic HBM bytes of the SGH kernels (unique bytes a launch must move).
 
 Counting rule: every array element a kernel must read or write is counted
-once (FP64 = 8 B, int32 index = 4 B); values recomputed on chip or re-read
-from cache count 0.  The ideal-gas, strength-free path reads no stress; with
+once (FP64 = 8 B, int32 index = 4 B, 16-bit index = 2 B); values recomputed
+on chip or re-read from cache count 0; index arrays count as the kernel
+reads them (the LDS-staged kernels read local indices and per-block lists in
+place of global connectivity).  The ideal-gas, strength-free path reads no stress; with
 one material the material id is not read.  E elements, N nodes, C = 8E.
 """
 from __future__ import annotations
@@ -19,14 +21,21 @@ def force(E, N):
     return 32 * E + 48 * N + 40 * E + 16 * N + 192 * E
 
 
-def vel_bc_pos(E, N):
-    # F 24C + corner CSR + m 8N + v0 24N + v1 24N (w) + x0 24N + x1 24N (w) + BC mask 1N
-    return 192 * E + (32 * E + 4 * (N + 1)) + 8 * N + 96 * N + N
+def vel_bc_pos(E, N, corner_staged=True):
+    # F 24C + corner CSR (starts 4(N+1) + the blocks' sorted corner lists 4C)
+    # + the entries' positions in them 2C when the block's corner forces are
+    # staged in LDS + m 8N + v0 24N + v1 24N (w) + x0 24N + x1 24N (w) + BC mask 1N
+    return (192 * E + (32 * E + 4 * (N + 1)) + (16 * E if corner_staged else 0) + 8 * N
+            + 96 * N + N)
 
 
-def energy_state_div(E, N):
-    # F 24C + conn 32E + v1,v0,x1 72N + M, e0 16E + e1,V,rho,p,c,div 48E (w) + gamma,csmin 16E
-    return 192 * E + 32 * E + 72 * N + 16 * E + 48 * E + 16 * E
+def energy_state_div(E, N, list_entries=None):
+    # F 24C + connectivity + v1,v0,x1 72N + M, e0 16E + e1,V,rho,p,c,div 48E (w)
+    # + gamma,csmin 16E.  Connectivity: with the blocks' nodes staged in LDS,
+    # 16-bit local connectivity 16E + the blocks' node lists 4 x list_entries;
+    # otherwise conn 32E.
+    conn = 32 * E if list_entries is None else 16 * E + 4 * list_entries
+    return 192 * E + conn + 72 * N + 16 * E + 48 * E + 16 * E
 
 
 KERNELS = {"div_extremes": div_extremes, "force": force, "vel_bc_pos": vel_bc_pos,
@@ -60,7 +69,17 @@ RZ_KERNELS = {"rz_div_extremes": rz_div_extremes, "rz_force": rz_force, "rz_node
               "rz_energy_state_div": rz_energy_state_div}
 
 
-def cycle(E, N):
+def cycle(E, N, list_entries=None, corner_staged=True):
     """Bytes of one fused RK2 cycle (2 stages; dt and control kernels are O(1)
     per block)."""
-    return 2 * (div_extremes(E, N) + force(E, N) + vel_bc_pos(E, N) + energy_state_div(E, N))
+    return 2 * (div_extremes(E, N) + force(E, N) + vel_bc_pos(E, N, corner_staged)
+                + energy_state_div(E, N, list_entries))
+
+
+def kernel_bytes(name, E, N, list_entries=None, corner_staged=True):
+    """Algorithmic bytes of one launch of a fused 3D kernel."""
+    if name == "vel_bc_pos":
+        return vel_bc_pos(E, N, corner_staged)
+    if name == "energy_state_div":
+        return energy_state_div(E, N, list_entries)
+    return KERNELS[name](E, N)

@@ -65,6 +65,8 @@ DMesh fsgh::make_mesh(fsgh_ctx* c)
     m.minw = 1;
     m.cshift = c->dims == 2 ? 2 : 3;
     m.pad_ = 0;
+    attach_node_stage(c, m, kBlock);
+    attach_corner_stage(c, m, kBlock);
     return m;
 }
 
@@ -383,6 +385,8 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
     HIPCHK(c, hipMemcpyAsync(c->d_eblk, etab.data(), etab.size() * sizeof(BlockItem), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_nblk, ntab.data(), ntab.size() * sizeof(BlockItem), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = build_node_stage(c, kBlock)) != FSGH_OK) return rc;
+    if ((rc = build_corner_stage(c, kBlock)) != FSGH_OK) return rc;
     // coordinates into both bins (K/read_mesh.cpp:129-137)
     for (int b = 0; b < 2; ++b)
         for (int d = 0; d < 3; ++d)
@@ -409,6 +413,7 @@ int fsgh_destroy(fsgh_ctx* c)
         if (p) hipFree(p);
     for (auto& kv : c->tables)
         if (kv.second.d) hipFree(kv.second.d);
+    free_node_stages(c);
     for (double* p : {c->d_stress[0], c->d_stress[1], c->d_sparams})
         if (p) hipFree(p);
     for (int b = 0; b < 2; ++b)
@@ -441,6 +446,17 @@ int fsgh_get_device_csr(fsgh_ctx* c, int32_t* start, int32_t* corners)
     HIPCHK(c, hipMemcpyAsync(corners, c->d_cin, static_cast<size_t>(c->npe * c->E) * 4,
                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FSGH_OK;
+}
+
+int fsgh_get_stage_info(fsgh_ctx* c, int32_t elem_block, int64_t* list_entries,
+                        int32_t* max_block_nodes)
+{
+    CHECK_CTX(c);
+    if (!list_entries || !max_block_nodes) return FSGH_E_INVALID;
+    auto it = c->stages.find(elem_block);
+    *list_entries = it == c->stages.end() ? 0 : it->second.total;
+    *max_block_nodes = it == c->stages.end() ? 0 : it->second.max_nodes;
     return FSGH_OK;
 }
 

@@ -63,6 +63,23 @@ struct fsgh_ctx {
         std::vector<int32_t> inst_begin;
     };
     std::map<int32_t, WorkTable> tables;
+    // per element block size: the blocks' distinct-node lists that the
+    // LDS-staged element kernels read (fsgh_stage.hip)
+    struct NodeStage {
+        int32_t* start = nullptr;
+        int32_t* nodes = nullptr;
+        uint16_t* lconn = nullptr;
+        int64_t total = 0;
+        int32_t max_nodes = 0;
+    };
+    std::map<int32_t, NodeStage> stages;
+    // per node block size: the blocks' sorted corner lists that the
+    // corner-staged velocity kernel reads (fsgh_stage.hip)
+    struct CornerStage {
+        int32_t* scor = nullptr;
+        uint16_t* spos = nullptr;
+    };
+    std::map<int32_t, CornerStage> cstages;
     double* d_scalar = nullptr;             // 2 doubles
     double* d_scratch = nullptr;            // energy partial sums
     int32_t nscratch = 1024;
@@ -114,6 +131,16 @@ DMesh dt_mesh(fsgh_ctx* c);
 int resync_dt_ranges(fsgh_ctx* c);
 // build (once) the work table of a configuration; FSGH_OK or an error code
 int ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd);
+// per-block node lists of the element partition with `bs` elements per block
+// (3D device contexts; no-op otherwise), and the mesh view's pointers to the
+// lists of the partition its element table `m.eblk` has (ebs per block)
+int build_node_stage(fsgh_ctx* c, int32_t bs);
+void attach_node_stage(fsgh_ctx* c, DMesh& m, int32_t ebs);
+// sorted corner lists of the node partition with `bs` nodes per block, and
+// the mesh view's pointers to those of its node table's partition (nbs)
+int build_corner_stage(fsgh_ctx* c, int32_t bs);
+void attach_corner_stage(fsgh_ctx* c, DMesh& m, int32_t nbs);
+void free_node_stages(fsgh_ctx* c);
 void issue_cycle(fsgh_ctx* c, int old, const double* dt_replay, double* dt_record);
 // 2D RZ contexts (fsgh_rz.hip)
 int rz_build_maps(fsgh_ctx* c);

@@ -61,7 +61,23 @@ struct DMesh {
     int32_t minw;              // occupancy floor (waves per SIMD) of the kernel variant to run
     int32_t cshift;            // log2(corners per element): 3 for hexes, 2 for RZ quads
     int32_t pad_;
+    // per logical element block of this partition (bs elements): its distinct
+    // nodes (sn_nodes[sn_start[b] .. sn_start[b+1]), ascending) and each
+    // corner's position in that list (sn_lconn, 8 per element); null when not
+    // built (2D, or a partition without lists)
+    const int32_t* sn_start;
+    const int32_t* sn_nodes;
+    const uint16_t* sn_lconn;
+    // per logical node block of this partition: its corner-list entries'
+    // corners sorted ascending (cs_cor) and each entry's position among them
+    // (cs_pos), both indexed like cin; null when not built
+    const int32_t* cs_cor;
+    const uint16_t* cs_pos;
 };
+
+// LDS capacity (corner-list entries) of a node block of bs nodes in the
+// corner-staged velocity kernel; larger blocks gather from global memory
+constexpr int corner_stage_cap(int bs) { return bs * 9; }
 
 // Launch configuration of one fused-cycle kernel, chosen by fsgh_autotune.
 struct KernelCfg {

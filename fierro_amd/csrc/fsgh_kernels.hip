@@ -40,6 +40,99 @@ __device__ __forceinline__ void gather3(const double* __restrict__ a, const doub
     }
 }
 
+// LDS staging of a block's nodes (lists built in fsgh_stage.hip): capacity
+// in nodes per element block of BS elements (a Morton brick of BS elements
+// has (a+1)(b+1)(c+1) nodes, 405 for 256; irregular blocks above the
+// capacity read from global memory instead)
+template <int BS>
+constexpr int stage_cap()
+{
+    return BS <= 64 ? 192 : BS <= 128 ? 320 : BS <= 256 ? 640 : 1024;
+}
+
+// the block's node range in its list, or nn = cap + 1 when the mesh view has
+// no lists (block-uniform)
+__device__ __forceinline__ void stage_range(const DMesh& m, const BlockItem& bi, int cap, int& q0,
+                                            int& nn)
+{
+    q0 = 0;
+    nn = cap + 1;
+    if (m.sn_start != nullptr) {
+        q0 = m.sn_start[bi.lid];
+        nn = m.sn_start[bi.lid + 1] - q0;
+    }
+}
+
+__device__ __forceinline__ void load_lconn(const uint16_t* lconn, int e, int32_t lc[8])
+{
+    const uint4 p = *reinterpret_cast<const uint4*>(lconn + 8 * static_cast<size_t>(e));
+    lc[0] = static_cast<int32_t>(p.x & 0xffffu);
+    lc[1] = static_cast<int32_t>(p.x >> 16);
+    lc[2] = static_cast<int32_t>(p.y & 0xffffu);
+    lc[3] = static_cast<int32_t>(p.y >> 16);
+    lc[4] = static_cast<int32_t>(p.z & 0xffffu);
+    lc[5] = static_cast<int32_t>(p.z >> 16);
+    lc[6] = static_cast<int32_t>(p.w & 0xffffu);
+    lc[7] = static_cast<int32_t>(p.w >> 16);
+}
+
+// The 24 corner forces of this lane's element, read cooperatively by the
+// wave: the wave's elements own one contiguous stretch of F, which the lanes
+// load as consecutive 16-byte pieces (coalesced) in two halves of 32
+// elements and pass through a padded LDS region (13 x 16 B per element) to
+// the lane of each element.  buf: 32 x 13 double2 of this wave; e0: the
+// wave's first element; nvalid: elements of the wave inside the work item.
+__device__ __forceinline__ void wave_load_forces(const double* __restrict__ F, double2* buf, int e0,
+                                                 int nvalid, double f[24])
+{
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const double2* src = reinterpret_cast<const double2*>(F + 24 * static_cast<size_t>(e0));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        double2 t[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const int idx = 64 * r + lane;            // piece within the half
+            const int j = idx / 12;
+            t[r] = (32 * h + j < nvalid) ? src[384 * h + idx] : make_double2(0.0, 0.0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const int idx = 64 * r + lane;
+            buf[(idx / 12) * 13 + idx % 12] = t[r];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if ((lane >> 5) == h) {
+            const double2* mine = buf + (lane & 31) * 13;
+#pragma unroll
+            for (int i = 0; i < 12; ++i) {
+                const double2 v = mine[i];
+                f[2 * i] = v.x;
+                f[2 * i + 1] = v.y;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+__device__ __forceinline__ void lds_gather3(const double* __restrict__ a, const double* __restrict__ b,
+                                            const double* __restrict__ c, const int32_t lc[8],
+                                            double x[8], double y[8], double z[8])
+{
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        x[i] = a[lc[i]];
+        y[i] = b[lc[i]];
+        z[i] = c[lc[i]];
+    }
+}
+
 __device__ __forceinline__ double wave_min(double v)
 {
 #pragma unroll
@@ -126,12 +219,15 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
     }
     const int e = bi.start + static_cast<int>(threadIdx.x);
     // the block's corner forces are staged in LDS (row stride 13 x 16 B, free
-    // of bank conflicts) and written out as contiguous 16-byte stores
-    __shared__ double2 sF[BS * 13];
+    // of bank conflicts) and written out as contiguous 16-byte stores.  (The
+    // node reads stay global gathers here: this kernel is bound by latency at
+    // its register-limited occupancy, not by the address units, and a staging
+    // phase with its barrier measured slower.)
+    __shared__ double2 smem[BS * 13];
     if (e < bi.end) {
         int32_t nd[8];
-        load_conn(m.conn, e, nd);
         double x[8], y[8], z[8], u[8], v[8], w[8];
+        load_conn(m.conn, e, nd);
         gather3(s.x1, s.y1, s.z1, nd, x, y, z);
         gather3(s.u1, s.v1, s.w1, nd, u, v, w);
         const double vol = s.vol[e];
@@ -163,7 +259,7 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
                    mt.q2ex, fuzz, small, f);
 #pragma unroll
         for (int i = 0; i < 12; ++i)
-            sF[threadIdx.x * 13 + i] = make_double2(f[2 * i], f[2 * i + 1]);
+            smem[threadIdx.x * 13 + i] = make_double2(f[2 * i], f[2 * i + 1]);
     }
     __syncthreads();
     const int cnt = 12 * (bi.end - bi.start);
@@ -171,7 +267,7 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
 #pragma unroll
     for (int r = 0; r < 12; ++r) {
         const int idx = r * BS + static_cast<int>(threadIdx.x);
-        if (idx < cnt) out[idx] = sF[(idx / 12) * 13 + idx % 12];
+        if (idx < cnt) out[idx] = smem[(idx / 12) * 13 + idx % 12];
     }
 }
 
@@ -237,6 +333,36 @@ __device__ __forceinline__ void gather_node_force(const DMesh& m, const double* 
         f0 += F[k];
         f1 += F[k + 1];
         f2 += F[k + 2];
+    }
+}
+
+// gather_node_force from the block's LDS copy of its corner forces (the
+// entries' positions in the block's sorted corner list), same order and sums
+__device__ __forceinline__ void lds_node_force(const DMesh& m, const double* sF, int n, double& f0,
+                                               double& f1, double& f2)
+{
+    f0 = 0.0;
+    f1 = 0.0;
+    f2 = 0.0;
+    const int q0 = m.cin_start[n], cnt = m.cin_start[n + 1] - q0;
+    if (cnt <= 0) return;
+    const int last = cnt - 1;
+    int p[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) p[j] = 3 * m.cs_pos[q0 + (j < cnt ? j : last)];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const bool ok = j < cnt;
+        const double g0 = sF[p[j]], g1 = sF[p[j] + 1], g2 = sF[p[j] + 2];
+        f0 = ok ? f0 + g0 : f0;
+        f1 = ok ? f1 + g1 : f1;
+        f2 = ok ? f2 + g2 : f2;
+    }
+    for (int q = q0 + 8; q < q0 + cnt; ++q) {
+        const int k = 3 * m.cs_pos[q];
+        f0 += sF[k];
+        f1 += sF[k + 1];
+        f2 += sF[k + 2];
     }
 }
 
@@ -494,10 +620,48 @@ __global__ __launch_bounds__(BS) void k_vel_bc_pos(DMesh m, DState s, const Inst
     const InstCtl& c = ctl[bi.inst];
     if (c.stopped) return;
     const int n = bi.start + static_cast<int>(threadIdx.x);
+    // The block's corner forces in LDS, loaded in ascending corner order (the
+    // block's sorted corner list): consecutive lanes read consecutive doubles
+    // of few elements' force rows.  Blocks over the capacity gather from
+    // global memory.
+    constexpr int CAPC = corner_stage_cap(BS);
+    __shared__ double sF[3 * CAPC];
+    const int q0 = m.cin_start[bi.start];
+    const int nc = m.cin_start[bi.end] - q0;
+    const bool staged = m.cs_cor != nullptr && nc <= CAPC;
+    if (staged) {
+        // CAPC / BS corners per lane, every load issued before any store
+        constexpr int R = CAPC / BS;
+        int cid[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k = r * BS + static_cast<int>(threadIdx.x);
+            cid[r] = k < nc ? m.cs_cor[q0 + k] : -1;
+        }
+        double g[R][3];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double* src = s.F + 3 * static_cast<size_t>(cid[r] < 0 ? 0 : cid[r]);
+            g[r][0] = src[0];
+            g[r][1] = src[1];
+            g[r][2] = src[2];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k = r * BS + static_cast<int>(threadIdx.x);
+            if (cid[r] >= 0) {
+                sF[3 * k] = g[r][0];
+                sF[3 * k + 1] = g[r][1];
+                sF[3 * k + 2] = g[r][2];
+            }
+        }
+        __syncthreads();
+    }
     if (n >= bi.end) return;
     const double dt = c.dt;
     double f[3], v0[3], x0[3], v1[3], x1[3];
-    gather_node_force(m, s.F, n, f[0], f[1], f[2]);
+    if (staged) lds_node_force(m, sF, n, f[0], f[1], f[2]);
+    else gather_node_force(m, s.F, n, f[0], f[1], f[2]);
     v0[0] = s.u0[n];
     v0[1] = s.v0[n];
     v0[2] = s.w0[n];
@@ -526,19 +690,54 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
     const InstCtl& c = ctl[bi.inst];
     if (c.stopped) return;
     const int e = bi.start + static_cast<int>(threadIdx.x);
+    // the block's distinct nodes in LDS: u1, v1, w1, u0, v0, w0, x1, y1, z1;
+    // per wave a region for its corner forces (wave_load_forces)
+    constexpr int CAP = stage_cap<BS>();
+    __shared__ double sd[9 * CAP];
+    __shared__ double2 sfw[(BS / 64) * 32 * 13];
+    int q0, nn;
+    stage_range(m, bi, CAP, q0, nn);
+    const bool staged = nn <= CAP;
+    if (staged) {
+        for (int i = static_cast<int>(threadIdx.x); i < nn; i += BS) {
+            const int n = m.sn_nodes[q0 + i];
+            sd[0 * CAP + i] = s.u1[n];
+            sd[1 * CAP + i] = s.v1[n];
+            sd[2 * CAP + i] = s.w1[n];
+            sd[3 * CAP + i] = s.u0[n];
+            sd[4 * CAP + i] = s.v0[n];
+            sd[5 * CAP + i] = s.w0[n];
+            sd[6 * CAP + i] = s.x1[n];
+            sd[7 * CAP + i] = s.y1[n];
+            sd[8 * CAP + i] = s.z1[n];
+        }
+        __syncthreads();
+    }
     double dtl = DBL_MAX;
+    double f[24];
+    {
+        const int w0e = bi.start + (static_cast<int>(threadIdx.x) & ~63);
+        if (w0e < bi.end)   // wave-uniform
+            wave_load_forces(s.F, sfw + (threadIdx.x >> 6) * 32 * 13, w0e, bi.end - w0e, f);
+    }
     if (e < bi.end) {
         int32_t nd[8];
-        load_conn(m.conn, e, nd);
-        double f[24], x[8], y[8], z[8], u1[8], v1[8], w1[8], u0[8], v0[8], w0[8];
-        load_forces(s, e, f);
-        gather3(s.u1, s.v1, s.w1, nd, u1, v1, w1);
-        gather3(s.u0, s.v0, s.w0, nd, u0, v0, w0);
+        double x[8], y[8], z[8], u1[8], v1[8], w1[8], u0[8], v0[8], w0[8];
+        if (staged) {
+            load_lconn(m.sn_lconn, e, nd);
+            lds_gather3(sd, sd + CAP, sd + 2 * CAP, nd, u1, v1, w1);
+            lds_gather3(sd + 3 * CAP, sd + 4 * CAP, sd + 5 * CAP, nd, u0, v0, w0);
+        } else {
+            load_conn(m.conn, e, nd);
+            gather3(s.u1, s.v1, s.w1, nd, u1, v1, w1);
+            gather3(s.u0, s.v0, s.w0, nd, u0, v0, w0);
+        }
         const double emass = s.emass[e];
         const double sie =
             energy_update(f, u1, v1, w1, u0, v0, w0, s.sie0[e], emass, alpha, c.dt);
         s.sie1[e] = sie;
-        gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+        if (staged) lds_gather3(sd + 6 * CAP, sd + 7 * CAP, sd + 8 * CAP, nd, x, y, z);
+        else gather3(s.x1, s.y1, s.z1, nd, x, y, z);
         double vol, den, p, cs, div;
         const int32_t kind = USER ? s.mats[s.nmat > 1 ? s.mat[e] : 0].eos : kEosIdealGas;
         derived_state(x, y, z, u1, v1, w1, emass, sie, s.gamma[e], s.csmin[e], kind, vol, den, p, cs,

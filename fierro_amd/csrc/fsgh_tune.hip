@@ -44,6 +44,10 @@ DMesh mesh_for(fsgh_ctx* c, int k, const KernelCfg& g)
     DMesh m = make_mesh(c);
     m.bs = g.bs;
     m.minw = g.minw;
+    // the element table below is the kernel's own partition for an element
+    // kernel and the default one otherwise
+    attach_node_stage(c, m, elem_kernel(k) ? g.bs : kBlock);
+    attach_corner_stage(c, m, elem_kernel(k) ? kBlock : g.bs);
     if (!default_table(g)) {
         const fsgh_ctx::WorkTable& w = c->tables.at(table_key(elem_kernel(k), g.bs, g.xcd));
         if (elem_kernel(k)) {
@@ -187,7 +191,7 @@ int fsgh::ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd)
                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->tables[key] = std::move(w);
-    return FSGH_OK;
+    return elem ? build_node_stage(c, bs) : build_corner_stage(c, bs);
 }
 
 DMesh fsgh::tuned_mesh(fsgh_ctx* c, int kernel) { return mesh_for(c, kernel, c->tune[kernel]); }

@@ -178,6 +178,14 @@ class SGHSolver:
         check(self.L.fsgh_get_renumbering(self.ctx, _p(e), _p(n)), self.ctx, "renumbering")
         return e, n
 
+    def stage_info(self, elem_block=256):
+        """(list entries, largest block) of the element kernels' per-block
+        node lists for blocks of elem_block elements (zeros if not built)."""
+        n, mx = C.c_int64(), C.c_int32()
+        check(self.L.fsgh_get_stage_info(self.ctx, int(elem_block), C.byref(n), C.byref(mx)),
+              self.ctx, "stage_info")
+        return n.value, mx.value
+
     def device_csr(self):
         """The device's corner CSR in internal numbering (diagnostic)."""
         start = np.zeros(self.N + 1, np.int32)

@@ -6,15 +6,16 @@ without re-timing; ``SGHSolver.autotune()`` measures again on the caller's
 own mesh.  Every configuration gives bit-identical results.
 
 Measured (mean per launch, default = 256 threads, no floor, XCD dealing):
-div_extremes 0.326 ms with 512 threads in launch order (default 0.338 ms);
-vel_bc_pos 1.482 ms in launch order (default 1.522 ms); force and
-energy_state_div fastest at the default (the 3-wave variants spill and run
-1.7x slower).
+div_extremes 0.327 ms with 512 threads in launch order (default 0.339 ms);
+vel_bc_pos 1.119 ms with 128 threads in launch order (default 1.296 ms: its
+LDS copy of the block's corner forces limits blocks per CU); force and
+energy_state_div within noise of the default (their 3-wave variants cannot
+reach 3 waves without spilling).
 """
 
 TABLE = {
     "div_extremes": {"block": 512, "min_waves": 1, "xcd_order": 0},
     "force": {"block": 256, "min_waves": 1, "xcd_order": 1},
-    "vel_bc_pos": {"block": 256, "min_waves": 1, "xcd_order": 0},
+    "vel_bc_pos": {"block": 128, "min_waves": 1, "xcd_order": 0},
     "energy_state_div": {"block": 256, "min_waves": 1, "xcd_order": 1},
 }

@@ -98,6 +98,13 @@ int fsgh_get_renumbering(fsgh_ctx *ctx, int64_t *new_of_old_elem, int64_t *new_o
  * corner ids 8 * internal element + local node [8E]); each node's list is in
  * ascending caller element order.  Device contexts only. */
 int fsgh_get_device_csr(fsgh_ctx *ctx, int32_t *start, int32_t *corners);
+/* Diagnostic (no counterpart in the reference): the per-block node lists of
+ * the LDS-staged element kernels for element blocks of `elem_block`
+ * elements (64, 128, 256 or 512): total list entries (sum over blocks of
+ * their distinct nodes) and the largest block's count; zeros when not built
+ * (2D contexts, or a block size no kernel configuration uses yet). */
+int fsgh_get_stage_info(fsgh_ctx *ctx, int32_t elem_block, int64_t *list_entries,
+                        int32_t *max_block_nodes);
 
 /* ------------------------------------------------------------ materials --- */
 typedef struct fsgh_material {

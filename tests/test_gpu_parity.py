@@ -53,6 +53,9 @@ CASES = {
     "sedov12": lambda: P.sedov(12),
     "sod40": lambda: P.sod_tube(40, 0.125, 0.1, 1.4),
     "sedov8_user": lambda: P.sedov_user_region(8),
+    # a long row: 256-element blocks touch > 640 nodes, so the element kernels
+    # read their nodes from global memory instead of LDS
+    "sod400x2x1": lambda: P.sod_tube(400, 0.125, 0.1, 1.4, ny=2, nz=1),
 }
 
 
@@ -183,6 +186,79 @@ def test_fused_cycles_moving_start(oracle, Solver, case):
     den = g.download("elem_den")
     l1 = float(np.sum(np.abs(den - pr.elem_den)) / np.sum(np.abs(pr.elem_den)))
     assert l1 <= 1e-9, l1
+
+
+def block_node_counts(g, bs=256):
+    """Distinct nodes of each logical element block (bs consecutive internal
+    elements per instance), from the library's renumbering."""
+    eo, no = g.renumbering()
+    iconn = np.empty_like(g.conn)
+    iconn[eo] = no[g.conn]
+    out = []
+    for i in range(len(g.elem_off) - 1):
+        for s in range(int(g.elem_off[i]), int(g.elem_off[i + 1]), bs):
+            e = min(s + bs, int(g.elem_off[i + 1]))
+            out.append(len(np.unique(iconn[s:e])))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("case", ["staged", "global"])
+def test_fused_cycles_node_staging(oracle, Solver, case):
+    """The element kernels stage a block's nodes in LDS when they fit (640
+    for 256-element blocks) and gather from global memory otherwise; both
+    paths match the oracle."""
+    if case == "staged":
+        spec, renumber = P.sedov(12, cycles=80), True
+    else:
+        spec, renumber = P.sod_tube(400, 0.125, 0.1, 1.4, ny=2, nz=1, cycles=80), False
+    g = Solver(spec, renumber=renumber)
+    cnt = block_node_counts(g)
+    # staged: every block fits; global: the full 256-element rows do not
+    assert (cnt.max() <= 640) if case == "staged" else (np.sum(cnt > 640) >= 3)
+    pr = oracle.OracleProblem(spec)
+    ro = pr.solve()
+    rg = g.sgh_solve(use_graph=True)[0]
+    assert rg["cycles"] == ro["cycles"] == 80
+    assert abs(rg["te1"] - ro["te1"]) <= 1e-12 * abs(ro["te1"])
+    for k in STATE:
+        assert rel(g.download(k), oracle_fields(pr)[k]) <= 1e-9, k
+
+
+def write_geo_hex(path, coords, conn):
+    with open(path, "w") as f:
+        f.write("This is the 1st description line of the EnSight Gold geometry file\n")
+        f.write("This is the 2nd description line\nnode id assign\nelement id assign\n")
+        f.write("part\n         1\nMesh\ncoordinates\n")
+        f.write("%10d\n" % coords.shape[0])
+        for d in range(3):
+            for v in coords[:, d]:
+                f.write("%.17e\n" % v)
+        f.write("hexa8\n%10d\n" % conn.shape[0])
+        for row in conn + 1:
+            f.write("".join("%10d" % v for v in row) + "\n")
+
+
+def test_high_valence_nodes_gather_from_global(oracle, Solver, tmp_path):
+    """Every element twice: interior nodes have 16 corners, more than the
+    velocity kernel's LDS copy of a node block holds, so it sums the corner
+    forces through the corner list in global memory; against the oracle."""
+    import dataclasses
+    base = P.sedov(6, cycles=40)
+    coords, conn = base.mesh_arrays()
+    path = str(tmp_path / "dup.geo")
+    write_geo_hex(path, coords, np.repeat(conn, 2, axis=0))
+    spec = dataclasses.replace(base, mesh=dict(kind="geo", path=path))
+    g = Solver(spec, renumber=True)
+    start, _ = g.device_csr()
+    nc = [int(start[min(b + 256, g.N)] - start[b]) for b in range(0, g.N, 256)]
+    assert max(nc) > 9 * 256
+    pr = oracle.OracleProblem(spec)
+    ro = pr.solve()
+    rg = g.sgh_solve(use_graph=True)[0]
+    assert rg["cycles"] == ro["cycles"] == 40
+    assert abs(rg["te1"] - ro["te1"]) <= 1e-12 * abs(ro["te1"])
+    for k in STATE:
+        assert rel(g.download(k), oracle_fields(pr)[k]) <= 1e-9, k
 
 
 def test_batched_instances_equal_plain_runs(Solver):
