@@ -1265,7 +1265,11 @@ void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, cons
                    const SetPtr& in, int assign, const ExtPtr* ext, const double* dts)
 {
     AdjointData& D = *c->adj;
-    DMesh m = make_mesh(c);
+    // 64 elements / nodes per work item: at up to 512 registers per lane these
+    // kernels run one wave per SIMD, and small work items spread a mesh of a
+    // few hundred blocks over every CU
+    const DMesh m = small_block_mesh(c);
+    const dim3 tb(static_cast<unsigned>(m.bs));
     AdjView A = D.A;
     for (int d = 0; d < 3; ++d) {
         A.xo[d] = out.x[d];
@@ -1281,24 +1285,24 @@ void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, cons
         A.alo = ext->alo;
         A.ahi = ext->ahi;
     }
-    hipLaunchKernelGGL(k_adj_energy, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_out, F, A, dts,
+    hipLaunchKernelGGL(k_adj_energy, dim3(m.n_eblk), tb, 0, c->stream, m, s_out, F, A, dts,
                        alpha, assign);
-    hipLaunchKernelGGL(k_adj_node, dim3(m.n_nblk), dim3(kB), 0, c->stream, m, s_out, F, A, dts,
+    hipLaunchKernelGGL(k_adj_node, dim3(m.n_nblk), tb, 0, c->stream, m, s_out, F, A, dts,
                        alpha, assign);
     if (!ext)
-        hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
+        hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), tb, 0, c->stream, m,
                            static_cast<const double*>(s_in.div),
                            static_cast<const double*>(nullptr), A.lo, A.hi, A.alo, A.ahi,
                            static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
-    hipLaunchKernelGGL(k_adj_force, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_in, A,
+    hipLaunchKernelGGL(k_adj_force, dim3(m.n_eblk), tb, 0, c->stream, m, s_in, A,
                        static_cast<const InstCtl*>(c->d_ctl), static_cast<const double*>(s_in.vol),
                        static_cast<const double*>(s_in.div));
-    hipLaunchKernelGGL(k_adj_ext_node, dim3(m.n_nblk), dim3(kB), 0, c->stream, m, A);
-    hipLaunchKernelGGL(k_adj_derived, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, s_in, A,
+    hipLaunchKernelGGL(k_adj_ext_node, dim3(m.n_nblk), tb, 0, c->stream, m, A);
+    hipLaunchKernelGGL(k_adj_derived, dim3(m.n_eblk), tb, 0, c->stream, m, s_in, A,
                        static_cast<const double*>(s_in.vol), static_cast<const double*>(s_in.div),
                        static_cast<const double*>(s_in.sie1), in.e,
                        (cyc == 0 && stage == 0) ? 1 : 0, assign);
-    hipLaunchKernelGGL(k_adj_gather, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
+    hipLaunchKernelGGL(k_adj_gather, dim3(m.n_nblk), tb, 0, c->stream, m,
                        static_cast<const double*>(A.cb), in.x[0], in.x[1], in.x[2], in.v[0],
                        in.v[1], in.v[2], assign);
 }
@@ -1366,7 +1370,8 @@ int ensure_reverse_buffers(fsgh_ctx* c)
     const bool two = D.ncyc > D.every && hipMemGetInfo(&mfree, &mtotal) == hipSuccess &&
                      2 * bank_bytes < mfree / 4;
     D.nbank = two ? 2 : 1;
-    int rc = ensure_extreme_buffers(c);
+    int rc = small_block_tables(c);
+    if (!rc) rc = ensure_extreme_buffers(c);
     for (int b = 0; b < D.nbank && !rc; ++b) rc |= alloc_bank(c, D.bank[b], K, two);
     if (!rc && two) {
         if (hipStreamCreateWithFlags(&D.rs, hipStreamNonBlocking) != hipSuccess) rc = FSGH_E_HIP;

@@ -129,8 +129,14 @@ DMesh dt_mesh(fsgh_ctx* c);
 // point a running loop's dt block ranges at dt_block_begin and recompute the
 // partial minima from the current state
 int resync_dt_ranges(fsgh_ctx* c);
-// build (once) the work table of a configuration; FSGH_OK or an error code
-int ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd);
+// build (once) the work table of a configuration (and, for an element table
+// with `lists`, its per-block node lists); FSGH_OK or an error code
+int ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd, bool lists = true);
+// element and node tables of 64 entries per work item (XCD-dealt), and the
+// mesh view over them: the reverse-pass kernels, which hold up to 512
+// registers per lane, spread a small mesh over every CU this way
+int small_block_tables(fsgh_ctx* c);
+DMesh small_block_mesh(fsgh_ctx* c);
 // per-block node lists of the element partition with `bs` elements per block
 // (3D device contexts; no-op otherwise), and the mesh view's pointers to the
 // lists of the partition its element table `m.eblk` has (ebs per block)

@@ -71,9 +71,18 @@ int resync_dt(fsgh_ctx* c)
 }
 
 
-int apply(fsgh_ctx* c, int k, const KernelCfg& g)
+// the work table of a configuration and the per-block lists its kernel
+// reads (node lists: element kernels, with the table; sorted corner lists:
+// the velocity kernel only)
+int ensure_kernel_data(fsgh_ctx* c, int k, const KernelCfg& g)
 {
     if (int rc = ensure_table(c, elem_kernel(k), g.bs, g.xcd)) return rc;
+    return k == kTunedVelBcPos ? build_corner_stage(c, g.bs) : FSGH_OK;
+}
+
+int apply(fsgh_ctx* c, int k, const KernelCfg& g)
+{
+    if (int rc = ensure_kernel_data(c, k, g)) return rc;
     const int32_t old_bs = c->tune[kTunedEnergy].bs;
     c->tune[k] = g;
     drop_cycle_graphs(c);
@@ -157,11 +166,11 @@ void to_public(int k, const KernelCfg& g, double ms, double ms_def, fsgh_tuning*
 
 }  // namespace
 
-int fsgh::ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd)
+int fsgh::ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd, bool lists)
 {
     if (bs == kBlock && xcd == 1) return FSGH_OK;
     const int32_t key = table_key(elem, bs, xcd);
-    if (c->tables.count(key)) return FSGH_OK;
+    if (c->tables.count(key)) return (elem && lists) ? build_node_stage(c, bs) : FSGH_OK;
     std::vector<BlockItem> tab;
     fsgh_ctx::WorkTable w;
     block_table(elem ? c->L.elem_off : c->L.node_off, tab, w.inst_begin, bs, xcd != 0);
@@ -191,10 +200,31 @@ int fsgh::ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd)
                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->tables[key] = std::move(w);
-    return elem ? build_node_stage(c, bs) : build_corner_stage(c, bs);
+    return (elem && lists) ? build_node_stage(c, bs) : FSGH_OK;
 }
 
 DMesh fsgh::tuned_mesh(fsgh_ctx* c, int kernel) { return mesh_for(c, kernel, c->tune[kernel]); }
+
+int fsgh::small_block_tables(fsgh_ctx* c)
+{
+    if (int rc = ensure_table(c, true, 64, 1, false)) return rc;
+    return ensure_table(c, false, 64, 1, false);
+}
+
+DMesh fsgh::small_block_mesh(fsgh_ctx* c)
+{
+    DMesh m = make_mesh(c);
+    const fsgh_ctx::WorkTable& we = c->tables.at(table_key(true, 64, 1));
+    const fsgh_ctx::WorkTable& wn = c->tables.at(table_key(false, 64, 1));
+    m.bs = 64;
+    m.eblk = we.d;
+    m.n_eblk = we.n;
+    m.nblk = wn.d;
+    m.n_nblk = wn.n;
+    attach_node_stage(c, m, 64);
+    attach_corner_stage(c, m, 64);
+    return m;
+}
 
 DMesh fsgh::dt_mesh(fsgh_ctx* c)
 {
@@ -295,7 +325,7 @@ int fsgh_autotune(fsgh_ctx* c, int32_t reps, fsgh_tuning* out)
     for (int k = 0; k < kNumTuned; ++k) {
         cand[k] = candidates(k);
         for (const KernelCfg& g : cand[k])
-            if (int rc = ensure_table(c, elem_kernel(k), g.bs, g.xcd)) return rc;
+            if (int rc = ensure_kernel_data(c, k, g)) return rc;
     }
     Snapshot S;
     snapshot_parts(c, S);
