@@ -78,30 +78,35 @@ __device__ __forceinline__ void load_lconn(const uint16_t* lconn, int e, int32_t
 
 // The 24 corner forces of this lane's element, read cooperatively by the
 // wave: the wave's elements own one contiguous stretch of F, which the lanes
-// load as consecutive 16-byte pieces (coalesced) in two halves of 32
-// elements and pass through a padded LDS region (13 x 16 B per element) to
-// the lane of each element.  buf: 32 x 13 double2 of this wave; e0: the
-// wave's first element; nvalid: elements of the wave inside the work item.
-__device__ __forceinline__ void wave_load_forces(const double* __restrict__ F, double2* buf, int e0,
-                                                 int nvalid, double f[24])
+// load as consecutive 16-byte pieces (coalesced; wave_issue_forces, whose
+// loads can be in flight while other work proceeds) and pass through a
+// padded LDS region (13 x 16 B per element, two halves of 32 elements) to the
+// lane of each element (wave_scatter_forces).  buf: 32 x 13 double2 of this
+// wave; e0: the wave's first element; nvalid: elements of the wave inside the
+// work item.
+__device__ __forceinline__ void wave_issue_forces(const double* __restrict__ F, int e0, int nvalid,
+                                                  double2 t[12])
 {
     const int lane = static_cast<int>(threadIdx.x) & 63;
     const double2* src = reinterpret_cast<const double2*>(F + 24 * static_cast<size_t>(e0));
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        double2 t[6];
+    for (int r = 0; r < 12; ++r) {
+        const int idx = 64 * r + lane;                // piece within the wave's stretch
+        t[r] = (idx / 12 < nvalid) ? src[idx] : make_double2(0.0, 0.0);
+    }
+}
+
+__device__ __forceinline__ void wave_scatter_forces(const double2 t[12], double2* buf, double f[24])
+{
+    const int lane = static_cast<int>(threadIdx.x) & 63;
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            const int idx = 64 * r + lane;            // piece within the half
-            const int j = idx / 12;
-            t[r] = (32 * h + j < nvalid) ? src[384 * h + idx] : make_double2(0.0, 0.0);
-        }
+    for (int h = 0; h < 2; ++h) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            const int idx = 64 * r + lane;
-            buf[(idx / 12) * 13 + idx % 12] = t[r];
+            const int idx = 64 * r + lane;            // piece within the half
+            buf[(idx / 12) * 13 + idx % 12] = t[6 * h + r];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -691,10 +696,16 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
     if (c.stopped) return;
     const int e = bi.start + static_cast<int>(threadIdx.x);
     // the block's distinct nodes in LDS: u1, v1, w1, u0, v0, w0, x1, y1, z1;
-    // per wave a region for its corner forces (wave_load_forces)
+    // per wave a region for its corner forces (wave_scatter_forces)
     constexpr int CAP = stage_cap<BS>();
     __shared__ double sd[9 * CAP];
     __shared__ double2 sfw[(BS / 64) * 32 * 13];
+    // this wave's corner forces are requested first, so that they arrive
+    // while the block stages its nodes
+    const int w0e = bi.start + (static_cast<int>(threadIdx.x) & ~63);
+    const bool wave_live = w0e < bi.end;   // wave-uniform
+    double2 ft[12];
+    if (wave_live) wave_issue_forces(s.F, w0e, bi.end - w0e, ft);
     int q0, nn;
     stage_range(m, bi, CAP, q0, nn);
     const bool staged = nn <= CAP;
@@ -715,11 +726,7 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
     }
     double dtl = DBL_MAX;
     double f[24];
-    {
-        const int w0e = bi.start + (static_cast<int>(threadIdx.x) & ~63);
-        if (w0e < bi.end)   // wave-uniform
-            wave_load_forces(s.F, sfw + (threadIdx.x >> 6) * 32 * 13, w0e, bi.end - w0e, f);
-    }
+    if (wave_live) wave_scatter_forces(ft, sfw + (threadIdx.x >> 6) * 32 * 13, f);
     if (e < bi.end) {
         int32_t nd[8];
         double x[8], y[8], z[8], u1[8], v1[8], w1[8], u0[8], v0[8], w0[8];
