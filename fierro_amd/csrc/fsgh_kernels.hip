@@ -706,6 +706,12 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
     const bool wave_live = w0e < bi.end;   // wave-uniform
     double2 ft[12];
     if (wave_live) wave_issue_forces(s.F, w0e, bi.end - w0e, ft);
+    // and the element's own inputs
+    const bool live = e < bi.end;
+    const double emass = live ? s.emass[e] : 1.0;
+    const double sie0 = live ? s.sie0[e] : 0.0;
+    const double gam = live ? s.gamma[e] : 0.0;
+    const double csmin = live ? s.csmin[e] : 0.0;
     int q0, nn;
     stage_range(m, bi, CAP, q0, nn);
     const bool staged = nn <= CAP;
@@ -739,16 +745,14 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
             gather3(s.u1, s.v1, s.w1, nd, u1, v1, w1);
             gather3(s.u0, s.v0, s.w0, nd, u0, v0, w0);
         }
-        const double emass = s.emass[e];
         const double sie =
-            energy_update(f, u1, v1, w1, u0, v0, w0, s.sie0[e], emass, alpha, c.dt);
+            energy_update(f, u1, v1, w1, u0, v0, w0, sie0, emass, alpha, c.dt);
         s.sie1[e] = sie;
         if (staged) lds_gather3(sd + 6 * CAP, sd + 7 * CAP, sd + 8 * CAP, nd, x, y, z);
         else gather3(s.x1, s.y1, s.z1, nd, x, y, z);
         double vol, den, p, cs, div;
         const int32_t kind = USER ? s.mats[s.nmat > 1 ? s.mat[e] : 0].eos : kEosIdealGas;
-        derived_state(x, y, z, u1, v1, w1, emass, sie, s.gamma[e], s.csmin[e], kind, vol, den, p, cs,
-                      div);
+        derived_state(x, y, z, u1, v1, w1, emass, sie, gam, csmin, kind, vol, den, p, cs, div);
         s.vol[e] = vol;
         s.den[e] = den;
         s.pres[e] = p;
