@@ -255,19 +255,23 @@ def sedov_rz(args):
 
 def kernel_table(g, prof, tuning):
     """Per fused 3D kernel: mean launch time, launches, algorithmic bytes and
-    GB/s.  The energy kernel's bytes count the node lists of its element
-    partition (the library reports their size); the velocity kernel's count
+    GB/s.  The element kernels' bytes count the node lists of their element
+    partitions (the library reports their size); the velocity kernel's count
     the staged corner lists (every node block of these hex meshes, at most 8
     corners per node, fits the kernel's LDS capacity of 9 per node)."""
     from fierro_amd import bytemodel
 
-    eb = int(tuning.get("energy_state_div", {}).get("block", 256))
-    lists = g.stage_info(eb)[0] or None
+    lists = {}
+    for name in ("force", "energy_state_div"):
+        bs = int(tuning.get(name, {}).get("block", 256))
+        n = g.stage_info(bs)[0]
+        if n:
+            lists[name] = n
     kernels = {}
     for name in bytemodel.KERNELS:
         ms, calls = prof.get(name, (0.0, 0))
         avg = ms / max(calls, 1)
-        b = bytemodel.kernel_bytes(name, g.E, g.N, lists)
+        b = bytemodel.kernel_bytes(name, g.E, g.N, lists.get(name))
         kernels[name] = {"avg_ms": avg, "calls": calls, "alg_bytes": b,
                          "GBs": b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0}
     return kernels, lists

@@ -15,10 +15,14 @@ def div_extremes(E, N):
     return (32 * E + 4 * (N + 1)) + 8 * E + 32 * N
 
 
-def force(E, N):
-    # conn 32E + x1,v1 48N + vol,div,den,sspd,pres 40E + one divergence-extreme pair 16N
-    # + F 24C (w)
-    return 32 * E + 48 * N + 40 * E + 16 * N + 192 * E
+def force(E, N, list_entries=None):
+    # connectivity + x1,v1 48N + vol,div,den,sspd,pres 40E + the divergence-extreme
+    # pairs 16N + F 24C (w).  Connectivity: 16-bit local connectivity 16E + the
+    # blocks' node lists 4 x list_entries when the nodes are staged in LDS (both
+    # extreme pairs are then staged: 32N), else conn 32E.
+    if list_entries is None:
+        return 32 * E + 48 * N + 40 * E + 16 * N + 192 * E
+    return 16 * E + 4 * list_entries + 48 * N + 40 * E + 32 * N + 192 * E
 
 
 def vel_bc_pos(E, N, corner_staged=True):
@@ -69,17 +73,23 @@ RZ_KERNELS = {"rz_div_extremes": rz_div_extremes, "rz_force": rz_force, "rz_node
               "rz_energy_state_div": rz_energy_state_div}
 
 
-def cycle(E, N, list_entries=None, corner_staged=True):
+def cycle(E, N, lists=None, corner_staged=True):
     """Bytes of one fused RK2 cycle (2 stages; dt and control kernels are O(1)
-    per block)."""
-    return 2 * (div_extremes(E, N) + force(E, N) + vel_bc_pos(E, N, corner_staged)
-                + energy_state_div(E, N, list_entries))
+    per block).  lists: node-list entries per element kernel
+    ({"force": n, "energy_state_div": n}; absent = global gathers)."""
+    lists = lists or {}
+    return 2 * (div_extremes(E, N) + force(E, N, lists.get("force"))
+                + vel_bc_pos(E, N, corner_staged)
+                + energy_state_div(E, N, lists.get("energy_state_div")))
 
 
 def kernel_bytes(name, E, N, list_entries=None, corner_staged=True):
-    """Algorithmic bytes of one launch of a fused 3D kernel."""
+    """Algorithmic bytes of one launch of a fused 3D kernel (list_entries:
+    the node-list entries of that kernel's element partition, if staged)."""
     if name == "vel_bc_pos":
         return vel_bc_pos(E, N, corner_staged)
     if name == "energy_state_div":
         return energy_state_div(E, N, list_entries)
+    if name == "force":
+        return force(E, N, list_entries)
     return KERNELS[name](E, N)

@@ -223,25 +223,57 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
         small = ctl[bi.inst].small;
     }
     const int e = bi.start + static_cast<int>(threadIdx.x);
-    // the block's corner forces are staged in LDS (row stride 13 x 16 B, free
-    // of bank conflicts) and written out as contiguous 16-byte stores.  (The
-    // node reads stay global gathers here: this kernel is bound by latency at
-    // its register-limited occupancy, not by the address units, and a staging
-    // phase with its barrier measured slower.)
-    __shared__ double2 smem[BS * 13];
-    if (e < bi.end) {
-        int32_t nd[8];
+    // One LDS buffer, used twice: first the block's distinct nodes (x, y, z,
+    // u, v, w as [6][CAP] doubles, then the divergence extremes as [2][CAP]
+    // double2; lists from fsgh_stage.hip), then the block's corner forces
+    // (row stride 13 x 16 B, free of bank conflicts), written out as
+    // contiguous 16-byte stores.  The element's own inputs and connectivity
+    // are requested before the staging barrier.  Blocks over the capacity
+    // gather from global memory (block-uniform branch, same results).
+    constexpr int CAP = stage_cap<BS>();
+    constexpr int kWords = BS * 13 > CAP * 5 ? BS * 13 : CAP * 5;
+    __shared__ double2 smem[kWords];
+    double* sd = reinterpret_cast<double*>(smem);
+    double2* sx = smem + 3 * CAP;
+    const bool live = e < bi.end;
+    const double vol = live ? s.vol[e] : 1.0;
+    const double div = live ? s.div[e] : 0.0;
+    const double den = live ? s.den[e] : 0.0;
+    const double sspd = live ? s.sspd[e] : 0.0;
+    const double pres = live ? s.pres[e] : 0.0;
+    const Material mt = s.mats[(s.nmat > 1 && live) ? s.mat[e] : 0];
+    int q0, nn;
+    stage_range(m, bi, CAP, q0, nn);
+    const bool staged = nn <= CAP;
+    int32_t nd[8];
+    if (live) {
+        if (staged) load_lconn(m.sn_lconn, e, nd);
+        else load_conn(m.conn, e, nd);
+    }
+    if (staged) {
+        for (int i = static_cast<int>(threadIdx.x); i < nn; i += BS) {
+            const int n = m.sn_nodes[q0 + i];
+            sd[0 * CAP + i] = s.x1[n];
+            sd[1 * CAP + i] = s.y1[n];
+            sd[2 * CAP + i] = s.z1[n];
+            sd[3 * CAP + i] = s.u1[n];
+            sd[4 * CAP + i] = s.v1[n];
+            sd[5 * CAP + i] = s.w1[n];
+            sx[i] = s.ext_lo[n];
+            sx[CAP + i] = s.ext_hi[n];
+        }
+        __syncthreads();
+    }
+    double f[24];
+    if (live) {
         double x[8], y[8], z[8], u[8], v[8], w[8];
-        load_conn(m.conn, e, nd);
-        gather3(s.x1, s.y1, s.z1, nd, x, y, z);
-        gather3(s.u1, s.v1, s.w1, nd, u, v, w);
-        const double vol = s.vol[e];
-        const double div = s.div[e];
-        const double den = s.den[e];
-        const double sspd = s.sspd[e];
-        const double pres = s.pres[e];
-        const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
-
+        if (staged) {
+            lds_gather3(sd, sd + CAP, sd + 2 * CAP, nd, x, y, z);
+            lds_gather3(sd + 3 * CAP, sd + 4 * CAP, sd + 5 * CAP, nd, u, v, w);
+        } else {
+            gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+            gather3(s.u1, s.v1, s.w1, nd, u, v, w);
+        }
         // Shock detector neighbour extreme (K/force_sgh.cpp:297-310).  r_face =
         // 0.9 (d_nb + small)/(div + small) is monotone in d_nb (every rounding
         // step is monotone), so the minimum over neighbours is reached at min d_nb
@@ -251,17 +283,29 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
         // elements (with multiplicity); removing one copy of this element's own
         // value leaves the extreme over the other elements of that node.
         const bool lo = div + small >= 0.0;
-        const double2* __restrict__ xs = lo ? s.ext_lo : s.ext_hi;
         double ext = lo ? HUGE_VAL : -HUGE_VAL;
+        if (staged) {
+            const double2* xs = lo ? sx : sx + CAP;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const double2 t = xs[nd[a]];
-            const double other = (t.x == div) ? t.y : t.x;
-            ext = lo ? fmin(ext, other) : fmax(ext, other);
+            for (int a = 0; a < 8; ++a) {
+                const double2 t = xs[nd[a]];
+                const double other = (t.x == div) ? t.y : t.x;
+                ext = lo ? fmin(ext, other) : fmax(ext, other);
+            }
+        } else {
+            const double2* __restrict__ xs = lo ? s.ext_lo : s.ext_hi;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const double2 t = xs[nd[a]];
+                const double other = (t.x == div) ? t.y : t.x;
+                ext = lo ? fmin(ext, other) : fmax(ext, other);
+            }
         }
-        double f[24];
         force_elem(x, y, z, u, v, w, den, pres, sspd, vol, div, ext, mt.q1, mt.q1ex, mt.q2,
                    mt.q2ex, fuzz, small, f);
+    }
+    if (staged) __syncthreads();   // every node read is done before smem is reused
+    if (live) {
 #pragma unroll
         for (int i = 0; i < 12; ++i)
             smem[threadIdx.x * 13 + i] = make_double2(f[2 * i], f[2 * i + 1]);

@@ -8,14 +8,15 @@ own mesh.  Every configuration gives bit-identical results.
 Measured (mean per launch, default = 256 threads, no floor, XCD dealing):
 div_extremes 0.327 ms with 512 threads in launch order (default 0.339 ms);
 vel_bc_pos 1.119 ms with 128 threads in launch order (default 1.296 ms: its
-LDS copy of the block's corner forces limits blocks per CU); force and
-energy_state_div within noise of the default (their 3-wave variants cannot
-reach 3 waves without spilling).
+LDS copy of the block's corner forces limits blocks per CU);
+energy_state_div 1.426 ms in launch order (default 1.446 ms); force
+1.501 ms with 128 threads, a 3-wave floor (6 VGPRs spilled) and XCD dealing
+(default 1.790 ms).
 """
 
 TABLE = {
     "div_extremes": {"block": 512, "min_waves": 1, "xcd_order": 0},
-    "force": {"block": 256, "min_waves": 1, "xcd_order": 1},
+    "force": {"block": 128, "min_waves": 3, "xcd_order": 1},
     "vel_bc_pos": {"block": 128, "min_waves": 1, "xcd_order": 0},
-    "energy_state_div": {"block": 256, "min_waves": 1, "xcd_order": 1},
+    "energy_state_div": {"block": 256, "min_waves": 1, "xcd_order": 0},
 }
