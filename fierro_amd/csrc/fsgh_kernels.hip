@@ -40,6 +40,20 @@ __device__ __forceinline__ void gather3(const double* __restrict__ a, const doub
     }
 }
 
+// Streaming 16-byte accesses (non-temporal): the corner forces are written
+// once by the force kernel and read once per element by the energy kernel,
+// gigabytes apart at large sizes, so they gain nothing from the caches.
+typedef double v2d_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void stream_store(double2* p, double2 v)
+{
+    __builtin_nontemporal_store(v2d_t{v.x, v.y}, reinterpret_cast<v2d_t*>(p));
+}
+__device__ __forceinline__ double2 stream_load(const double2* p)
+{
+    const v2d_t q = __builtin_nontemporal_load(reinterpret_cast<const v2d_t*>(p));
+    return make_double2(q.x, q.y);
+}
+
 // LDS staging of a block's nodes (lists built in fsgh_stage.hip): capacity
 // in nodes per element block of BS elements (a Morton brick of BS elements
 // has (a+1)(b+1)(c+1) nodes, 405 for 256; irregular blocks above the
@@ -92,7 +106,7 @@ __device__ __forceinline__ void wave_issue_forces(const double* __restrict__ F, 
 #pragma unroll
     for (int r = 0; r < 12; ++r) {
         const int idx = 64 * r + lane;                // piece within the wave's stretch
-        t[r] = (idx / 12 < nvalid) ? src[idx] : make_double2(0.0, 0.0);
+        t[r] = (idx / 12 < nvalid) ? stream_load(src + idx) : make_double2(0.0, 0.0);
     }
 }
 
@@ -236,34 +250,104 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
     double* sd = reinterpret_cast<double*>(smem);
     double2* sx = smem + 3 * CAP;
     const bool live = e < bi.end;
-    const double vol = live ? s.vol[e] : 1.0;
-    const double div = live ? s.div[e] : 0.0;
-    const double den = live ? s.den[e] : 0.0;
-    const double sspd = live ? s.sspd[e] : 0.0;
-    const double pres = live ? s.pres[e] : 0.0;
-    const Material mt = s.mats[(s.nmat > 1 && live) ? s.mat[e] : 0];
+    // lanes past the work item read the item's first element (in bounds,
+    // results unused), so the loads need no divergent branches
+    const int ec = live ? e : bi.start;
+    const double vol = s.vol[ec];
+    const double div = s.div[ec];
+    const double den = s.den[ec];
+    const double sspd = s.sspd[ec];
     int q0, nn;
     stage_range(m, bi, CAP, q0, nn);
     const bool staged = nn <= CAP;
-    int32_t nd[8];
-    if (live) {
-        if (staged) load_lconn(m.sn_lconn, e, nd);
-        else load_conn(m.conn, e, nd);
-    }
+    // connectivity, kept packed until after the barrier (unpacking it here
+    // would make the staging loads wait for it)
+    uint4 craw0, craw1;
     if (staged) {
-        for (int i = static_cast<int>(threadIdx.x); i < nn; i += BS) {
-            const int n = m.sn_nodes[q0 + i];
-            sd[0 * CAP + i] = s.x1[n];
-            sd[1 * CAP + i] = s.y1[n];
-            sd[2 * CAP + i] = s.z1[n];
-            sd[3 * CAP + i] = s.u1[n];
-            sd[4 * CAP + i] = s.v1[n];
-            sd[5 * CAP + i] = s.w1[n];
-            sx[i] = s.ext_lo[n];
-            sx[CAP + i] = s.ext_hi[n];
+        craw0 = *reinterpret_cast<const uint4*>(m.sn_lconn + 8 * static_cast<size_t>(ec));
+    } else {
+        const uint4* p = reinterpret_cast<const uint4*>(m.conn + 8 * static_cast<size_t>(ec));
+        craw0 = p[0];
+        craw1 = p[1];
+    }
+    // staging: every list index is requested first, then every node value,
+    // and only then are the values stored to LDS, so no load waits behind
+    // another load's latency (R = CAP / BS rounded up, entries per lane)
+    constexpr int R = (CAP + BS - 1) / BS;
+    int nid[R];
+    double g[R][10];   // x, y, z, u, v, w, ext_lo pair, ext_hi pair
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            nid[r] = i < nn ? m.sn_nodes[q0 + i] : -1;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (nid[r] >= 0) {
+                const int n = nid[r];
+                g[r][0] = s.x1[n];
+                g[r][1] = s.y1[n];
+                g[r][2] = s.z1[n];
+                g[r][3] = s.u1[n];
+                g[r][4] = s.v1[n];
+                g[r][5] = s.w1[n];
+                const double2 lo = s.ext_lo[n], hi = s.ext_hi[n];
+                g[r][6] = lo.x;
+                g[r][7] = lo.y;
+                g[r][8] = hi.x;
+                g[r][9] = hi.y;
+            }
+        }
+    }
+    // the element-only terms, formed while the node values are in flight
+    // (this also shortens what stays live across the staging barrier); one
+    // material is read uniformly, without a per-element index load
+    Material mt;
+    if (s.nmat > 1) mt = s.mats[s.mat[ec]];
+    else mt = s.mats[0];
+    ForceCoef<double> kc =
+        force_coef(den, 0.0, sspd, vol, div, mt.q1, mt.q1ex, mt.q2, mt.q2ex, fuzz);
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            if (nid[r] >= 0) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sd[k * CAP + i] = g[r][k];
+                sx[i] = make_double2(g[r][6], g[r][7]);
+                sx[CAP + i] = make_double2(g[r][8], g[r][9]);
+            }
         }
         __syncthreads();
     }
+    int32_t nd[8];
+    if (staged) {
+        const uint4 p = craw0;
+        nd[0] = static_cast<int32_t>(p.x & 0xffffu);
+        nd[1] = static_cast<int32_t>(p.x >> 16);
+        nd[2] = static_cast<int32_t>(p.y & 0xffffu);
+        nd[3] = static_cast<int32_t>(p.y >> 16);
+        nd[4] = static_cast<int32_t>(p.z & 0xffffu);
+        nd[5] = static_cast<int32_t>(p.z >> 16);
+        nd[6] = static_cast<int32_t>(p.w & 0xffffu);
+        nd[7] = static_cast<int32_t>(p.w >> 16);
+    } else {
+        nd[0] = static_cast<int32_t>(craw0.x);
+        nd[1] = static_cast<int32_t>(craw0.y);
+        nd[2] = static_cast<int32_t>(craw0.z);
+        nd[3] = static_cast<int32_t>(craw0.w);
+        nd[4] = static_cast<int32_t>(craw1.x);
+        nd[5] = static_cast<int32_t>(craw1.y);
+        nd[6] = static_cast<int32_t>(craw1.z);
+        nd[7] = static_cast<int32_t>(craw1.w);
+    }
+    // the pressure is needed only by the last loop of the corner forces:
+    // requested after the barrier, its latency hides behind the B matrix
+    kc.pres = s.pres[ec];
+    kc.den = s.den[ec];
+    kc.div = s.div[ec];
+    const double div_e = kc.div;
     double f[24];
     if (live) {
         double x[8], y[8], z[8], u[8], v[8], w[8];
@@ -282,14 +366,14 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
         // stores per node the two smallest and two largest divergences of its
         // elements (with multiplicity); removing one copy of this element's own
         // value leaves the extreme over the other elements of that node.
-        const bool lo = div + small >= 0.0;
+        const bool lo = div_e + small >= 0.0;
         double ext = lo ? HUGE_VAL : -HUGE_VAL;
         if (staged) {
             const double2* xs = lo ? sx : sx + CAP;
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 const double2 t = xs[nd[a]];
-                const double other = (t.x == div) ? t.y : t.x;
+                const double other = (t.x == div_e) ? t.y : t.x;
                 ext = lo ? fmin(ext, other) : fmax(ext, other);
             }
         } else {
@@ -297,12 +381,11 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 const double2 t = xs[nd[a]];
-                const double other = (t.x == div) ? t.y : t.x;
+                const double other = (t.x == div_e) ? t.y : t.x;
                 ext = lo ? fmin(ext, other) : fmax(ext, other);
             }
         }
-        force_elem(x, y, z, u, v, w, den, pres, sspd, vol, div, ext, mt.q1, mt.q1ex, mt.q2,
-                   mt.q2ex, fuzz, small, f);
+        force_elem_coef(x, y, z, u, v, w, kc, ext, fuzz, small, f);
     }
     if (staged) __syncthreads();   // every node read is done before smem is reused
     if (live) {
@@ -316,7 +399,7 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
 #pragma unroll
     for (int r = 0; r < 12; ++r) {
         const int idx = r * BS + static_cast<int>(threadIdx.x);
-        if (idx < cnt) out[idx] = smem[(idx / 12) * 13 + idx % 12];
+        if (idx < cnt) stream_store(out + idx, smem[(idx / 12) * 13 + idx % 12]);
     }
 }
 
@@ -718,12 +801,13 @@ __global__ __launch_bounds__(BS) void k_vel_bc_pos(DMesh m, DState s, const Inst
     x0[1] = s.y0[n];
     x0[2] = s.z0[n];
     node_update(f, s.nmass[n], v0, x0, s.bcmask[n], alpha, dt, v1, x1);
-    s.u1[n] = v1[0];
-    s.v1[n] = v1[1];
-    s.w1[n] = v1[2];
-    s.x1[n] = x1[0];
-    s.y1[n] = x1[1];
-    s.z1[n] = x1[2];
+    // streaming stores: the next reader is another kernel, megabytes later
+    __builtin_nontemporal_store(v1[0], &s.u1[n]);
+    __builtin_nontemporal_store(v1[1], &s.v1[n]);
+    __builtin_nontemporal_store(v1[2], &s.w1[n]);
+    __builtin_nontemporal_store(x1[0], &s.x1[n]);
+    __builtin_nontemporal_store(x1[1], &s.y1[n]);
+    __builtin_nontemporal_store(x1[2], &s.z1[n]);
 }
 
 // energy + volume + EOS + next divergence (+ next cycle's dt candidates),
@@ -760,17 +844,37 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
     stage_range(m, bi, CAP, q0, nn);
     const bool staged = nn <= CAP;
     if (staged) {
-        for (int i = static_cast<int>(threadIdx.x); i < nn; i += BS) {
-            const int n = m.sn_nodes[q0 + i];
-            sd[0 * CAP + i] = s.u1[n];
-            sd[1 * CAP + i] = s.v1[n];
-            sd[2 * CAP + i] = s.w1[n];
-            sd[3 * CAP + i] = s.u0[n];
-            sd[4 * CAP + i] = s.v0[n];
-            sd[5 * CAP + i] = s.w0[n];
-            sd[6 * CAP + i] = s.x1[n];
-            sd[7 * CAP + i] = s.y1[n];
-            sd[8 * CAP + i] = s.z1[n];
+        // every list index first, then every node value, then the LDS stores
+        constexpr int R = (CAP + BS - 1) / BS;
+        int nid[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            nid[r] = i < nn ? m.sn_nodes[q0 + i] : -1;
+        }
+        double g[R][9];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (nid[r] >= 0) {
+                const int n = nid[r];
+                g[r][0] = s.u1[n];
+                g[r][1] = s.v1[n];
+                g[r][2] = s.w1[n];
+                g[r][3] = s.u0[n];
+                g[r][4] = s.v0[n];
+                g[r][5] = s.w0[n];
+                g[r][6] = s.x1[n];
+                g[r][7] = s.y1[n];
+                g[r][8] = s.z1[n];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            if (nid[r] >= 0) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) sd[k * CAP + i] = g[r][k];
+            }
         }
         __syncthreads();
     }

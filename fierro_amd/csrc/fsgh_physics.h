@@ -85,15 +85,42 @@ FSGH_HD inline void velgrad(const T* B, const T* u, const T* v, const T* w, T vo
         }
 }
 
-// Corner forces of one element, 3D, strength-free (K/force_sgh.cpp:35-359).
-// `ext` is the extreme neighbour divergence chosen by the shock detector:
-// min over neighbours when div + small >= 0, max otherwise (the kernels
-// find it; r_face is monotone in the neighbour divergence).
+// Element-only terms of the corner force (K/force_sgh.cpp:217-224, :318-321):
+// the impedance coefficients chosen by compression or expansion (qas = qa c,
+// qb) and the shock limiter alpha = min(1, 20 V^(1/3) |div| / (c + fuzz)).
+// They depend on no node value, so a kernel can form them while its node
+// values are still in flight.
 template <typename T>
-FSGH_HD inline void force_elem(const T* x, const T* y, const T* z, const T* u, const T* v,
-                               const T* w, T den, T pres, T sspd, T vol, T div, T ext,
-                               double q1, double q1ex, double q2, double q2ex, double fuzz,
-                               double small, T* f)
+struct ForceCoef {
+    T den, pres, qas, sspd, vol, div;
+    double qb;
+};
+
+template <typename T>
+FSGH_HD inline ForceCoef<T> force_coef(T den, T pres, T sspd, T vol, T div, double q1,
+                                       double q1ex, double q2, double q2ex, double fuzz)
+{
+    ForceCoef<T> k;
+    const double qa = div < 0.0 ? q1 : q1ex;
+    k.qb = div < 0.0 ? q2 : q2ex;
+    k.qas = qa * sspd;
+    k.sspd = sspd;
+    k.vol = vol;
+    k.den = den;
+    k.pres = pres;
+    k.div = div;
+    return k;
+}
+
+// Corner forces of one element, 3D, strength-free (K/force_sgh.cpp:35-359),
+// from the element terms of force_coef.  `ext` is the extreme neighbour
+// divergence chosen by the shock detector: min over neighbours when
+// div + small >= 0, max otherwise (the kernels find it; r_face is monotone
+// in the neighbour divergence).
+template <typename T>
+FSGH_HD inline void force_elem_coef(const T* x, const T* y, const T* z, const T* u, const T* v,
+                                    const T* w, const ForceCoef<T>& k, T ext, double fuzz,
+                                    double small, T* f)
 {
     T B[24];
     hex_bmatrix(x, y, z, B);
@@ -105,16 +132,14 @@ FSGH_HD inline void force_elem(const T* x, const T* y, const T* z, const T* u, c
         vsy += 0.125 * v[a];
         vsz += 0.125 * w[a];
     }
-    // impedance x |corner area| per node (:177-255); compression vs expansion (:217-224)
-    const double qa = div < 0.0 ? q1 : q1ex;
-    const double qb = div < 0.0 ? q2 : q2ex;
+    // impedance x |corner area| per node (:177-255): den (qa c + qb |v - v*|)
     T sum0 = 0.0, sum1 = 0.0, sum2 = 0.0, sum3 = 0.0;
     T mu[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         const T dx = u[a] - vsx, dy = v[a] - vsy, dz = w[a] - vsz;
         const T mag_vel = sqrt(dx * dx + dy * dy + dz * dz);
-        const T muc = den * (qa * sspd + qb * mag_vel);
+        const T muc = k.den * (k.qas + k.qb * mag_vel);
         const T bx = B[3 * a], by = B[3 * a + 1], bz = B[3 * a + 2];
         const T mu_term = muc * sqrt(bx * bx + by * by + bz * bz);
         sum0 += mu_term * u[a];
@@ -134,21 +159,32 @@ FSGH_HD inline void force_elem(const T* x, const T* y, const T* z, const T* u, c
         vsz = 0.0;
     }
     // shock detector (:287-328); pow(phi, 1.0) == phi
-    const T b = div + small;
+    const T b = k.div + small;
     const T r_face = 0.9 * (ext + small) / b;
     const T r_min = fmin(r_face, 1.0);
     T phi = 1.0 - fmax(0.0, r_min);
-    const T c_length = pow(vol, 1.0 / 3.0);
-    const T alpha = fmin(1.0, 20.0 * (c_length * fabs(div)) / (sspd + fuzz));
+    const T c_length = pow(k.vol, 1.0 / 3.0);
+    const T alpha = fmin(1.0, 20.0 * (c_length * fabs(k.div)) / (k.sspd + fuzz));
     phi = alpha * phi;
     // corner forces (:338-359): a = -B, tau = -p I, so a_i . tau_d = B_id p
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         const T pm = phi * mu[a];
-        f[3 * a + 0] = B[3 * a + 0] * pres + pm * (vsx - u[a]);
-        f[3 * a + 1] = B[3 * a + 1] * pres + pm * (vsy - v[a]);
-        f[3 * a + 2] = B[3 * a + 2] * pres + pm * (vsz - w[a]);
+        f[3 * a + 0] = B[3 * a + 0] * k.pres + pm * (vsx - u[a]);
+        f[3 * a + 1] = B[3 * a + 1] * k.pres + pm * (vsy - v[a]);
+        f[3 * a + 2] = B[3 * a + 2] * k.pres + pm * (vsz - w[a]);
     }
+}
+
+// Corner forces of one element, 3D, strength-free (K/force_sgh.cpp:35-359).
+template <typename T>
+FSGH_HD inline void force_elem(const T* x, const T* y, const T* z, const T* u, const T* v,
+                               const T* w, T den, T pres, T sspd, T vol, T div, T ext,
+                               double q1, double q1ex, double q2, double q2ex, double fuzz,
+                               double small, T* f)
+{
+    const ForceCoef<T> k = force_coef(den, pres, sspd, vol, div, q1, q1ex, q2, q2ex, fuzz);
+    force_elem_coef(x, y, z, u, v, w, k, ext, fuzz, small, f);
 }
 
 // Corner forces of one element, 3D, with a stress field: tau = stress - p I
