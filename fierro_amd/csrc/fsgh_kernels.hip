@@ -77,6 +77,15 @@ __device__ __forceinline__ void stage_range(const DMesh& m, const BlockItem& bi,
     }
 }
 
+// true when this wave's entries r0 + (threadIdx.x & ~63) ... start inside a
+// list of nn entries (wave-uniform, a scalar branch); lanes past the end
+// then load clamped entries whose values are not stored
+__device__ __forceinline__ bool wave_in_list(int r0, int nn)
+{
+    const int first = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) & ~63);
+    return r0 + first < nn;
+}
+
 __device__ __forceinline__ void load_lconn(const uint16_t* lconn, int e, int32_t lc[8])
 {
     const uint4 p = *reinterpret_cast<const uint4*>(lconn + 8 * static_cast<size_t>(e));
@@ -105,8 +114,10 @@ __device__ __forceinline__ void wave_issue_forces(const double* __restrict__ F, 
     const double2* src = reinterpret_cast<const double2*>(F + 24 * static_cast<size_t>(e0));
 #pragma unroll
     for (int r = 0; r < 12; ++r) {
+        // pieces past the wave's valid rows re-read piece 0 (nvalid >= 1):
+        // unconditional loads, whose values go unused
         const int idx = 64 * r + lane;                // piece within the wave's stretch
-        t[r] = (idx / 12 < nvalid) ? stream_load(src + idx) : make_double2(0.0, 0.0);
+        t[r] = stream_load(src + (idx < 12 * nvalid ? idx : 0));
     }
 }
 
@@ -253,13 +264,32 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
     // lanes past the work item read the item's first element (in bounds,
     // results unused), so the loads need no divergent branches
     const int ec = live ? e : bi.start;
-    const double vol = s.vol[ec];
-    const double div = s.div[ec];
-    const double den = s.den[ec];
-    const double sspd = s.sspd[ec];
+    // Staging: every list index is requested first (the load counter
+    // retires in issue order, so the node loads then wait for nothing
+    // else), then the element's inputs and connectivity, then every node
+    // value, and only then are the values stored to LDS (R = CAP / BS
+    // rounded up, list entries per lane).
     int q0, nn;
     stage_range(m, bi, CAP, q0, nn);
     const bool staged = nn <= CAP;
+    constexpr int R = (CAP + BS - 1) / BS;
+    uint32_t nid[R];   // unsigned: its address arithmetic needs no sign extension
+    double g[R][10];   // x, y, z, u, v, w, ext_lo pair, ext_hi pair
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            // clamped, unconditional loads: nothing computed from them before
+            // the node loads, so those wait for these alone
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            nid[r] = static_cast<uint32_t>(m.sn_nodes[q0 + (i < nn ? i : nn - 1)]);
+        }
+    }
+    // the divergence and sound speed select the impedance coefficients
+    // now; every element value the arithmetic needs is read again after the
+    // barrier (cache hits or a short wait behind the B matrix), so none of
+    // them occupies a register across the staging
+    const double div0 = s.div[ec];
+    const double sspd0 = s.sspd[ec];
     // connectivity, kept packed until after the barrier (unpacking it here
     // would make the staging loads wait for it)
     uint4 craw0, craw1;
@@ -270,22 +300,11 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
         craw0 = p[0];
         craw1 = p[1];
     }
-    // staging: every list index is requested first, then every node value,
-    // and only then are the values stored to LDS, so no load waits behind
-    // another load's latency (R = CAP / BS rounded up, entries per lane)
-    constexpr int R = (CAP + BS - 1) / BS;
-    int nid[R];
-    double g[R][10];   // x, y, z, u, v, w, ext_lo pair, ext_hi pair
     if (staged) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int i = r * BS + static_cast<int>(threadIdx.x);
-            nid[r] = i < nn ? m.sn_nodes[q0 + i] : -1;
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (nid[r] >= 0) {
-                const int n = nid[r];
+            if (wave_in_list(r * BS, nn)) {
+                const uint32_t n = nid[r];
                 g[r][0] = s.x1[n];
                 g[r][1] = s.y1[n];
                 g[r][2] = s.z1[n];
@@ -307,12 +326,12 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
     if (s.nmat > 1) mt = s.mats[s.mat[ec]];
     else mt = s.mats[0];
     ForceCoef<double> kc =
-        force_coef(den, 0.0, sspd, vol, div, mt.q1, mt.q1ex, mt.q2, mt.q2ex, fuzz);
+        force_coef(0.0, 0.0, sspd0, 1.0, div0, mt.q1, mt.q1ex, mt.q2, mt.q2ex, fuzz);
     if (staged) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = r * BS + static_cast<int>(threadIdx.x);
-            if (nid[r] >= 0) {
+            if (i < nn) {
 #pragma unroll
                 for (int k = 0; k < 6; ++k) sd[k * CAP + i] = g[r][k];
                 sx[i] = make_double2(g[r][6], g[r][7]);
@@ -342,11 +361,13 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
         nd[6] = static_cast<int32_t>(craw1.z);
         nd[7] = static_cast<int32_t>(craw1.w);
     }
-    // the pressure is needed only by the last loop of the corner forces:
-    // requested after the barrier, its latency hides behind the B matrix
+    // the element values, requested after the barrier: their latency hides
+    // behind the B matrix
     kc.pres = s.pres[ec];
     kc.den = s.den[ec];
     kc.div = s.div[ec];
+    kc.sspd = s.sspd[ec];
+    kc.vol = s.vol[ec];
     const double div_e = kc.div;
     double f[24];
     if (live) {
@@ -828,35 +849,41 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
     constexpr int CAP = stage_cap<BS>();
     __shared__ double sd[9 * CAP];
     __shared__ double2 sfw[(BS / 64) * 32 * 13];
-    // this wave's corner forces are requested first, so that they arrive
-    // while the block stages its nodes
+    // The load counter retires in issue order, so what the staging needs
+    // first is requested first: the block's node-list indices, then this
+    // wave's corner forces (they arrive while the block stages its nodes),
+    // then the element's own inputs.
+    int q0, nn;
+    stage_range(m, bi, CAP, q0, nn);
+    const bool staged = nn <= CAP;
+    constexpr int R = (CAP + BS - 1) / BS;
+    uint32_t nid[R];   // unsigned: its address arithmetic needs no sign extension
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            // clamped, unconditional loads: nothing computed from them before
+            // the node loads, so those wait for these alone
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            nid[r] = static_cast<uint32_t>(m.sn_nodes[q0 + (i < nn ? i : nn - 1)]);
+        }
+    }
     const int w0e = bi.start + (static_cast<int>(threadIdx.x) & ~63);
     const bool wave_live = w0e < bi.end;   // wave-uniform
     double2 ft[12];
-    if (wave_live) wave_issue_forces(s.F, w0e, bi.end - w0e, ft);
-    // and the element's own inputs
+    // a wave past the item reads the item's first row (values unused)
+    wave_issue_forces(s.F, wave_live ? w0e : bi.start, wave_live ? bi.end - w0e : 1, ft);
     const bool live = e < bi.end;
     const double emass = live ? s.emass[e] : 1.0;
     const double sie0 = live ? s.sie0[e] : 0.0;
     const double gam = live ? s.gamma[e] : 0.0;
     const double csmin = live ? s.csmin[e] : 0.0;
-    int q0, nn;
-    stage_range(m, bi, CAP, q0, nn);
-    const bool staged = nn <= CAP;
     if (staged) {
-        // every list index first, then every node value, then the LDS stores
-        constexpr int R = (CAP + BS - 1) / BS;
-        int nid[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = r * BS + static_cast<int>(threadIdx.x);
-            nid[r] = i < nn ? m.sn_nodes[q0 + i] : -1;
-        }
+        // every node value, then the LDS stores
         double g[R][9];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            if (nid[r] >= 0) {
-                const int n = nid[r];
+            if (wave_in_list(r * BS, nn)) {
+                const uint32_t n = nid[r];
                 g[r][0] = s.u1[n];
                 g[r][1] = s.v1[n];
                 g[r][2] = s.w1[n];
@@ -871,7 +898,7 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = r * BS + static_cast<int>(threadIdx.x);
-            if (nid[r] >= 0) {
+            if (i < nn) {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) sd[k * CAP + i] = g[r][k];
             }
