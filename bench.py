@@ -82,6 +82,65 @@ def measured_traffic(workload, E, N, kernel):
     return rec["kernels"][kernel]["traffic_bytes"]
 
 
+# kernel keys of the traffic records: workload -> {key: kernel-name prefix}
+TRAFFIC_KERNELS = {
+    "sedov": {"div_extremes": "k_div_extremes<", "force": "k_force<",
+              "vel_bc_pos": "k_vel_bc_pos<", "energy_state_div": "k_energy_state_div<"},
+    "sedov-rz": {"rz_div_extremes": "k_div_extremes<", "rz_force": "k_rz_force",
+                 "rz_node": "k_rz_node", "rz_energy_state_div": "k_rz_energy_state_div"},
+}
+
+
+def traffic_record(workload, n, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out):
+    """Write profiles/<workload>_traffic.json from four rocprofv3 --pmc runs
+    (FETCH_SIZE and WRITE_SIZE of a bench run of this workload, and of
+    --copy-calib 1024): per kernel the mean counter value per dispatch, in
+    KiB, scaled by the copy's bytes over its counter value."""
+    import csv
+
+    def per_kernel(d, counter):
+        acc = {}
+        with open(os.path.join(d, "run_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                name = r["Kernel_Name"]
+                name = name[name.find("k_"):] if "::k_" in name else name
+                acc.setdefault(name.split("(")[0], []).append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+
+    copy_bytes = 1024 << 20
+    cal = {}
+    for counter, d in (("FETCH_SIZE", copy_fetch_dir), ("WRITE_SIZE", copy_write_dir)):
+        vals = [v for k, v in per_kernel(d, counter).items() if "copyBuffer" in k]
+        cal[counter] = copy_bytes / (1024.0 * vals[0])
+    fetch = per_kernel(fetch_dir, "FETCH_SIZE")
+    write = per_kernel(write_dir, "WRITE_SIZE")
+    from fierro_amd import bytemodel, problems as P
+    from fierro_amd.sgh import SGHSolver
+
+    spec = P.sedov_rz(n, cycles=1) if workload == "sedov-rz" else P.sedov(n, cycles=1)
+    g = SGHSolver(spec, renumber=True)
+    if workload == "sedov-rz":
+        alg = {k: fn(g.E, g.N) for k, fn in bytemodel.RZ_KERNELS.items()}
+    else:
+        kernels, _ = kernel_table(g, {}, tune(g, argparse.Namespace(tune="table")))
+        alg = {k: v["alg_bytes"] for k, v in kernels.items()}
+    rec = {"workload": f"{workload.replace('-', '_')}{n}", "elements": g.E, "nodes": g.N,
+           "calibration": cal, "kernels": {}}
+    for key, prefix in TRAFFIC_KERNELS[workload].items():
+        names = [k for k in fetch if k.startswith(prefix)]
+        if len(names) != 1:
+            raise SystemExit(f"traffic: kernel {prefix!r} matched {names}")
+        fb = fetch[names[0]] * 1024 * cal["FETCH_SIZE"]
+        wb = write[names[0]] * 1024 * cal["WRITE_SIZE"]
+        rec["kernels"][key] = {"variant": names[0], "fetch_bytes": fb, "write_bytes": wb,
+                               "traffic_bytes": fb + wb, "alg_bytes": alg[key]}
+    with open(out, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(json.dumps(rec, indent=1), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,7 +162,17 @@ def main():
                     help="time the per-instance loop on every k-th instance")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
+    ap.add_argument("--traffic-from", nargs=4, default=None,
+                    metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="build profiles/<workload>_traffic.json from rocprofv3 --pmc output "
+                         "directories (this workload's FETCH_SIZE and WRITE_SIZE runs, and "
+                         "the same counters of --copy-calib 1024)")
     args = ap.parse_args()
+    if args.traffic_from:
+        name = "sedov_rz" if args.workload == "sedov-rz" else "sedov"
+        n = args.n if args.workload == "sedov" else (args.n if args.n != 256 else 4096)
+        return traffic_record(args.workload, n, *args.traffic_from,
+                              os.path.join(ROOT, "profiles", f"{name}{n}_traffic.json"))
     if args.copy_calib:
         return copy_calibration(args.copy_calib)
     if args.gpus != 1:
