@@ -899,6 +899,9 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
         for (int r = 0; r < R; ++r) {
             const int i = r * BS + static_cast<int>(threadIdx.x);
             if (i < nn) {
+                // slots 3..5 hold the node's mean velocity (v1 + v0) / 2
+#pragma unroll
+                for (int k = 0; k < 3; ++k) g[r][3 + k] = half_sum(g[r][k], g[r][3 + k]);
 #pragma unroll
                 for (int k = 0; k < 9; ++k) sd[k * CAP + i] = g[r][k];
             }
@@ -910,21 +913,28 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
     if (wave_live) wave_scatter_forces(ft, sfw + (threadIdx.x >> 6) * 32 * 13, f);
     if (e < bi.end) {
         int32_t nd[8];
-        double x[8], y[8], z[8], u1[8], v1[8], w1[8], u0[8], v0[8], w0[8];
+        double sie;
         if (staged) {
             load_lconn(m.sn_lconn, e, nd);
-            lds_gather3(sd, sd + CAP, sd + 2 * CAP, nd, u1, v1, w1);
-            lds_gather3(sd + 3 * CAP, sd + 4 * CAP, sd + 5 * CAP, nd, u0, v0, w0);
+            double hu[8], hv[8], hw[8];
+            lds_gather3(sd + 3 * CAP, sd + 4 * CAP, sd + 5 * CAP, nd, hu, hv, hw);
+            sie = energy_update_h(f, hu, hv, hw, sie0, emass, alpha, c.dt);
         } else {
             load_conn(m.conn, e, nd);
+            double u1[8], v1[8], w1[8], u0[8], v0[8], w0[8];
             gather3(s.u1, s.v1, s.w1, nd, u1, v1, w1);
             gather3(s.u0, s.v0, s.w0, nd, u0, v0, w0);
+            sie = energy_update(f, u1, v1, w1, u0, v0, w0, sie0, emass, alpha, c.dt);
         }
-        const double sie =
-            energy_update(f, u1, v1, w1, u0, v0, w0, sie0, emass, alpha, c.dt);
         s.sie1[e] = sie;
-        if (staged) lds_gather3(sd + 6 * CAP, sd + 7 * CAP, sd + 8 * CAP, nd, x, y, z);
-        else gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+        double x[8], y[8], z[8], u1[8], v1[8], w1[8];
+        if (staged) {
+            lds_gather3(sd, sd + CAP, sd + 2 * CAP, nd, u1, v1, w1);
+            lds_gather3(sd + 6 * CAP, sd + 7 * CAP, sd + 8 * CAP, nd, x, y, z);
+        } else {
+            gather3(s.u1, s.v1, s.w1, nd, u1, v1, w1);
+            gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+        }
         double vol, den, p, cs, div;
         const int32_t kind = USER ? s.mats[s.nmat > 1 ? s.mat[e] : 0].eos : kEosIdealGas;
         derived_state(x, y, z, u1, v1, w1, emass, sie, gam, csmin, kind, vol, den, p, cs, div);

@@ -271,20 +271,44 @@ FSGH_HD inline void node_update(const T* f, T mass, const T* v0, const T* x0, ui
     }
 }
 
-// Element energy update (K/energy_sgh.cpp:18-50): F . (v1+v0)/2, corner by
-// corner, then e1 = e0 - alpha dt / M * power.
+// the node's mean velocity over the stage, (v1 + v0) / 2 per component, as
+// the energy update evaluates it (K/energy_sgh.cpp:34-43)
 template <typename T>
-FSGH_HD inline T energy_update(const T* f, const T* u1, const T* v1, const T* w1, const T* u0,
-                               const T* v0, const T* w0, T e0, T mass, double alpha, double dt)
+FSGH_HD inline T half_sum(T a1, T a0)
+{
+    return (a1 + a0) * 0.5;
+}
+
+// Element energy update (K/energy_sgh.cpp:18-50) from the nodes' mean
+// velocities hu, hv, hw = half_sum(v1, v0): F . (v1+v0)/2, corner by corner,
+// then e1 = e0 - alpha dt / M * power.
+template <typename T>
+FSGH_HD inline T energy_update_h(const T* f, const T* hu, const T* hv, const T* hw, T e0,
+                                 T mass, double alpha, double dt)
 {
     T power = 0.0;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        power += f[3 * a + 0] * ((u1[a] + u0[a]) * 0.5);
-        power += f[3 * a + 1] * ((v1[a] + v0[a]) * 0.5);
-        power += f[3 * a + 2] * ((w1[a] + w0[a]) * 0.5);
+        power += f[3 * a + 0] * hu[a];
+        power += f[3 * a + 1] * hv[a];
+        power += f[3 * a + 2] * hw[a];
     }
     return e0 - alpha * dt / mass * power;
+}
+
+// Element energy update (K/energy_sgh.cpp:18-50) from both velocity bins.
+template <typename T>
+FSGH_HD inline T energy_update(const T* f, const T* u1, const T* v1, const T* w1, const T* u0,
+                               const T* v0, const T* w0, T e0, T mass, double alpha, double dt)
+{
+    T hu[8], hv[8], hw[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        hu[a] = half_sum(u1[a], u0[a]);
+        hv[a] = half_sum(v1[a], v0[a]);
+        hw[a] = half_sum(w1[a], w0[a]);
+    }
+    return energy_update_h(f, hu, hv, hw, e0, mass, alpha, dt);
 }
 
 // Element state from coordinates, velocities, mass and energy: volume
