@@ -271,7 +271,7 @@ __global__ __launch_bounds__(BS, MINW) void k_force(DMesh m, DState s, const Ins
     // rounded up, list entries per lane).
     int q0, nn;
     stage_range(m, bi, CAP, q0, nn);
-    const bool staged = nn <= CAP;
+    const bool staged = nn > 0 && nn <= CAP;   // lists are never empty; the test keeps nn - 1 in range
     constexpr int R = (CAP + BS - 1) / BS;
     uint32_t nid[R];   // unsigned: its address arithmetic needs no sign extension
     double g[R][10];   // x, y, z, u, v, w, ext_lo pair, ext_hi pair
@@ -855,7 +855,7 @@ __global__ __launch_bounds__(BS, MINW) void k_energy_state_div(DMesh m, DState s
     // then the element's own inputs.
     int q0, nn;
     stage_range(m, bi, CAP, q0, nn);
-    const bool staged = nn <= CAP;
+    const bool staged = nn > 0 && nn <= CAP;   // lists are never empty; the test keeps nn - 1 in range
     constexpr int R = (CAP + BS - 1) / BS;
     uint32_t nid[R];   // unsigned: its address arithmetic needs no sign extension
     if (staged) {
