@@ -51,11 +51,13 @@ void morton_renumber(const Layout& L, const int64_t* nodes_in_elem, const double
 void corner_csr(int64_t N, int64_t E, const int64_t* nodes_in_elem, std::vector<int64_t>& start,
                 std::vector<int64_t>& corners, int npe = 8);
 
-// Work items of `bs` entries per instance.  With `xcd` set they are dealt to
-// XCDs so that each XCD sweeps one contiguous range of logical blocks
-// (physical block b runs on XCD b % 8); otherwise physical = logical order.
+// Work items of `bs` entries per instance, in a physical order for the
+// hardware's round-robin dealing of workgroups to XCDs (physical block p runs
+// on XCD p % 8): deal = 0 keeps the logical order; 1 gives each XCD one
+// contiguous range of logical blocks; k >= 2 deals chunks of k consecutive
+// logical blocks round-robin to the XCDs.
 void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
-                 std::vector<int32_t>& inst_begin, int32_t bs = kBlock, bool xcd = true);
+                 std::vector<int32_t>& inst_begin, int32_t bs = kBlock, int32_t deal = 1);
 
 // Boundary node sets of one instance (K/setup.cpp:431-665): nodes of the
 // boundary faces whose 4 nodes all satisfy the surface test.  Each set is

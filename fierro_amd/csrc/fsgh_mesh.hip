@@ -76,7 +76,7 @@ void corner_csr(int64_t N, int64_t E, const int64_t* conn, std::vector<int64_t>&
 }
 
 void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
-                 std::vector<int32_t>& inst_begin, int32_t bs, bool xcd)
+                 std::vector<int32_t>& inst_begin, int32_t bs, int32_t deal)
 {
     std::vector<BlockItem> logical;
     const int32_t ninst = static_cast<int32_t>(off.size()) - 1;
@@ -94,18 +94,35 @@ void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
     }
     inst_begin[static_cast<size_t>(ninst)] = static_cast<int32_t>(logical.size());
     const int64_t nb = static_cast<int64_t>(logical.size());
-    if (!xcd) {
+    if (deal == 0) {
         table = logical;
         return;
     }
     table.resize(logical.size());
-    int64_t base[kNumXcd + 1];
-    base[0] = 0;
-    for (int x = 0; x < kNumXcd; ++x) base[x + 1] = base[x] + nb / kNumXcd + (x < nb % kNumXcd ? 1 : 0);
-    for (int64_t b = 0; b < nb; ++b) {
-        const int x = static_cast<int>(b % kNumXcd);
-        table[static_cast<size_t>(b)] = logical[static_cast<size_t>(base[x] + b / kNumXcd)];
+    if (deal == 1) {
+        // one contiguous range of logical blocks per XCD
+        int64_t base[kNumXcd + 1];
+        base[0] = 0;
+        for (int x = 0; x < kNumXcd; ++x)
+            base[x + 1] = base[x] + nb / kNumXcd + (x < nb % kNumXcd ? 1 : 0);
+        for (int64_t b = 0; b < nb; ++b) {
+            const int x = static_cast<int>(b % kNumXcd);
+            table[static_cast<size_t>(b)] = logical[static_cast<size_t>(base[x] + b / kNumXcd)];
+        }
+        return;
     }
+    // chunks of `deal` consecutive logical blocks dealt round-robin to the
+    // XCDs (chunk c to XCD c mod 8): neighbouring blocks share an L2 and
+    // every XCD sweeps the whole mesh, so their tails end together.  The
+    // physical order interleaves the XCDs' queues (workgroup p is dispatched
+    // to XCD p mod 8); queues that run out are skipped.
+    std::vector<std::vector<int64_t>> q(kNumXcd);
+    for (int64_t l = 0; l < nb; ++l) q[static_cast<size_t>((l / deal) % kNumXcd)].push_back(l);
+    std::vector<size_t> pos(kNumXcd, 0);
+    int64_t p = 0;
+    while (p < nb)
+        for (int x = 0; x < kNumXcd && p < nb; ++x)
+            if (pos[x] < q[x].size()) table[static_cast<size_t>(p++)] = logical[static_cast<size_t>(q[x][pos[x]++])];
 }
 
 int boundary_sets(const Layout& L, int32_t inst, const int64_t* conn, const double* xyz,

@@ -184,7 +184,7 @@ int build_node_stage(fsgh_ctx* c, int32_t bs)
     if (bs != 64 && bs != 128 && bs != 256 && bs != 512) return FSGH_OK;
     std::vector<BlockItem> logical;
     std::vector<int32_t> ib;
-    block_table(c->L.elem_off, logical, ib, bs, false);
+    block_table(c->L.elem_off, logical, ib, bs, 0);
     const int32_t nb = static_cast<int32_t>(logical.size());
     if (nb == 0) return FSGH_OK;
     fsgh_ctx::NodeStage ns;
@@ -258,7 +258,7 @@ int build_corner_stage(fsgh_ctx* c, int32_t bs)
     if (bs != 64 && bs != 128 && bs != 256 && bs != 512) return FSGH_OK;
     std::vector<BlockItem> logical;
     std::vector<int32_t> ib;
-    block_table(c->L.node_off, logical, ib, bs, false);
+    block_table(c->L.node_off, logical, ib, bs, 0);
     const int32_t nb = static_cast<int32_t>(logical.size());
     if (nb == 0) return FSGH_OK;
     fsgh_ctx::CornerStage cs;

@@ -21,9 +21,12 @@ using namespace fsgh;
 
 namespace {
 
+// xcd_order: 0 launch order, 1 contiguous per XCD, 2..kMaxDealChunk chunks
+constexpr int32_t kMaxDealChunk = 4096;
+
 int32_t table_key(bool elem, int32_t bs, int32_t xcd)
 {
-    return (elem ? (1 << 20) : 0) | (bs << 1) | (xcd ? 1 : 0);
+    return (elem ? (1 << 30) : 0) | (bs << 16) | xcd;
 }
 
 bool elem_kernel(int k) { return k == kTunedForce || k == kTunedEnergy; }
@@ -92,7 +95,7 @@ int apply(fsgh_ctx* c, int k, const KernelCfg& g)
 
 bool valid(int k, const KernelCfg& g)
 {
-    return k >= 0 && k < kNumTuned && (g.xcd == 0 || g.xcd == 1) &&
+    return k >= 0 && k < kNumTuned && g.xcd >= 0 && g.xcd <= kMaxDealChunk &&
            tuned_variant_ok(k, g.bs, g.minw);
 }
 
@@ -101,7 +104,7 @@ std::vector<KernelCfg> candidates(int k)
     std::vector<KernelCfg> out;
     for (int32_t bs : {256, 128, 64, 512})
         for (int32_t minw : {1, 3})
-            for (int32_t xcd : {1, 0}) {
+            for (int32_t xcd : {1, 0, 4, 8, 16, 32}) {
                 KernelCfg g;
                 g.bs = bs;
                 g.minw = minw;
@@ -173,7 +176,7 @@ int fsgh::ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd, bool lis
     if (c->tables.count(key)) return (elem && lists) ? build_node_stage(c, bs) : FSGH_OK;
     std::vector<BlockItem> tab;
     fsgh_ctx::WorkTable w;
-    block_table(elem ? c->L.elem_off : c->L.node_off, tab, w.inst_begin, bs, xcd != 0);
+    block_table(elem ? c->L.elem_off : c->L.node_off, tab, w.inst_begin, bs, xcd);
     w.n = static_cast<int32_t>(tab.size());
     if (elem && tab.size() > c->partial_cap) {
         // the dt partial minima are indexed by logical element block

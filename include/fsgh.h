@@ -282,8 +282,8 @@ int fsgh_run_status(fsgh_ctx *ctx, int32_t instance, fsgh_instance_status *out);
  * four kernels of the fused cycle run in one of several compiled variants:
  * workgroup size (elements or nodes per workgroup), an occupancy floor asked
  * of the register allocator (waves per SIMD; 1 = the compiler's choice) for
- * the two element kernels, and whether consecutive work ranges are dealt to
- * one XCD.  Every variant computes bit-identical results. */
+ * the two element kernels, and how work ranges are dealt to the XCDs.
+ * Every variant computes bit-identical results. */
 #define FSGH_KERNEL_DIV_EXTREMES 0
 #define FSGH_KERNEL_FORCE 1
 #define FSGH_KERNEL_VEL_BC_POS 2
@@ -294,7 +294,9 @@ typedef struct fsgh_tuning {
     int32_t kernel;       /* FSGH_KERNEL_* */
     int32_t block;        /* 64, 128, 256 (default); 512 for the two node kernels */
     int32_t min_waves;    /* 1 (default) or 3 (force and energy kernels only) */
-    int32_t xcd_order;    /* 1 (default): consecutive ranges on one XCD; 0: launch order */
+    int32_t xcd_order;    /* 1 (default): one contiguous range of work items per XCD;
+                             0: launch order; k = 2..4096: chunks of k consecutive
+                             work items dealt round-robin to the 8 XCDs */
     double ms;            /* fsgh_autotune: mean time per launch of this choice */
     double ms_default;    /* fsgh_autotune: the same for the default variant */
 } fsgh_tuning;

@@ -67,6 +67,19 @@ def test_variants_bitwise(reference, i):
     same(run_fields(cfg), reference)
 
 
+@pytest.mark.parametrize("chunk", [2, 3, 16])
+def test_chunk_dealing_bitwise(reference, chunk):
+    """Work items dealt to the XCDs in chunks of consecutive blocks (a chunk
+    count that does not divide the block count leaves uneven XCD queues): the
+    same results bit for bit."""
+    cfg = {"force": dict(block=64, min_waves=3, xcd_order=chunk),
+           "energy_state_div": dict(block=64, min_waves=1, xcd_order=chunk),
+           "vel_bc_pos": dict(block=64, min_waves=1, xcd_order=chunk),
+           "div_extremes": dict(block=128, min_waves=1, xcd_order=chunk)}
+    same(run_fields(cfg), reference)
+    same(run_fields(cfg, graph=False), reference)
+
+
 def test_autotune_midrun_leaves_state(reference):
     from fierro_amd.sgh import SGHSolver
     g = SGHSolver(batch())
@@ -118,7 +131,9 @@ def test_invalid_variants_rejected():
         with pytest.raises(FsghError):
             g.set_tuning({"force": dict(block=512)})
         with pytest.raises(FsghError):
-            g.set_tuning({"vel_bc_pos": dict(xcd_order=2)})
+            g.set_tuning({"vel_bc_pos": dict(xcd_order=-1)})
+        with pytest.raises(FsghError):
+            g.set_tuning({"energy_state_div": dict(xcd_order=5000)})
         assert all(v["block"] == 256 for v in g.get_tuning().values())
     finally:
         g.close()
