@@ -319,8 +319,8 @@ int fsgh_create(const fsgh_mesh_desc* d, const int64_t* nodes_in_elem, const dou
         corner_csr(c->N, c->E, c->conn.data(), c->cstart, c->corners, npe);
     }
     std::vector<BlockItem> etab, ntab;
-    block_table(c->L.elem_off, etab, c->eblk_begin);
-    block_table(c->L.node_off, ntab, c->nblk_begin);
+    block_table(c->L.elem_off, etab, c->eblk_begin, kBlock, kDefaultDeal);
+    block_table(c->L.node_off, ntab, c->nblk_begin, kBlock, kDefaultDeal);
     c->n_eblk = static_cast<int32_t>(etab.size());
     c->n_nblk = static_cast<int32_t>(ntab.size());
 

@@ -80,10 +80,14 @@ struct DMesh {
 constexpr int corner_stage_cap(int bs) { return bs * 9; }
 
 // Launch configuration of one fused-cycle kernel, chosen by fsgh_autotune.
+// How the default work tables deal work items to the XCDs: chunks of 16
+// consecutive items round-robin (see block_table)
+constexpr int32_t kDefaultDeal = 16;
+
 struct KernelCfg {
     int32_t bs = kBlock;       // workgroup size (64, 128, 256; 512 for the node kernels)
     int32_t minw = 1;          // waves-per-SIMD floor asked of the register allocator
-    int32_t xcd = 1;           // 1: consecutive work items dealt to one XCD
+    int32_t xcd = kDefaultDeal;  // XCD dealing: 0 launch order, 1 contiguous, k chunks of k
 };
 enum TunedKernel { kTunedDivExtremes = 0, kTunedForce, kTunedVelBcPos, kTunedEnergy, kNumTuned };
 

@@ -31,7 +31,7 @@ int32_t table_key(bool elem, int32_t bs, int32_t xcd)
 
 bool elem_kernel(int k) { return k == kTunedForce || k == kTunedEnergy; }
 
-bool default_table(const KernelCfg& g) { return g.bs == kBlock && g.xcd == 1; }
+bool default_table(const KernelCfg& g) { return g.bs == kBlock && g.xcd == kDefaultDeal; }
 
 void drop_cycle_graphs(fsgh_ctx* c)
 {
@@ -171,7 +171,7 @@ void to_public(int k, const KernelCfg& g, double ms, double ms_def, fsgh_tuning*
 
 int fsgh::ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd, bool lists)
 {
-    if (bs == kBlock && xcd == 1) return FSGH_OK;
+    if (bs == kBlock && xcd == kDefaultDeal) return FSGH_OK;
     const int32_t key = table_key(elem, bs, xcd);
     if (c->tables.count(key)) return (elem && lists) ? build_node_stage(c, bs) : FSGH_OK;
     std::vector<BlockItem> tab;

@@ -294,9 +294,9 @@ typedef struct fsgh_tuning {
     int32_t kernel;       /* FSGH_KERNEL_* */
     int32_t block;        /* 64, 128, 256 (default); 512 for the two node kernels */
     int32_t min_waves;    /* 1 (default) or 3 (force and energy kernels only) */
-    int32_t xcd_order;    /* 1 (default): one contiguous range of work items per XCD;
-                             0: launch order; k = 2..4096: chunks of k consecutive
-                             work items dealt round-robin to the 8 XCDs */
+    int32_t xcd_order;    /* 0: launch order; 1: one contiguous range of work items
+                             per XCD; k = 2..4096: chunks of k consecutive work
+                             items dealt round-robin to the 8 XCDs (default 16) */
     double ms;            /* fsgh_autotune: mean time per launch of this choice */
     double ms_default;    /* fsgh_autotune: the same for the default variant */
 } fsgh_tuning;
