@@ -110,7 +110,7 @@ def test_tuning_entry_points_on_host_context():
         out = (_abi.Tuning * _abi.NUM_TUNED_KERNELS)()
         assert L.fsgh_get_tuning(ctx, out, _abi.NUM_TUNED_KERNELS) == 0
         assert [(t.kernel, t.block, t.min_waves, t.xcd_order) for t in out] == \
-            [(k, 256, 1, 1) for k in range(_abi.NUM_TUNED_KERNELS)]
+            [(k, 256, 1, 16) for k in range(_abi.NUM_TUNED_KERNELS)]   # chunks of 16 per XCD
         assert L.fsgh_autotune(ctx, 1, out) == -5
         cfg = (_abi.Tuning * 1)(_abi.Tuning(_abi.KERNEL_FORCE, 128, 1, 1, 0.0, 0.0))
         assert L.fsgh_set_tuning(ctx, cfg, 1) == -5
