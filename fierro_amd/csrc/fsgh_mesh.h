@@ -55,7 +55,10 @@ void corner_csr(int64_t N, int64_t E, const int64_t* nodes_in_elem, std::vector<
 // hardware's round-robin dealing of workgroups to XCDs (physical block p runs
 // on XCD p % 8): deal = 0 keeps the logical order; 1 gives each XCD one
 // contiguous range of logical blocks; k >= 2 deals chunks of k consecutive
-// logical blocks round-robin to the XCDs.
+// logical blocks round-robin to the XCDs when the table has at least
+// kMinChunkedBlocks blocks (fewer per chunk when it has fewer than 32 k
+// blocks, so that every XCD gets at least 4 chunks), and is 1 otherwise.
+constexpr int64_t kMinChunkedBlocks = 4096;
 void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
                  std::vector<int32_t>& inst_begin, int32_t bs = kBlock, int32_t deal = 1);
 

@@ -22,16 +22,16 @@ def batch():
             P.anvil(3, 3, 10, cycles=25)]
 
 
-def run_fields(cfg=None, cycles=25, graph=True):
+def run_fields(cfg=None, cycles=25, graph=True, make=batch):
     from fierro_amd.sgh import SGHSolver
-    g = SGHSolver(batch())
+    g = SGHSolver(make())
     try:
         if cfg:
             g.set_tuning(cfg)
         g.run_begin()
         g.run(cycles, use_graph=graph)
         out = {k: g.download(k) for k in FIELDS}
-        out["status"] = [g.status(i) for i in range(3)]
+        out["status"] = [g.status(i) for i in range(g.ninst)]
         return out
     finally:
         g.close()
@@ -67,17 +67,28 @@ def test_variants_bitwise(reference, i):
     same(run_fields(cfg), reference)
 
 
+def big_batch():
+    # 292,296 elements: 4,568 work items of 64 (tables of fewer than 4,096
+    # keep one contiguous range per XCD; chunks are at most 1/32 of the table)
+    return [P.sod_tube(300, 0.2, 0.3, 1.5, cycles=25), P.sedov(66, cycles=25)]
+
+
+@pytest.fixture(scope="module")
+def big_reference():
+    return run_fields(make=big_batch)
+
+
 @pytest.mark.parametrize("chunk", [2, 3, 16])
-def test_chunk_dealing_bitwise(reference, chunk):
+def test_chunk_dealing_bitwise(big_reference, chunk):
     """Work items dealt to the XCDs in chunks of consecutive blocks (a chunk
-    count that does not divide the block count leaves uneven XCD queues): the
-    same results bit for bit."""
+    count that does not divide the block count leaves uneven XCD queues; 16
+    is cut to the table's limit): the same results bit for bit."""
     cfg = {"force": dict(block=64, min_waves=3, xcd_order=chunk),
            "energy_state_div": dict(block=64, min_waves=1, xcd_order=chunk),
            "vel_bc_pos": dict(block=64, min_waves=1, xcd_order=chunk),
            "div_extremes": dict(block=128, min_waves=1, xcd_order=chunk)}
-    same(run_fields(cfg), reference)
-    same(run_fields(cfg, graph=False), reference)
+    same(run_fields(cfg, make=big_batch), big_reference)
+    same(run_fields(cfg, graph=False, make=big_batch), big_reference)
 
 
 def test_autotune_midrun_leaves_state(reference):
