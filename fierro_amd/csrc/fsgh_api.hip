@@ -996,11 +996,16 @@ int fsgh_get_force_sgh(fsgh_ctx* c, double fuzz, double small, double dt, double
     if (int rc = ready(c)) return rc;
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
-    timed(c, "div_extremes", [&] { launch_div_extremes(m, s, nullptr, c->stream); });
+    // the 3D strength-free kernels run in the context's tuned launch
+    // configuration, as in the fused loop (bit-identical in any)
+    const bool tuned = c->dims == 3 && !s.sig1;
+    const DMesh md = tuned ? tuned_mesh(c, kTunedDivExtremes) : m;
+    const DMesh mf = tuned ? tuned_mesh(c, kTunedForce) : m;
+    timed(c, "div_extremes", [&] { launch_div_extremes(md, s, nullptr, c->stream); });
     timed(c, "force", [&] {
         if (c->dims == 2) launch_rz_force(m, s, nullptr, fuzz, small, dt, rk_alpha, c->stream);   // K/force_sgh.cpp:407-808
         else if (s.sig1) launch_force_stress(m, s, nullptr, fuzz, small, dt, rk_alpha, c->stream);
-        else launch_force(m, s, nullptr, fuzz, small, c->stream);
+        else launch_force(mf, s, nullptr, fuzz, small, c->stream);
     });
     OP_EPILOGUE(c);
 }

@@ -91,6 +91,40 @@ def test_chunk_dealing_bitwise(big_reference, chunk):
     same(run_fields(cfg, graph=False, make=big_batch), big_reference)
 
 
+def test_operator_path_under_tuning():
+    """The per-stage force operator (and its divergence-extremes prepass) run
+    in the context's launch configuration: the reference's cycle through the
+    operator API gives the same fields bit for bit under any configuration."""
+    from fierro_amd.sgh import SGHSolver
+    from fierro_amd.tuning import TABLE
+
+    def ops(cfg):
+        spec = P.sedov(40, cycles=6)
+        r = spec.run
+        g = SGHSolver(spec)
+        try:
+            if cfg:
+                g.set_tuning(cfg)
+            t, dt = 0.0, r["dt_start"]
+            for _ in range(6):
+                dt = g.get_timestep(t, r["graphics_dt_ival"], r["time_final"], r["dt_max"],
+                                    r["dt_min"], r["dt_cfl"], dt, r["fuzz"])
+                g.rk_init()
+                for stage in range(2):
+                    g.stage(dt, 1.0 / (2.0 - stage))
+                t += dt
+            return {k: g.download(k) for k in FIELDS}
+        finally:
+            g.close()
+
+    ref = ops(None)
+    for cfg in (TABLE, {"force": dict(block=64, min_waves=3, xcd_order=0),
+                        "div_extremes": dict(block=512, min_waves=1, xcd_order=1)}):
+        got = ops(cfg)
+        for k in FIELDS:
+            assert np.array_equal(got[k], ref[k]), k
+
+
 def test_autotune_midrun_leaves_state(reference):
     from fierro_amd.sgh import SGHSolver
     g = SGHSolver(batch())
