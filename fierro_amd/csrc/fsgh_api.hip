@@ -1017,7 +1017,11 @@ int fsgh_update_velocity_sgh(fsgh_ctx* c, double rk_alpha, double dt)
     DState s = ref_state(c);
     timed(c, "update_velocity", [&] {
         if (c->dims == 2) launch_rz_update_velocity(m, s, rk_alpha, dt, c->stream);
-        else launch_update_velocity(m, s, rk_alpha, dt, c->stream);
+        else if (s.sig1) launch_update_velocity(m, s, rk_alpha, dt, c->stream);
+        // the node kernel of the fused loop with its corner forces staged in
+        // LDS, in the tuned launch configuration, without boundary and position
+        else launch_update_velocity_staged(tuned_mesh(c, kTunedVelBcPos), s, rk_alpha, dt,
+                                           c->stream);
     });
     OP_EPILOGUE(c);
 }

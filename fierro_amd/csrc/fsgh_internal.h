@@ -139,6 +139,9 @@ void launch_update_state_stress(const DMesh&, const DState&, double dt, double a
                                 hipStream_t);
 void launch_div_extremes(const DMesh&, const DState&, const InstCtl*, hipStream_t);
 void launch_update_velocity(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
+// the same update through the staged node kernel (mesh with sorted corner lists)
+void launch_update_velocity_staged(const DMesh&, const DState&, double alpha, double dt,
+                                   hipStream_t);
 void launch_boundary_velocity(const int32_t* nodes, const uint8_t* masks, int32_t count,
                               const DState&, hipStream_t);
 void launch_update_energy(const DMesh&, const DState&, double alpha, double dt, hipStream_t);

@@ -765,13 +765,19 @@ __global__ __launch_bounds__(1024) void k_cycle_dt(InstCtl* ctl, int32_t ninst,
 
 // velocity + boundary + position, node-centric (K/momentum.cpp:23-49,
 // K/boundary.cpp:16-55, K/geometry.cpp:17-24)
-template <int BS>
+// FULL = false: the velocity update alone (the update_velocity_sgh operator,
+// K/momentum.cpp:23-49) with dt given by the caller (ctl == nullptr); the
+// boundary and position operators then run separately.
+template <int BS, bool FULL = true>
 __global__ __launch_bounds__(BS) void k_vel_bc_pos(DMesh m, DState s, const InstCtl* ctl,
-                                                   double alpha)
+                                                   double alpha, double dt_arg = 0.0)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
-    const InstCtl& c = ctl[bi.inst];
-    if (c.stopped) return;
+    double dt = dt_arg;
+    if (ctl != nullptr) {
+        if (ctl[bi.inst].stopped) return;
+        dt = ctl[bi.inst].dt;
+    }
     const int n = bi.start + static_cast<int>(threadIdx.x);
     // The block's corner forces in LDS, loaded in ascending corner order (the
     // block's sorted corner list): consecutive lanes read consecutive doubles
@@ -811,13 +817,20 @@ __global__ __launch_bounds__(BS) void k_vel_bc_pos(DMesh m, DState s, const Inst
         __syncthreads();
     }
     if (n >= bi.end) return;
-    const double dt = c.dt;
     double f[3], v0[3], x0[3], v1[3], x1[3];
     if (staged) lds_node_force(m, sF, n, f[0], f[1], f[2]);
     else gather_node_force(m, s.F, n, f[0], f[1], f[2]);
     v0[0] = s.u0[n];
     v0[1] = s.v0[n];
     v0[2] = s.w0[n];
+    if (!FULL) {
+        // v1 = v0 + alpha dt F / m, as node_update evaluates it
+        const double mass = s.nmass[n];
+        s.u1[n] = v0[0] + alpha * dt * f[0] / mass;
+        s.v1[n] = v0[1] + alpha * dt * f[1] / mass;
+        s.w1[n] = v0[2] + alpha * dt * f[2] / mass;
+        return;
+    }
     x0[0] = s.x0[n];
     x0[1] = s.y0[n];
     x0[2] = s.z0[n];
@@ -1299,6 +1312,19 @@ void launch_update_velocity(const DMesh& m, const DState& s, double alpha, doubl
     if (m.n_nblk)
         hipLaunchKernelGGL(k_update_velocity, dim3(m.n_nblk), dim3(kBlock), 0, st, m, s, alpha, dt);
 }
+void launch_update_velocity_staged(const DMesh& m, const DState& s, double alpha, double dt,
+                                   hipStream_t st)
+{
+    if (!m.n_nblk) return;
+#define FSGH_VO(B)                                                                          \
+    if (m.bs == B) {                                                                       \
+        hipLaunchKernelGGL((k_vel_bc_pos<B, false>), dim3(m.n_nblk), dim3(B), 0, st, m, s,   \
+                           nullptr, alpha, dt);                                            \
+        return;                                                                            \
+    }
+    FSGH_LIGHT_VARIANTS(FSGH_VO)
+#undef FSGH_VO
+}
 void launch_boundary_velocity(const int32_t* nodes, const uint8_t* masks, int32_t count,
                               const DState& s, hipStream_t st)
 {
@@ -1356,8 +1382,8 @@ void launch_vel_bc_pos(const DMesh& m, const DState& s, const InstCtl* ctl, doub
     if (!m.n_nblk) return;
 #define FSGH_VL(B)                                                                          \
     if (m.bs == B) {                                                                       \
-        hipLaunchKernelGGL((k_vel_bc_pos<B>), dim3(m.n_nblk), dim3(B), 0, st, m, s, ctl,     \
-                           alpha);                                                         \
+        hipLaunchKernelGGL((k_vel_bc_pos<B, true>), dim3(m.n_nblk), dim3(B), 0, st, m, s,    \
+                           ctl, alpha, 0.0);                                               \
         return;                                                                            \
     }
     FSGH_LIGHT_VARIANTS(FSGH_VL)
