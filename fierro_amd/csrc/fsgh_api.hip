@@ -1042,7 +1042,10 @@ int fsgh_update_energy_sgh(fsgh_ctx* c, double rk_alpha, double dt)
     DState s = ref_state(c);
     timed(c, "update_energy", [&] {
         if (c->dims == 2) launch_rz_update_energy(m, s, rk_alpha, dt, c->stream);
-        else launch_update_energy(m, s, rk_alpha, dt, c->stream);
+        else if (s.sig1) launch_update_energy(m, s, rk_alpha, dt, c->stream);
+        // staged like the fused loop's energy kernel, in its tuned partition
+        else launch_update_energy_staged(tuned_mesh(c, kTunedEnergy), s, rk_alpha, dt,
+                                         c->stream);
     });
     OP_EPILOGUE(c);
 }

@@ -621,6 +621,85 @@ __global__ __launch_bounds__(kBlock) void k_update_energy(DMesh m, DState s, dou
     s.sie1[e] = energy_update(f, u1, v1, w1, u0, v0, w0, s.sie0[e], s.emass[e], alpha, dt);
 }
 
+// update_energy_sgh (K/energy_sgh.cpp:16-52) as the fused loop's energy
+// kernel reads its inputs: the block's nodes staged in LDS as their mean
+// stage velocities half_sum(v1, v0) (list indices requested first), the
+// wave's corner-force rows read as one coalesced stretch; the same
+// arithmetic as k_update_energy.  Blocks over the LDS capacity gather.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_update_energy_staged(DMesh m, DState s, double alpha,
+                                                             double dt)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    constexpr int CAP = stage_cap<BS>();
+    __shared__ double sd[3 * CAP];
+    __shared__ double2 sfw[(BS / 64) * 32 * 13];
+    int q0, nn;
+    stage_range(m, bi, CAP, q0, nn);
+    const bool staged = nn > 0 && nn <= CAP;
+    constexpr int R = (CAP + BS - 1) / BS;
+    uint32_t nid[R];
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            nid[r] = static_cast<uint32_t>(m.sn_nodes[q0 + (i < nn ? i : nn - 1)]);
+        }
+    }
+    const int w0e = bi.start + (static_cast<int>(threadIdx.x) & ~63);
+    const bool wave_live = w0e < bi.end;   // wave-uniform
+    double2 ft[12];
+    wave_issue_forces(s.F, wave_live ? w0e : bi.start, wave_live ? bi.end - w0e : 1, ft);
+    const bool live = e < bi.end;
+    const int ec = live ? e : bi.start;    // in bounds; unused past the item
+    const double emass = s.emass[ec];
+    const double sie0 = s.sie0[ec];
+    if (staged) {
+        double g[R][6];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (wave_in_list(r * BS, nn)) {
+                const uint32_t n = nid[r];
+                g[r][0] = s.u1[n];
+                g[r][1] = s.v1[n];
+                g[r][2] = s.w1[n];
+                g[r][3] = s.u0[n];
+                g[r][4] = s.v0[n];
+                g[r][5] = s.w0[n];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            if (i < nn) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) sd[k * CAP + i] = half_sum(g[r][k], g[r][3 + k]);
+            }
+        }
+        __syncthreads();
+    }
+    double f[24];
+    if (wave_live) wave_scatter_forces(ft, sfw + (threadIdx.x >> 6) * 32 * 13, f);
+    if (!live) return;
+    double sie;
+    if (staged) {
+        int32_t nd[8];
+        load_lconn(m.sn_lconn, e, nd);
+        double hu[8], hv[8], hw[8];
+        lds_gather3(sd, sd + CAP, sd + 2 * CAP, nd, hu, hv, hw);
+        sie = energy_update_h(f, hu, hv, hw, sie0, emass, alpha, dt);
+    } else {
+        int32_t nd[8];
+        load_conn(m.conn, e, nd);
+        double u1[8], v1[8], w1[8], u0[8], v0[8], w0[8];
+        gather3(s.u1, s.v1, s.w1, nd, u1, v1, w1);
+        gather3(s.u0, s.v0, s.w0, nd, u0, v0, w0);
+        sie = energy_update(f, u1, v1, w1, u0, v0, w0, sie0, emass, alpha, dt);
+    }
+    s.sie1[e] = sie;
+}
+
 __global__ __launch_bounds__(kBlock) void k_update_position(DMesh m, DState s, double alpha,
                                                             double dt)
 {
@@ -1336,6 +1415,22 @@ void launch_update_energy(const DMesh& m, const DState& s, double alpha, double 
 {
     if (m.n_eblk)
         hipLaunchKernelGGL(k_update_energy, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s, alpha, dt);
+}
+void launch_update_energy_staged(const DMesh& m, const DState& s, double alpha, double dt,
+                                 hipStream_t st)
+{
+    if (!m.n_eblk) return;
+    if (m.bs == 64)
+        hipLaunchKernelGGL((k_update_energy_staged<64>), dim3(m.n_eblk), dim3(64), 0, st, m, s,
+                           alpha, dt);
+    else if (m.bs == 128)
+        hipLaunchKernelGGL((k_update_energy_staged<128>), dim3(m.n_eblk), dim3(128), 0, st, m,
+                           s, alpha, dt);
+    else if (m.bs == 256)
+        hipLaunchKernelGGL((k_update_energy_staged<256>), dim3(m.n_eblk), dim3(256), 0, st, m,
+                           s, alpha, dt);
+    else
+        launch_update_energy(m, s, alpha, dt, st);
 }
 void launch_update_position(const DMesh& m, const DState& s, double alpha, double dt,
                             hipStream_t st)
