@@ -985,7 +985,8 @@ int fsgh_get_divergence(fsgh_ctx* c)
     DState s = ref_state(c);
     timed(c, "get_divergence", [&] {
         if (c->dims == 2) launch_rz_get_divergence(m, s, c->stream);   // K/momentum.cpp:298-378
-        else launch_get_divergence(m, s, c->stream);
+        // nodes staged in LDS, in the force kernel's tuned partition
+        else launch_get_divergence_staged(tuned_mesh(c, kTunedForce), s, c->stream);
     });
     OP_EPILOGUE(c);
 }

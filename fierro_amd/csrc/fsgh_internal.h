@@ -139,6 +139,8 @@ void launch_update_state_stress(const DMesh&, const DState&, double dt, double a
                                 hipStream_t);
 void launch_div_extremes(const DMesh&, const DState&, const InstCtl*, hipStream_t);
 void launch_update_velocity(const DMesh&, const DState&, double alpha, double dt, hipStream_t);
+// the divergence with the block's nodes staged in LDS (mesh with node lists)
+void launch_get_divergence_staged(const DMesh&, const DState&, hipStream_t);
 // the energy update with the block's nodes staged in LDS (mesh with node lists)
 void launch_update_energy_staged(const DMesh&, const DState&, double alpha, double dt,
                                  hipStream_t);

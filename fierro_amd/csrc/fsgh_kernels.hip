@@ -453,6 +453,67 @@ __global__ __launch_bounds__(kBlock) void k_get_divergence(DMesh m, DState s)
     s.div[e] = divergence(B, u, v, w, s.vol[e]);
 }
 
+// get_divergence with the block's nodes (x, y, z, u, v, w) staged in LDS,
+// list indices requested first; the same arithmetic as k_get_divergence.
+// Blocks over the LDS capacity gather.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_get_divergence_staged(DMesh m, DState s)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    constexpr int CAP = stage_cap<BS>();
+    __shared__ double sd[6 * CAP];
+    int q0, nn;
+    stage_range(m, bi, CAP, q0, nn);
+    const bool staged = nn > 0 && nn <= CAP;
+    if (staged) {
+        constexpr int R = (CAP + BS - 1) / BS;
+        uint32_t nid[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            nid[r] = static_cast<uint32_t>(m.sn_nodes[q0 + (i < nn ? i : nn - 1)]);
+        }
+        double g[R][6];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (wave_in_list(r * BS, nn)) {
+                const uint32_t n = nid[r];
+                g[r][0] = s.x1[n];
+                g[r][1] = s.y1[n];
+                g[r][2] = s.z1[n];
+                g[r][3] = s.u1[n];
+                g[r][4] = s.v1[n];
+                g[r][5] = s.w1[n];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * BS + static_cast<int>(threadIdx.x);
+            if (i < nn) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sd[k * CAP + i] = g[r][k];
+            }
+        }
+        __syncthreads();
+    }
+    if (e >= bi.end) return;
+    int32_t nd[8];
+    double x[8], y[8], z[8], u[8], v[8], w[8], B[24];
+    if (staged) {
+        load_lconn(m.sn_lconn, e, nd);
+        lds_gather3(sd, sd + CAP, sd + 2 * CAP, nd, x, y, z);
+        hex_bmatrix(x, y, z, B);
+        lds_gather3(sd + 3 * CAP, sd + 4 * CAP, sd + 5 * CAP, nd, u, v, w);
+    } else {
+        load_conn(m.conn, e, nd);
+        gather3(s.x1, s.y1, s.z1, nd, x, y, z);
+        hex_bmatrix(x, y, z, B);
+        gather3(s.u1, s.v1, s.w1, nd, u, v, w);
+    }
+    s.div[e] = divergence(B, u, v, w, s.vol[e]);
+}
+
 // K/momentum.cpp:23-49: CSR gather in corners_in_node order, no atomics.
 // The first eight entries are fetched with independent loads (indices clamped
 // to the list, so every load is in bounds and none waits on a branch), then
@@ -1343,6 +1404,20 @@ void launch_get_vol(const DMesh& m, const DState& s, hipStream_t st)
 void launch_get_divergence(const DMesh& m, const DState& s, hipStream_t st)
 {
     if (m.n_eblk) hipLaunchKernelGGL(k_get_divergence, dim3(m.n_eblk), dim3(kBlock), 0, st, m, s);
+}
+void launch_get_divergence_staged(const DMesh& m, const DState& s, hipStream_t st)
+{
+    if (!m.n_eblk) return;
+    if (m.bs == 64)
+        hipLaunchKernelGGL((k_get_divergence_staged<64>), dim3(m.n_eblk), dim3(64), 0, st, m, s);
+    else if (m.bs == 128)
+        hipLaunchKernelGGL((k_get_divergence_staged<128>), dim3(m.n_eblk), dim3(128), 0, st, m,
+                           s);
+    else if (m.bs == 256)
+        hipLaunchKernelGGL((k_get_divergence_staged<256>), dim3(m.n_eblk), dim3(256), 0, st, m,
+                           s);
+    else
+        launch_get_divergence(m, s, st);
 }
 // Variant dispatch of the tuned kernels: the work tables in `m` were built
 // for workgroups of m.bs entries; m.minw selects the occupancy floor.
