@@ -52,6 +52,17 @@ def misfit(den, bins, target, nbins=NBINS):
     return float(np.sum(r * r)), seed
 
 
+def deposit_sie(spec):
+    """Specific internal energy of the energy deposit: the blast's last fill,
+    a sphere around the origin holding only the origin element."""
+    from fierro_amd import problems as P
+
+    f = spec.fills[-1]
+    if f["volume"] != P.SPHERE:
+        raise ValueError(f"{spec.name}: the last fill is not the energy deposit")
+    return float(f["sie"])
+
+
 def origin_element(coords, conn):
     """The element holding the deposited energy (centroid nearest the origin)."""
     return int(np.argmin(np.linalg.norm(centroids(coords, conn), axis=1)))
@@ -64,6 +75,20 @@ def target_profile(Solver, spec, cycles, nbins=NBINS, rmax=RMAX):
     prof, _ = profile(g.download("elem_den"), bin_index(g.coords0, g.conn, nbins, rmax), nbins)
     g.close()
     return prof
+
+
+def objective(g, target, cycles, dt_replay=None):
+    """J of every instance of solver g after ``cycles`` cycles from the set-up
+    state (``dt_replay`` [cycles][instances] freezes the steps)."""
+    g.forward(cycles, checkpoint_every=cycles, dt_replay=dt_replay)
+    den = g.download("elem_den")
+    J = np.zeros(g.ninst)
+    for i in range(g.ninst):
+        e0, e1 = g.elem_off[i], g.elem_off[i + 1]
+        n0 = g.node_off[i]
+        bins = bin_index(g.coords0[n0:g.node_off[i + 1]], g.conn[e0:e1] - n0)
+        J[i], _ = misfit(den[e0:e1], bins, target)
+    return J
 
 
 def gradient(g, params, target, cycles, checkpoint_every=10, dt_replay=None):
@@ -88,10 +113,11 @@ def gradient(g, params, target, cycles, checkpoint_every=10, dt_replay=None):
     dE0, dgam, drho = np.zeros(g.ninst), np.zeros(g.ninst), np.zeros(g.ninst)
     for i, (e0, e1, o, x0, conn) in enumerate(meta):
         n, E0, gamma, rho0 = params[i]
-        h = 1.2 / n
-        V = h * h * h
+        # the deposit is e = E0 / (rho0 V) for whatever volume V the fill used:
+        # de/dE0 = e / E0 and de/drho0 = -e / rho0
+        sie_o = deposit_sie(g.specs[i])
         g_sie_o = gr["sie0"][e0 + o]
-        dE0[i] = g_sie_o / (rho0 * V)
+        dE0[i] = g_sie_o * sie_o / E0
         dgam[i] = float(np.sum(gr["gamma"][e0:e1]))
-        drho[i] = float(np.sum(gr["den0"][e0:e1])) - g_sie_o * E0 / (rho0 * rho0 * V)
+        drho[i] = float(np.sum(gr["den0"][e0:e1])) - g_sie_o * sie_o / rho0
     return J, dE0, dgam, drho, g.dt_history()

@@ -169,8 +169,19 @@ def _gas(gamma, csmin=1.0e-14, nmat=1):
 def sedov(n=32, q2=1.333, dt_cfl=0.5, cycles=200, gamma=5.0 / 3.0):
     """Point-energy blast on [0,1.2]^3 with n^3 hexes: rho=1, u=0, e=1e-10
     except the origin element, e=0.25/V (IE0 = 0.25); symmetry (reflected)
-    planes x=y=z=0; q1=1, q2 as given, q1ex=1, q2ex=0 (K/input.cpp:112-165);
-    RK2, CFL dt_cfl, dt_start 1e-5, fixed number of cycles."""
+    planes x=y=z=0; q1=1, q2 as given, q1ex=1, q2ex=0; RK2, CFL dt_cfl,
+    dt_start 1e-5, a fixed number of cycles.
+
+    This is the benchmark variant of the blast (the plain-mesh workloads at
+    32^3 and 256^3 are quoted on it), not the reference's compiled-in
+    Sedov3D input, which ``reference_sedov`` reproduces.  It differs from
+    K/input.cpp:105-202 in four ways: the energy goes into the origin
+    element only (sphere fill of radius h, e = 0.25/V) instead of the sphere
+    of radius 1.2/8 with sie = 963.652344 (1.2/30)^3 / r^3; q2 = 1.333 instead
+    of 1.3333; three reflected planes instead of six (the blast never reaches
+    x, y, z = 1.2 within the cycles run); and time_final 1e16 with a cycle
+    limit instead of time_final 1.0.  The material, EOS, q1 / q1ex / q2ex and
+    the time-step controls are the reference's."""
     h = 1.2 / n
     vol0 = h * h * h
     spec = ProblemSpec(
@@ -310,15 +321,24 @@ def sod_rz(nx=100, ny=10, cycles=200, dt_cfl=0.4):
     return spec
 
 
-def reference_sedov(mesh_path):
-    """The reference's default compiled-in problem (K/input.cpp:69-202 with
-    K/driver.cpp defaults) on a given .geo mesh: 6 reflected planes of the
-    [0,1.2]^3 box, sphere fill of radius 1.2/8, q2 = 1.3333, CFL 0.4."""
+def reference_sedov(mesh_path=None, n=None, cycles=100000):
+    """The reference's default compiled-in problem, Sedov3D (K/input.cpp:69-202
+    with the K/driver.cpp:35-62 defaults): 6 reflected planes of the
+    [0,1.2]^3 box, sphere fill of radius 1.2/8 with
+    sie = 963.652344 (1.2/30)^3 / (1.2/8)^3, q1 = q1ex = 1, q2 = 1.3333,
+    q2ex = 0, gamma 5/3, csmin 1e-14, CFL 0.4, dt_start 1e-5, time_final 1.0,
+    graphics every 0.25, cycle_stop 100000.  The mesh is a .geo file
+    (``mesh_path``, e.g. the reference's mesh_Sedov_32.geo) or the generated
+    n^3 box with the same numbering (``n``); ``cycles`` lowers cycle_stop."""
     r2 = 1.2 / 8.0
     sie = (963.652344 * (1.2 / 30.0) ** 3) / (r2 ** 3)
+    if mesh_path is not None:
+        mesh = dict(kind="geo", path=mesh_path)
+    else:
+        mesh = dict(kind="box", nx=n, ny=n, nz=n, lx=12.0, ly=12.0, lz=12.0, lden=10.0)
     spec = ProblemSpec(
-        name="reference_sedov3d",
-        mesh=dict(kind="geo", path=mesh_path),
+        name="reference_sedov3d" if n is None else f"reference_sedov3d_{n}",
+        mesh=mesh,
         materials=[(1.0, 1.0, 1.3333, 0.0)],
         state_vars=_gas(5.0 / 3.0),
         mat_num_state_vars=[3],
@@ -328,7 +348,7 @@ def reference_sedov(mesh_path):
              (X_PLANE, 1.2, REFLECTED), (Y_PLANE, 1.2, REFLECTED), (Z_PLANE, 1.2, REFLECTED)],
     )
     spec.run.update(time_final=1.0, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
-                    graphics_dt_ival=0.25, graphics_cyc_ival=1000000, cycle_stop=100000)
+                    graphics_dt_ival=0.25, graphics_cyc_ival=1000000, cycle_stop=cycles)
     return spec
 
 

@@ -67,7 +67,8 @@ class OrcRunParams(C.Structure):
                 ("graphics_time", C.c_double), ("graphics_dt_ival", C.c_double),
                 ("graphics_cyc_ival", C.c_int64), ("cycle_stop", C.c_int64),
                 ("rk_num_stages", C.c_int64), ("fuzz", C.c_double),
-                ("small", C.c_double), ("dt_replay", C.c_void_p), ("dt_record", C.c_void_p)]
+                ("small", C.c_double), ("dt_replay", C.c_void_p), ("dt_record", C.c_void_p),
+                ("write_record", C.c_void_p)]
 
 
 class OrcRunResult(C.Structure):
@@ -80,9 +81,54 @@ class OrcRunResult(C.Structure):
 _LIBS: dict[str, C.CDLL] = {}
 
 
+def _run_params(run, cycle_stop, dt_replay, record_dt, rk_num_stages, record_writes):
+    """orc_run_params for one sgh_solve call (start time 0, first graphics
+    time = graphics_dt_ival as K/driver.cpp:39 sets it).  The returned
+    arrays must stay alive for the call."""
+    r = dict(run)
+    if cycle_stop is not None:
+        r["cycle_stop"] = cycle_stop
+    ncyc = int(r["cycle_stop"])
+    rep = None if dt_replay is None else np.ascontiguousarray(dt_replay, np.float64)
+    if rep is not None and rep.size < ncyc:
+        raise ValueError("dt_replay shorter than cycle_stop")
+    rec = np.zeros(ncyc) if record_dt else None
+    wrec = np.zeros(ncyc, np.int64) if record_writes else None
+    p = OrcRunParams(0.0, r["time_final"], r["dt_start"], r["dt_max"], r["dt_min"],
+                     r["dt_cfl"], r["graphics_dt_ival"], r["graphics_dt_ival"],
+                     int(r["graphics_cyc_ival"]), ncyc, int(rk_num_stages), r["fuzz"],
+                     r["small"], _ptr(rep), _ptr(rec), _ptr(wrec))
+    p._keep = (rep, rec, wrec)
+    return p, rec, wrec
+
+
+def _run_result(res, rec, wrec):
+    out = {k: getattr(res, k) for k, _ in OrcRunResult._fields_}
+    if rec is not None:
+        out["dt_history"] = rec[:res.cycles]
+    if wrec is not None:
+        out["write_cycles"] = [int(c) + 1 for c in np.nonzero(wrec[:res.cycles])[0]]
+    return out
+
+
 def build() -> None:
     """Compile the oracle libraries (gcc, see oracle/Makefile)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load_library(path: str) -> C.CDLL:
+    """Open an oracle build and declare the non-int signatures."""
+    L = C.CDLL(path)
+    L.orc_get_vol_hex.restype = C.c_double
+    L.orc_get_vol_hex.argtypes = [f64p, f64p, f64p]
+    L.orc_get_bmatrix.argtypes = [f64p, f64p, f64p, f64p]
+    L.orc_build_elem_elem_connectivity.restype = C.c_int64
+    L.orc_build_patch_connectivity.restype = C.c_int64
+    L.orc_build_bdy_sets.restype = C.c_int64
+    L.orc_read_ensight.restype = C.c_int
+    L.orc_rk_alpha.restype = C.c_double
+    L.orc_rk_alpha.argtypes = [C.c_int64, C.c_int64]
+    return L
 
 
 def lib(fast: bool = False) -> C.CDLL:
@@ -91,15 +137,7 @@ def lib(fast: bool = False) -> C.CDLL:
         path = os.path.join(_HERE, name)
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
-        L.orc_get_vol_hex.restype = C.c_double
-        L.orc_get_vol_hex.argtypes = [f64p, f64p, f64p]
-        L.orc_get_bmatrix.argtypes = [f64p, f64p, f64p, f64p]
-        L.orc_build_elem_elem_connectivity.restype = C.c_int64
-        L.orc_build_patch_connectivity.restype = C.c_int64
-        L.orc_build_bdy_sets.restype = C.c_int64
-        L.orc_read_ensight.restype = C.c_int
-        _LIBS[name] = L
+        _LIBS[name] = load_library(path)
     return _LIBS[name]
 
 
@@ -301,26 +339,18 @@ class OracleProblem:
                                   C.byref(ke))
         return ie.value, ke.value
 
-    def solve(self, cycle_stop=None, dt_replay=None, record_dt=False):
-        r = dict(self.spec.run)
-        if cycle_stop is not None:
-            r["cycle_stop"] = cycle_stop
-        ncyc = int(r["cycle_stop"])
-        rep = None if dt_replay is None else np.ascontiguousarray(dt_replay, np.float64)
-        if rep is not None and rep.size < ncyc:
-            raise ValueError("dt_replay shorter than cycle_stop")
-        rec = np.zeros(ncyc) if record_dt else None
-        p = OrcRunParams(0.0, r["time_final"], r["dt_start"], r["dt_max"], r["dt_min"],
-                         r["dt_cfl"], r["graphics_dt_ival"], r["graphics_dt_ival"],
-                         int(r["graphics_cyc_ival"]), ncyc, 2, r["fuzz"],
-                         r["small"], _ptr(rep), _ptr(rec))
+    def solve(self, cycle_stop=None, dt_replay=None, record_dt=False, rk_num_stages=2,
+              record_writes=False):
+        """sgh_solve (K/sgh_solve.cpp:125-634).  ``record_dt`` returns the dt of
+        every cycle as ``dt_history``; ``record_writes`` returns the cycles
+        (1-based count of cycles done) after which graphics output is written
+        as ``write_cycles``."""
+        p, rec, wrec = _run_params(self.spec.run, cycle_stop, dt_replay, record_dt,
+                                   rk_num_stages, record_writes)
         res = OrcRunResult()
         self.L.orc_sgh_solve(C.byref(self._mesh), C.byref(self._state), self._mat, self._bc,
                              C.byref(p), C.byref(res))
-        out = {k: getattr(res, k) for k, _ in OrcRunResult._fields_}
-        if record_dt:
-            out["dt_history"] = rec[:res.cycles]
-        return out
+        return _run_result(res, rec, wrec)
 
     def stage(self, dt, rk_alpha):
         """One RK stage in the reference order (K/sgh_solve.cpp:355-486)."""
@@ -569,24 +599,15 @@ class OracleRZ:
         self.L.orz_energy_tallies(*self._ms(), C.byref(ie), C.byref(ke))
         return ie.value, ke.value
 
-    def solve(self, cycle_stop=None, dt_replay=None, record_dt=False):
-        r = dict(self.spec.run)
-        if cycle_stop is not None:
-            r["cycle_stop"] = cycle_stop
-        ncyc = int(r["cycle_stop"])
-        rep = None if dt_replay is None else np.ascontiguousarray(dt_replay, np.float64)
-        rec = np.zeros(ncyc) if record_dt else None
-        p = OrcRunParams(0.0, r["time_final"], r["dt_start"], r["dt_max"], r["dt_min"],
-                         r["dt_cfl"], r["graphics_dt_ival"], r["graphics_dt_ival"],
-                         int(r["graphics_cyc_ival"]), ncyc, 2, r["fuzz"], r["small"],
-                         _ptr(rep), _ptr(rec))
+    def solve(self, cycle_stop=None, dt_replay=None, record_dt=False, rk_num_stages=2,
+              record_writes=False):
+        """The 2D branch of sgh_solve; arguments as ``OracleProblem.solve``."""
+        p, rec, wrec = _run_params(self.spec.run, cycle_stop, dt_replay, record_dt,
+                                   rk_num_stages, record_writes)
         res = OrcRunResult()
         self.L.orz_sgh_solve(*self._ms(), self._mat, self._bc, C.byref(p),
-                             C.c_double(r["tiny"]), C.byref(res))
-        out = {k: getattr(res, k) for k, _ in OrcRunResult._fields_}
-        if record_dt:
-            out["dt_history"] = rec[:res.cycles]
-        return out
+                             C.c_double(self.spec.run["tiny"]), C.byref(res))
+        return _run_result(res, rec, wrec)
 
     def stage(self, dt, rk_alpha):
         """One RK stage of the 2D branch (K/sgh_solve.cpp:355-581)."""

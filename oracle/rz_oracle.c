@@ -442,7 +442,7 @@ void orz_sgh_solve(const orz_mesh *m, orc_state *s, const orc_material *mat,
         if (p->dt_record) p->dt_record[cycle] = dt;
         orz_rk_init(m, s);
         for (int64_t stage = 0; stage < p->rk_num_stages; ++stage) {
-            double rk_alpha = 1.0 / ((double)p->rk_num_stages - (double)stage);
+            double rk_alpha = orc_rk_alpha(p->rk_num_stages, stage);
             orz_get_divergence(m, s);
             orz_get_force(m, s, mat, p->fuzz, p->small, dt, rk_alpha);
             orz_update_velocity(m, s, rk_alpha, dt);
@@ -460,6 +460,7 @@ void orz_sgh_solve(const orz_mesh *m, orc_state *s, const orc_material *mat,
         else if (time_value >= p->time_final) write = 1;
         else if (time_value >= graphics_time) write = 1;
         if (write) graphics_time = time_value + p->graphics_dt_ival;
+        if (p->write_record) p->write_record[cycle] = write;
         if (time_value >= p->time_final) { done = 1; break; }
     }
     r->seconds = now_seconds() - t0;

@@ -490,6 +490,26 @@ static double now_seconds(void)
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* RK coefficient (K/sgh_solve.cpp:353).  ORC_PLANTED_MISTAKE selects a
+ * deliberately wrong coefficient; only the mutant builds made by
+ * tests/test_oracle_time_integration.py define it, to show that the temporal
+ * convergence tests reject such an integrator. */
+double orc_rk_alpha(int64_t num_stages, int64_t stage)
+{
+#if defined(ORC_PLANTED_MISTAKE) && ORC_PLANTED_MISTAKE == 1
+    (void)num_stages;
+    (void)stage;
+    return 1.0;                                          /* every stage a full step */
+#elif defined(ORC_PLANTED_MISTAKE) && ORC_PLANTED_MISTAKE == 2
+    return 1.0 / ((double)num_stages + (double)stage);   /* weights 1/2, 1/3 */
+#elif defined(ORC_PLANTED_MISTAKE) && ORC_PLANTED_MISTAKE == 3
+    return 0.5 / ((double)num_stages - (double)stage);   /* weights 1/4, 1/2: second
+                                                             order, but for y' = f/2 */
+#else
+    return 1.0 / ((double)num_stages - (double)stage);
+#endif
+}
+
 /* sgh_solve (K/sgh_solve.cpp:8-692), 3D branch, outputs disabled. */
 void orc_sgh_solve(const orc_mesh *m, orc_state *s, const orc_material *mat,
                    const orc_boundary *bc, const orc_run_params *p, orc_run_result *r)
@@ -508,7 +528,7 @@ void orc_sgh_solve(const orc_mesh *m, orc_state *s, const orc_material *mat,
         if (p->dt_record) p->dt_record[cycle] = dt;
         orc_rk_init(m, s);
         for (int64_t stage = 0; stage < p->rk_num_stages; ++stage) {
-            double rk_alpha = 1.0 / ((double)p->rk_num_stages - (double)stage);
+            double rk_alpha = orc_rk_alpha(p->rk_num_stages, stage);
             orc_get_divergence(m, s);
             orc_get_force_sgh(m, s, mat, p->fuzz, p->small, dt, rk_alpha);
             orc_update_velocity_sgh(m, s, rk_alpha, dt);
@@ -525,6 +545,7 @@ void orc_sgh_solve(const orc_mesh *m, orc_state *s, const orc_material *mat,
         else if (time_value >= p->time_final) write = 1;
         else if (time_value >= graphics_time) write = 1;
         if (write) graphics_time = time_value + p->graphics_dt_ival;
+        if (p->write_record) p->write_record[cycle] = write;
         if (time_value >= p->time_final) { done = 1; break; }
     }
     r->seconds = now_seconds() - t0;

@@ -104,6 +104,8 @@ typedef struct orc_run_params {
     const double *dt_replay;  /* non-NULL: use dt_replay[cycle] instead of get_timestep
                                  (frozen time steps, for derivative checks) */
     double *dt_record;        /* non-NULL: store the dt of every cycle */
+    int64_t *write_record;    /* non-NULL: 1 for each cycle after which graphics output
+                                 is written (K/sgh_solve.cpp:594-628), else 0 */
 } orc_run_params;
 
 typedef struct orc_run_result {
@@ -145,6 +147,9 @@ void orc_get_timestep(const orc_mesh *m, orc_state *s, double time_value, double
                       double time_final, double dt_max, double dt_min, double dt_cfl,
                       double *dt, double fuzz);
 void orc_energy_tallies(const orc_mesh *m, const orc_state *s, double *ie, double *ke);
+
+/* RK coefficient of stage s of an n-stage cycle: 1/(n - s) (K/sgh_solve.cpp:353). */
+double orc_rk_alpha(int64_t num_stages, int64_t stage);
 
 /* ---- cycle driver (K/sgh_solve.cpp:63-688) ---- */
 void orc_sgh_solve(const orc_mesh *m, orc_state *s, const orc_material *mat,

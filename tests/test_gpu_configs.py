@@ -1,0 +1,156 @@
+"""GPU parity at the benchmark workloads' own sizes and cycle counts.
+
+* Sedov point blast 32^3, 200 cycles, against the oracle; and the
+  reference's own compiled-in Sedov3D input (K/input.cpp:105-202: sphere
+  fill of radius 1.2/8, six reflected planes, q2 = 1.3333, CFL 0.4) on the
+  same 32^3 box, 200 cycles.
+* The batched Sod ensemble: all 512 tubes (nx ~ U{64..1024} x 4 x 4, seed 0)
+  for 500 cycles in one ragged batch and one graph per cycle; a seeded sample
+  of instances (always the smallest and the largest tube) against the oracle
+  run per instance, and total-energy conservation of every instance.
+* The calibration sweep: the 64 blasts (48^3..96^3, seed 0) for 150 cycles
+  in one batch; the two smallest instances against the oracle; the batched
+  forward + adjoint gradient against central differences on three instances.
+
+Bars as everywhere: time to 1e-14, total energy to 1e-12 relative, L1
+density <= 1e-9 relative, every state field within 1e-9 normwise.
+"""
+import numpy as np
+import pytest
+
+from fierro_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+STATE = ["node_coords", "node_vel", "node_mass", "elem_den", "elem_pres", "elem_sspd",
+         "elem_sie", "elem_vol", "elem_mass"]
+
+
+def rel(a, b):
+    d = float(np.max(np.abs(np.asarray(a, float) - np.asarray(b, float))))
+    s = float(np.max(np.abs(b)))
+    return d / s if s > 0 else d
+
+
+def oracle_fields(pr):
+    return dict(node_coords=pr.node_coords, node_vel=pr.node_vel, node_mass=pr.node_mass,
+                elem_den=pr.elem_den, elem_pres=pr.elem_pres, elem_sspd=pr.elem_sspd,
+                elem_sie=pr.elem_sie, elem_vol=pr.elem_vol, elem_mass=pr.elem_mass)
+
+
+@pytest.fixture(scope="module")
+def Solver():
+    from fierro_amd.sgh import SGHSolver
+    return SGHSolver
+
+
+def compare_instance(tag, ro, pr, rg, fields):
+    """One instance of a GPU run (its result dict and downloaded fields in
+    caller numbering) against the oracle's run of the same instance."""
+    print(f"{tag}: cycles {rg['cycles']} t {rg['time_value']:.9e} "
+          f"dTE {abs(rg['te1'] - ro['te1']) / abs(ro['te1']):.2e}", flush=True)
+    assert rg["cycles"] == ro["cycles"]
+    assert abs(rg["time_value"] - ro["time_value"]) <= 1e-14 * ro["time_value"]
+    assert abs(rg["te1"] - ro["te1"]) <= 1e-12 * abs(ro["te1"])
+    den = fields["elem_den"]
+    l1 = float(np.sum(np.abs(den - pr.elem_den)) / np.sum(np.abs(pr.elem_den)))
+    assert l1 <= 1e-9, (tag, l1)
+    for k, v in oracle_fields(pr).items():
+        assert rel(fields[k], v) <= 1e-9, (tag, k, rel(fields[k], v))
+
+
+@pytest.mark.parametrize("which", ["baseline_blast", "reference_sedov3d"])
+def test_sedov32_200_cycles(oracle, Solver, which):
+    spec = (P.sedov(32, cycles=200) if which == "baseline_blast"
+            else P.reference_sedov(n=32, cycles=200))
+    print(f"[{which}] 32^3, 200 cycles", flush=True)
+    pr = oracle.OracleProblem(spec, fast=True)
+    ro = pr.solve()
+    assert ro["cycles"] == 200
+    g = Solver(spec, renumber=True)
+    rg = g.sgh_solve(use_graph=True)[0]
+    compare_instance(which, ro, pr, rg, {k: g.download(k) for k in STATE})
+    if which == "reference_sedov3d":
+        hot = pr.elem_sie[1] > 1.0
+        assert hot.sum() > 1            # the sphere fill covers several elements
+
+
+def sample_instances(sizes, k, seed):
+    """Indices of the smallest and largest instance plus k - 2 seeded others."""
+    order = np.argsort(sizes, kind="stable")
+    pick = [int(order[0]), int(order[-1])]
+    rng = np.random.default_rng(seed)
+    rest = [i for i in range(len(sizes)) if i not in pick]
+    pick += [int(i) for i in rng.choice(rest, size=k - 2, replace=False)]
+    return pick
+
+
+def test_sod_ensemble_512_tubes_500_cycles(oracle, Solver):
+    specs = P.sod_ensemble(512, seed=0, cycles=500)
+    print("[sod ensemble] 512 tubes, 500 cycles", flush=True)
+    g = Solver(specs, renumber=True)
+    res = g.sgh_solve(use_graph=True)
+    assert len(res) == 512
+    for i, r in enumerate(res):
+        assert r["cycles"] == 500, i
+        assert abs(r["te1"] - r["te0"]) <= 1e-12 * abs(r["te0"]), (i, r["te0"], r["te1"])
+    full = {k: g.download(k) for k in STATE}
+    sizes = [s.num_elems for s in specs]
+    for i in sample_instances(sizes, 6, seed=7):
+        e0, e1 = g.elem_off[i], g.elem_off[i + 1]
+        n0, n1 = g.node_off[i], g.node_off[i + 1]
+        mine = {}
+        for k, v in full.items():
+            if k.startswith("node"):
+                mine[k] = v[..., n0:n1, :] if v.ndim == 3 else v[n0:n1]
+            else:
+                mine[k] = v[..., e0:e1]
+        pr = oracle.OracleProblem(specs[i], fast=True)
+        ro = pr.solve()
+        compare_instance(f"tube {i} nx={specs[i].mesh['nx']}", ro, pr, res[i], mine)
+
+
+def test_calibration_sweep_64_blasts(oracle, Solver):
+    from fierro_amd import calibration as CAL
+
+    cycles = 150
+    specs, params, tspec, _ = P.blast_ensemble(64, seed=0, cycles=cycles)
+    print("[calibration] 64 blasts, 150 cycles: forward", flush=True)
+    g = Solver(specs, renumber=True)
+    res = g.sgh_solve(use_graph=True)
+    full = {k: g.download(k) for k in STATE}
+    sizes = [p[0] for p in params]
+    for i in [int(j) for j in np.argsort(sizes, kind="stable")[:2]]:
+        e0, e1 = g.elem_off[i], g.elem_off[i + 1]
+        n0, n1 = g.node_off[i], g.node_off[i + 1]
+        mine = {k: (v[..., n0:n1, :] if v.ndim == 3 else v[n0:n1]) if k.startswith("node")
+                else v[..., e0:e1] for k, v in full.items()}
+        pr = oracle.OracleProblem(specs[i], fast=True)
+        ro = pr.solve()
+        compare_instance(f"blast {i} n={params[i][0]}", ro, pr, res[i], mine)
+    g.close()
+
+    print("[calibration] batched forward + adjoint", flush=True)
+    target = CAL.target_profile(Solver, tspec, cycles)
+    g = Solver(specs, renumber=True)
+    J, dE0, dgam, drho, dt = CAL.gradient(g, params, target, cycles, checkpoint_every=10)
+    g.close()
+    assert np.all(np.isfinite([dE0, dgam, drho])) and np.all(J > 0)
+
+    # central differences of single-instance runs with the batch's frozen steps
+    # (a batched instance evolves exactly as the plain solver on it)
+    h = 1e-6
+    check = [int(j) for j in np.argsort(sizes, kind="stable")[:3]]
+    for i in check:
+        for which, grad in ((1, dE0), (2, dgam), (3, drho)):
+            Js = []
+            for sgn in (1.0, -1.0):
+                p = list(params[i])
+                p[which] += sgn * h
+                gi = Solver([P.blast(*p, cycles=cycles)], renumber=True)
+                Ji = CAL.objective(gi, target, cycles, dt_replay=dt[:, i:i + 1])
+                gi.close()
+                Js.append(Ji[0])
+            fd = (Js[0] - Js[1]) / (2 * h)
+            print(f"blast {i} param {which}: fd {fd:+.10e} adjoint {grad[i]:+.10e}", flush=True)
+            assert fd == pytest.approx(grad[i], rel=1e-5, abs=1e-9 * abs(grad[i]) + 1e-14)

@@ -285,20 +285,19 @@ def test_batched_instances_equal_plain_runs(Solver):
 
 def test_output_cadence_and_dumps(oracle, Solver, tmp_path):
     """Graphics dumps from the fused loop happen at the reference's output
-    cycles (K/sgh_solve.cpp:594-628) and hold the state of that cycle."""
+    cycles (K/sgh_solve.cpp:594-628) and hold the state of that cycle.
+    Every step is exactly dt_max = 2^-10 (dt_start = dt_max; the CFL limit is
+    far larger and the graphics clamp never binds), so with
+    graphics_dt_ival = 5 * 2^-10 and graphics_cyc_ival = 13 the outputs come,
+    by hand, after cycles 5, 10, 13 (cycle rule), 18, 23, 26 (cycle rule),
+    31, 36 and 39 (cycle rule), plus the initial dump at t = 0."""
     from fierro_amd import outputs
 
-    spec = P.noh(6, cycles=40).with_run(graphics_dt_ival=0.0005)
-    pr = oracle.OracleProblem(spec)
-    ro = pr.solve(record_dt=True)
-    # output times implied by the oracle's dt sequence and the write rule
-    t, gt, expect, cyc = 0.0, spec.run["graphics_dt_ival"], [0.0], [0]
-    for c, dt in enumerate(ro["dt_history"]):
-        t += dt
-        if (c + 1) % spec.run["graphics_cyc_ival"] == 0 and c > 0 or t >= gt:
-            expect.append(t)
-            cyc.append(c + 1)
-            gt = t + spec.run["graphics_dt_ival"]
+    h = 2.0 ** -10
+    spec = P.noh(6, cycles=40).with_run(dt_start=h, dt_max=h, graphics_dt_ival=5 * h,
+                                        graphics_cyc_ival=13)
+    cyc = [0, 5, 10, 13, 18, 23, 26, 31, 36, 39]
+    expect = [c * h for c in cyc]
     g = Solver(spec, renumber=True)
     w = outputs.EnsightWriter(str(tmp_path))
     got = []
@@ -308,8 +307,7 @@ def test_output_cadence_and_dumps(oracle, Solver, tmp_path):
         w.write(outputs.gather_fields(solver), time)
 
     g.sgh_solve(output=dump)
-    assert len(got) == len(expect) > 3
-    assert np.allclose(got, expect, rtol=1e-14, atol=0)
+    assert got == expect
     x, c = oracle.read_ensight(str(tmp_path / "ensight" / "data" /
                                    f"Outputs_SGH.{len(got) - 1:05d}.geo"))
     # the last dump holds the state after cycle cyc[-1]
