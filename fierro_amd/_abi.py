@@ -39,7 +39,9 @@ EXPORTS = [
     "fsgh_autotune", "fsgh_set_tuning", "fsgh_get_tuning",
     "fsgh_rz_extensive_mass", "fsgh_rz_update_node_mass",
     "fsgh_set_strength_params", "fsgh_get_velgrad", "fsgh_get_stage_info",
+    "fsgh_energy_noalloc",
 ]
+NOALLOC_MAX_ENTRIES = 2147221503
 
 KERNEL_DIV_EXTREMES, KERNEL_FORCE, KERNEL_VEL_BC_POS, KERNEL_ENERGY_STATE_DIV = 0, 1, 2, 3
 NUM_TUNED_KERNELS = 4
@@ -147,6 +149,7 @@ def load() -> C.CDLL:
         "fsgh_get_stage_info": [vp, i32, vp, vp],
         "fsgh_get_velgrad": [vp, vp],
         "fsgh_rz_update_node_mass": [vp, dbl],
+        "fsgh_energy_noalloc": [i32, i32, i64, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

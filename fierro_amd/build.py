@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfsgh.so")
 SOURCES = ["fsgh_kernels.hip", "fsgh_mesh.hip", "fsgh_api.hip", "fsgh_adjoint.hip",
-           "fsgh_setup.hip", "fsgh_tune.hip", "fsgh_rz.hip", "fsgh_stage.hip"]
+           "fsgh_setup.hip", "fsgh_tune.hip", "fsgh_rz.hip", "fsgh_stage.hip",
+           "fsgh_noalloc.hip"]
 ARCH = os.environ.get("FSGH_ARCH", "gfx950")
 
 FLAGS = [

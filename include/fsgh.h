@@ -226,6 +226,29 @@ int fsgh_update_state(fsgh_ctx *ctx, double dt, double rk_alpha);   /* K/propert
 /* extensive energies IE = sum M e, KE = 1/2 sum m |v|^2 of one instance
  * (K/sgh_solve.cpp:66-104; 2D: KE = 1/2 sum m R |v|^2); synchronises. */
 int fsgh_energy(fsgh_ctx *ctx, int32_t instance, double *ie, double *ke);
+/* Energy tallies of caller-held DEVICE arrays, allocating nothing.  No
+ * counterpart in the reference: this is a variant of fsgh_energy for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc -- and needs no context.
+ * Same kernels and summation order as fsgh_energy (IE = sum M e,
+ * KE = 1/2 sum m |v|^2; num_dims = 2: KE = 1/2 sum m R |v|^2 with the node
+ * radius R), so equal arrays in equal order give bitwise-equal results.
+ * Inputs: elem_mass, elem_sie [num_elems]; node_mass, node_u, node_v
+ * [num_nodes]; node_w [num_nodes] in 3D (ignored in 2D); node_radius
+ * [num_nodes] in 2D (ignored in 3D); all on hip_device.  Output: ie_ke, two
+ * doubles in device memory (IE, KE).  The call is asynchronous on hip_stream
+ * (NULL: the null stream) and never synchronises; it restores the caller's
+ * current device.  The partial sums live in one fixed 16 KiB array per GPU
+ * (1024 pairs of doubles) that the library holds until the process ends;
+ * every call rewrites all of it, and calls that overlap on one GPU race on
+ * it.  num_elems and num_nodes may not exceed FSGH_NOALLOC_MAX_ENTRIES
+ * (FSGH_E_INVALID). */
+#define FSGH_NOALLOC_MAX_ENTRIES 2147221503LL  /* 2^31 - 1 - 1024 * 256 */
+int fsgh_energy_noalloc(int32_t hip_device, int32_t num_dims, int64_t num_elems,
+                        const double *elem_mass, const double *elem_sie, int64_t num_nodes,
+                        const double *node_mass, const double *node_u, const double *node_v,
+                        const double *node_w, const double *node_radius, double *ie_ke,
+                        void *hip_stream);
 /* 2D RZ only.  The node extensive mass = areal node mass x radius, which
  * sgh_solve takes once at its start (K/sgh_solve.cpp:107-116), and the
  * per-stage areal mass update from it with the on-axis fix
