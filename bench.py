@@ -23,46 +23,104 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "FP64 SGH zone-cycles/s & %HBM roofline, plain vs batched; adjoint/fwd time"
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n, cycles, threads):
-    """The oracle (a C/OpenMP restatement of the reference operators) timed
-    on the host cores over a bounded sample of the same blast problem."""
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    os.environ.setdefault("OMP_PROC_BIND", "spread")
-    os.environ.setdefault("OMP_PLACES", "threads")
+def host_cpu():
+    """(CPU model, physical cores among the CPUs this process may run on)."""
+    model, cores, cur = "unknown", set(), {}
+    try:
+        allowed = os.sched_getaffinity(0)
+    except AttributeError:
+        allowed = set(range(os.cpu_count() or 1))
+    try:
+        with open("/proc/cpuinfo") as f:
+            blocks = f.read().strip().split("\n\n")
+        for blk in blocks:
+            cur = dict(line.split(":", 1) for line in blk.splitlines() if ":" in line)
+            cur = {k.strip(): v.strip() for k, v in cur.items()}
+            if int(cur.get("processor", -1)) in allowed:
+                model = cur.get("model name", model)
+                cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+    except (OSError, ValueError, KeyError):
+        pass
+    return model, max(len(cores), 1) if cores else len(allowed)
+
+
+def baseline_threads():
+    """Physical cores of this process's affinity, capped by the OpenMP thread
+    share the machine grants (OMP_NUM_THREADS when set)."""
+    model, phys = host_cpu()
+    cap = os.environ.get("OMP_NUM_THREADS")
+    threads = min(phys, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else phys
+    return model, phys, threads
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # noqa: E402  (test/baseline infrastructure)
+    return O
+
+
+def cpu_baseline(n, cycles, one_n=64, one_cycles=10):
+    """The oracle (a C/OpenMP restatement of the reference operators, built
+    with gcc -O3) timed on the host cores: ``cycles`` RK2 cycles of the same
+    blast at n^3 on every core this process may use (set-up excluded, as
+    K/sgh_solve.cpp:122,637-641 times only the cycle loop), and a 1-thread
+    rate on one_n^3."""
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "threads")
+    O = _oracle()
     from fierro_amd import problems as P
 
-    spec = P.sedov(n, cycles=cycles)
-    pr = O.OracleProblem(spec, fast=True)
+    model, phys, threads = baseline_threads()
+    L = O.lib(fast=True)
+    L.orc_set_num_threads(threads)
+    t = time.perf_counter()
+    pr = O.OracleProblem(P.sedov(n, cycles=cycles), fast=True)
+    log(f"cpu baseline: oracle set-up {n}^3 in {time.perf_counter() - t:.1f} s")
     r = pr.solve()
-    zc = pr.E * r["cycles"] / r["seconds"]
+    E = pr.E
+    del pr
+    zc = E * r["cycles"] / r["seconds"]
+    log(f"cpu baseline: {r['cycles']} cycles at {n}^3 on {threads} threads in "
+        f"{r['seconds']:.2f} s")
+    L.orc_set_num_threads(1)
+    p1 = O.OracleProblem(P.sedov(one_n, cycles=one_cycles), fast=True)
+    r1 = p1.solve()
+    one = p1.E * r1["cycles"] / r1["seconds"]
+    L.orc_set_num_threads(threads)
     return {"value": zc, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
-            "sample": f"sedov {n}^3 ({pr.E} zones), {r['cycles']} RK2 cycles, "
-                      f"{r['seconds']:.2f} s, oracle/liboracle_sgh_fast.so (gcc -O3, OpenMP)"}
+            "cpu_model": model, "physical_cores_in_affinity": phys,
+            "one_thread": {"value": one, "unit": "zone-cycles/s",
+                           "sample": f"sedov {one_n}^3, {r1['cycles']} cycles, "
+                                     f"{r1['seconds']:.2f} s"},
+            "sample": f"sedov {n}^3 ({E} zones), {r['cycles']} RK2 cycles, "
+                      f"{r['seconds']:.2f} s on {threads} threads, "
+                      f"oracle/liboracle_sgh_fast.so (gcc -O3, OpenMP)"}
 
 
-def cpu_baseline_rz(n, cycles, threads):
+def cpu_baseline_rz(n, cycles):
     """The RZ oracle on the host cores over a bounded sample of the 2D RZ
     blast."""
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     os.environ.setdefault("OMP_PROC_BIND", "spread")
     os.environ.setdefault("OMP_PLACES", "threads")
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # noqa: E402  (test/baseline infrastructure)
+    O = _oracle()
     from fierro_amd import problems as P
+
+    model, phys, threads = baseline_threads()
+    O.lib(fast=True).orc_set_num_threads(threads)
 
     spec = P.sedov_rz(n, cycles=cycles)
     pr = O.OracleRZ(spec, fast=True)
     r = pr.solve()
     zc = pr.E * r["cycles"] / r["seconds"]
     return {"value": zc, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "cpu_model": model,
             "sample": f"sedov_rz {n}^2 ({pr.E} zones), {r['cycles']} RK2 cycles, "
                       f"{r['seconds']:.2f} s, oracle/liboracle_sgh_fast.so (gcc -O3, OpenMP)"}
 
@@ -148,8 +206,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256, help="elements per edge")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=64)
-    ap.add_argument("--cpu-cycles", type=int, default=100)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="the plain line only (no batched, adjoint, calibration and "
+                         "operator-path records)")
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--workload",
                     choices=["sedov", "sedov-rz", "sod-ensemble", "anvil-adjoint",
                              "calibration", "sedov-operators"],
@@ -158,8 +219,8 @@ def main():
     ap.add_argument("--tune", choices=["table", "auto", "default"], default="table",
                     help="launch variants of the fused kernels: the measured table "
                          "(fierro_amd/tuning.py), a fresh fsgh_autotune, or the defaults")
-    ap.add_argument("--loop-sample", type=int, default=8,
-                    help="time the per-instance loop on every k-th instance")
+    ap.add_argument("--loop-sample", type=int, default=1,
+                    help="time the per-instance loop on every k-th instance (1: all)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
     ap.add_argument("--traffic-from", nargs=4, default=None,
@@ -181,16 +242,50 @@ def main():
         sys.exit(2)
 
     if args.workload == "sod-ensemble":
-        return ensemble(args)
+        return emit(ensemble(args))
     if args.workload == "anvil-adjoint":
-        return adjoint(args)
+        return emit(adjoint(args))
     if args.workload == "calibration":
-        return calibration(args)
+        return emit(calibration(args))
     if args.workload == "sedov-rz":
         return sedov_rz(args)
     if args.workload == "sedov-operators":
-        return sedov_operators(args)
+        return emit(sedov_operators(args))
 
+    out = plain(args)
+    if not args.no_extras:
+        # the metric's other parts, each on its own workload (BASELINE configs
+        # 3, 4 and 5) and the drop-in operator path at the plain mesh
+        out["batched"] = part(ensemble(args, steps=500, warmup=5),
+                              ("batched_over_loop", "loop_plain", "roofline", "kernels"))
+        out["adjoint"] = part(adjoint(args), ("plain_forward_s", "checkpointed_forward_s",
+                                              "reverse_s", "plain_zone_cycles_per_s"))
+        out["calibration"] = part(calibration(args), ("plain_forward_s", "gradient_s",
+                                                      "plain_zone_cycles_per_s"))
+        out["operators"] = part(sedov_operators(args), ("l1_den_vs_fused",))
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_cycles)
+        except Exception as exc:  # the baseline is reported, never the product
+            out["cpu_baseline"] = {"value": None, "error": repr(exc)}
+    else:
+        out["cpu_baseline"] = None
+    emit(out)
+
+
+def emit(out):
+    print(json.dumps(out), flush=True)
+
+
+def part(rec, extra):
+    """The headline figures of one workload's record, for the combined line."""
+    keep = ("metric", "value", "unit", "higher_is_better", "steps", "ms_per_step", "config")
+    return {k: rec[k] for k in keep + tuple(extra) if k in rec}
+
+
+def plain(args):
+    """The plain 256^3 blast: fused, device-resident cycle loop, HIP events
+    per kernel in the timed region."""
     from fierro_amd import bytemodel, problems as P
     from fierro_amd.sgh import SGHSolver
 
@@ -219,6 +314,11 @@ def main():
     g.profile(False)
     st = g.status(0)
     log(f"timed loop: {args.steps} cycles in {wall:.3f} s; status {st}")
+    # every cycle of both loops ran (fsgh_run stops early at time_final or
+    # cycle_stop, which would inflate the rate)
+    if st["cycles"] != args.warmup + 2 * args.steps:
+        raise SystemExit(f"bench: {st['cycles']} cycles ran, expected "
+                         f"{args.warmup + 2 * args.steps}")
     ie, ke = g.energy(0)
 
     E, N = g.E, g.N
@@ -228,7 +328,7 @@ def main():
     cyc_bytes = bytemodel.cycle(E, N, lists)
     value = E * args.steps / wall
     out = {
-        "metric": "FP64 SGH zone-cycles/s (Sedov 256^3 blast, plain mesh)",
+        "metric": METRIC,
         "value": value,
         "unit": "zone-cycles/s",
         "n_gpus": 1,
@@ -251,16 +351,8 @@ def main():
         "energy": {"ie": ie, "ke": ke, "time": st["time_value"]},
         "tuning": tuning,
     }
-    del g
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_cycles, threads)
-        except Exception as exc:  # the baseline is reported, never the product
-            out["cpu_baseline"] = {"value": None, "error": repr(exc)}
-    else:
-        out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    g.close()
+    return out
 
 
 def sedov_rz(args):
@@ -314,9 +406,8 @@ def sedov_rz(args):
     }
     del g
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
         try:
-            out["cpu_baseline"] = cpu_baseline_rz(1024, 100, threads)
+            out["cpu_baseline"] = cpu_baseline_rz(1024, 100)
         except Exception as exc:  # the baseline is reported, never the product
             out["cpu_baseline"] = {"value": None, "error": repr(exc)}
     else:
@@ -379,7 +470,7 @@ def sedov_operators(args):
                    "renumber": "morton", "dt_to_host": "every cycle"},
         "l1_den_vs_fused": diff, "time": t, "tuning": tuning, "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def kernel_table(g, prof, tuning):
@@ -443,14 +534,15 @@ def copy_calibration(mib, reps=3):
     print(json.dumps({"copy_calibration_bytes": n, "reps": reps}), flush=True)
 
 
-def ensemble(args):
+def ensemble(args, steps=500, warmup=5):
     """Batched ensemble: 512 Sod tubes (nx ~ U{64..1024} x 4 x 4, seeded) in
-    one ragged context, 500 RK2 cycles with per-instance dt; compared with
-    looping the plain solver over the instances (timed on a sample)."""
+    one ragged context, ``steps`` (500) timed RK2 cycles after ``warmup``
+    with per-instance dt; compared with looping the plain solver over the
+    instances (every instance with --loop-sample 1, the default)."""
     from fierro_amd import bytemodel, problems as P
     from fierro_amd.sgh import SGHSolver
 
-    steps = args.steps
+    args = argparse.Namespace(**{**vars(args), "warmup": warmup})
     specs = [s.with_run(cycle_stop=args.warmup + steps)
              for s in P.sod_ensemble(args.instances, seed=0, cycles=steps)]
     t0 = time.perf_counter()
@@ -475,7 +567,10 @@ def ensemble(args):
     del g
     # loop of the plain solver, one context per instance
     loop_zc, loop_s = 0, 0.0
-    for s in specs[::args.loop_sample]:
+    sample = specs[::args.loop_sample]
+    for k, s in enumerate(sample):
+        if k % 64 == 0:
+            log(f"per-instance loop: instance {k * args.loop_sample} of {len(specs)}")
         p = SGHSolver(s, renumber=True)
         p.set_tuning({k: {q: v[q] for q in ("block", "min_waves", "xcd_order")}
                       for k, v in tuning.items()})
@@ -500,12 +595,14 @@ def ensemble(args):
                      "frac": kernels[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None},
         "kernels": kernels,
         "loop_plain": {"value": loop, "unit": "zone-cycles/s",
-                       "sample": f"every {args.loop_sample}th instance, hipGraph per cycle"},
+                       "sample": (f"all {len(specs)} instances" if args.loop_sample == 1 else
+                                  f"every {args.loop_sample}th instance") +
+                       f", one context each, {steps} cycles, hipGraph per cycle"},
         "batched_over_loop": value / loop,
         "tuning": tuning,
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def adjoint(args):
@@ -522,6 +619,7 @@ def adjoint(args):
     plain = []
     for _ in range(3):
         g = SGHSolver(spec, renumber=True)
+        tune(g, args)
         g.run_begin()
         t = time.perf_counter()
         g.run(cycles, use_graph=True)
@@ -532,6 +630,7 @@ def adjoint(args):
     fw, bw = [], []
     for _ in range(3):
         g = SGHSolver(spec, renumber=True)
+        tune(g, args)
         seed = P.anvil_objective(spec, g.coords0)
         t = time.perf_counter()
         g.forward(cycles, checkpoint_every=every)
@@ -557,7 +656,7 @@ def adjoint(args):
         "grad_norm_den0": float((grad["den0"] ** 2).sum() ** 0.5),
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def calibration(args):
@@ -581,12 +680,14 @@ def calibration(args):
     E = g.E
     log(f"calibration setup: {len(specs)} instances E={E} N={g.N} in "
         f"{time.perf_counter() - t0:.1f} s")
+    tune(g, args)
     g.run_begin()
     t = time.perf_counter()
     g.run(cycles, use_graph=True)
     t_plain = time.perf_counter() - t
     g.close()
     g = SGHSolver(specs, renumber=True)
+    tune(g, args)
     t = time.perf_counter()
     J, dE0, dgam, drho, _ = CAL.gradient(g, params, target, cycles, every)
     t_grad = time.perf_counter() - t
@@ -609,7 +710,7 @@ def calibration(args):
                        "rho0": float(np.linalg.norm(drho))},
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -12,6 +12,7 @@
 #include "sgh_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -488,6 +489,14 @@ static double now_seconds(void)
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* Threads of the OpenMP loops (for timing the CPU baseline at several
+ * counts in one process); returns the count now in effect. */
+int orc_set_num_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
 }
 
 /* RK coefficient (K/sgh_solve.cpp:353).  ORC_PLANTED_MISTAKE selects a

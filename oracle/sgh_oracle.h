@@ -148,6 +148,9 @@ void orc_get_timestep(const orc_mesh *m, orc_state *s, double time_value, double
                       double *dt, double fuzz);
 void orc_energy_tallies(const orc_mesh *m, const orc_state *s, double *ie, double *ke);
 
+/* Threads of the OpenMP loops; returns the count in effect (n <= 0: query). */
+int orc_set_num_threads(int n);
+
 /* RK coefficient of stage s of an n-stage cycle: 1/(n - s) (K/sgh_solve.cpp:353). */
 double orc_rk_alpha(int64_t num_stages, int64_t stage);
 
