@@ -216,6 +216,8 @@ def main():
                              "calibration", "sedov-operators"],
                     default="sedov")
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--checkpoint-every", type=int, default=4,
+                    help="calibration workload: cycles between checkpoints of the forward")
     ap.add_argument("--tune", choices=["table", "auto", "default"], default="table",
                     help="launch variants of the fused kernels: the measured table "
                          "(fierro_amd/tuning.py), a fresh fsgh_autotune, or the defaults")
@@ -664,13 +666,14 @@ def calibration(args):
     n ~ U{48..96}, E0 ~ U(0.1,1), gamma ~ U(1.3,1.8), rho0 = 1, 150 cycles,
     seed 0; gradient of each instance's radial density-profile misfit against
     the target (the next draw of the stream) with respect to (E0, gamma, rho0)
-    from one batched checkpointed forward (every 5 cycles) + one reverse
-    pass; compared with one plain batched forward (fused loop, hipGraph)."""
+    from one batched checkpointed forward (every --checkpoint-every cycles)
+    + one reverse pass; compared with one plain batched forward (fused loop,
+    hipGraph)."""
     import numpy as np
     from fierro_amd import calibration as CAL, problems as P
     from fierro_amd.sgh import SGHSolver
 
-    cycles, every = 150, 5
+    cycles, every = 150, args.checkpoint_every
     specs, params, tspec, tparams = P.blast_ensemble(64, seed=0, cycles=cycles)
     t0 = time.perf_counter()
     target = CAL.target_profile(SGHSolver, tspec, cycles)
@@ -688,8 +691,9 @@ def calibration(args):
     g.close()
     g = SGHSolver(specs, renumber=True)
     tune(g, args)
+    layout = CAL.Layout(g)          # bins and origin elements of the initial mesh (set-up)
     t = time.perf_counter()
-    J, dE0, dgam, drho, _ = CAL.gradient(g, params, target, cycles, every)
+    J, dE0, dgam, drho, _ = CAL.gradient(g, params, target, cycles, every, layout=layout)
     t_grad = time.perf_counter() - t
     g.close()
     log(f"plain forward {t_plain:.3f} s, forward + reverse {t_grad:.3f} s")

@@ -23,7 +23,12 @@ RMAX = 0.3
 
 
 def centroids(coords, conn):
-    return coords[conn].mean(axis=1)
+    """Element centroids (mean of the nodes), summed node by node so that no
+    E x nodes x 3 temporary is formed."""
+    c = np.zeros((conn.shape[0], coords.shape[1]))
+    for a in range(conn.shape[1]):
+        c += coords[conn[:, a]]
+    return c / conn.shape[1]
 
 
 def bin_index(coords, conn, nbins=NBINS, rmax=RMAX):
@@ -77,6 +82,24 @@ def target_profile(Solver, spec, cycles, nbins=NBINS, rmax=RMAX):
     return prof
 
 
+class Layout:
+    """What the objective needs of a solver's initial mesh, computed once:
+    per instance its element range, radial bins and origin element."""
+
+    def __init__(self, g, nbins=NBINS, rmax=RMAX):
+        self.inst = []
+        for i in range(g.ninst):
+            e0, e1 = int(g.elem_off[i]), int(g.elem_off[i + 1])
+            n0, n1 = int(g.node_off[i]), int(g.node_off[i + 1])
+            x0 = g.coords0[n0:n1]
+            conn = g.conn[e0:e1] - n0
+            c = centroids(x0, conn)
+            r = np.linalg.norm(c, axis=1)
+            k = np.floor(r / rmax * nbins).astype(np.int64)
+            k[k >= nbins] = -1
+            self.inst.append((e0, e1, k, int(np.argmin(r))))
+
+
 def objective(g, target, cycles, dt_replay=None):
     """J of every instance of solver g after ``cycles`` cycles from the set-up
     state (``dt_replay`` [cycles][instances] freezes the steps)."""
@@ -91,27 +114,23 @@ def objective(g, target, cycles, dt_replay=None):
     return J
 
 
-def gradient(g, params, target, cycles, checkpoint_every=10, dt_replay=None):
+def gradient(g, params, target, cycles, checkpoint_every=10, dt_replay=None, layout=None):
     """One batched forward + reverse pass over every instance of solver g.
+    ``layout`` (a ``Layout`` of g, built once) holds the bins and origin
+    elements; without it they are computed here.
 
     Returns (J [inst], dJ/dE0 [inst], dJ/dgamma [inst], dJ/drho0 [inst]) and
     the recorded time steps."""
+    lay = layout if layout is not None else Layout(g)
     g.forward(cycles, checkpoint_every=checkpoint_every, dt_replay=dt_replay)
     den = g.download("elem_den")
     seed = np.zeros(g.E)
     J = np.zeros(g.ninst)
-    meta = []
-    for i in range(g.ninst):
-        e0, e1 = g.elem_off[i], g.elem_off[i + 1]
-        n0, n1 = g.node_off[i], g.node_off[i + 1]
-        x0 = g.coords0[n0:n1]
-        conn = g.conn[e0:e1] - n0
-        bins = bin_index(x0, conn)
+    for i, (e0, e1, bins, _) in enumerate(lay.inst):
         J[i], seed[e0:e1] = misfit(den[e0:e1], bins, target)
-        meta.append((e0, e1, origin_element(x0, conn), x0, conn))
     gr = g.backward(seed_den=seed)
     dE0, dgam, drho = np.zeros(g.ninst), np.zeros(g.ninst), np.zeros(g.ninst)
-    for i, (e0, e1, o, x0, conn) in enumerate(meta):
+    for i, (e0, e1, _, o) in enumerate(lay.inst):
         n, E0, gamma, rho0 = params[i]
         # the deposit is e = E0 / (rho0 V) for whatever volume V the fill used:
         # de/dE0 = e / E0 and de/drho0 = -e / rho0

@@ -1363,12 +1363,14 @@ int ensure_reverse_buffers(fsgh_ctx* c)
     const size_t SW = 6 * N + E;
     const size_t K = static_cast<size_t>(std::min<int64_t>(D.every, D.ncyc));
     // a second bank (recompute overlapped with the reverse pass) when there
-    // is more than one segment and both banks take under a quarter of the
+    // is more than one segment and both banks, with the reverse pass's other
+    // buffers (~110 doubles per element), take under three quarters of the
     // free device memory
     const size_t bank_bytes = 8 * ((K + 1) * (6 * N + 6 * E) + K * (18 * N + 54 * E));
+    const size_t other_bytes = 8 * (3 * (6 * N + E) + 6 * N + 90 * E);
     size_t mfree = 0, mtotal = 0;
     const bool two = D.ncyc > D.every && hipMemGetInfo(&mfree, &mtotal) == hipSuccess &&
-                     2 * bank_bytes < mfree / 4;
+                     2 * bank_bytes + other_bytes < mfree / 4 * 3;
     D.nbank = two ? 2 : 1;
     int rc = small_block_tables(c);
     if (!rc) rc = ensure_extreme_buffers(c);
