@@ -363,13 +363,103 @@ struct AdjView {
     double *x0b[3], *v0b[3], *e0b;
     double *Mb, *mb, *gb, *rhob;
     double* ge;                 // element: -alpha dt / M * e-bar of the stage output energy
-    double *Fb, *cb;            // corners: 3, 6
+    // node: the stage's mean velocity h = (v1 + v0)/2 and kv = alpha dt / m
+    // times the masked velocity adjoint, one 48-byte record per node
+    // (h0, h1, h2, kv0, kv1, kv2); a corner's force adjoint is
+    // F-bar = g_e h + kv at its node
+    double* hk;
+    // corners, element order: position (cbx) and velocity (cbv) adjoints,
+    // rows of 24 doubles per element (8 corners x 3)
+    double *cbx, *cbv;
+    // element rows of 24: the force reverse's adjoint of the B matrix, which
+    // k_adj_derived adds to its own before the one B-matrix VJP of the stage
+    double* bbf;
     double* eadj;               // element: den, pres, sspd, vol, div, ext (6 x E)
     int32_t* argext;            // element: 4 * node + slot of its chosen extreme, or -1
     double* nacc;               // node: per-slot sums of ext-bar (4 x N)
     double2 *lo, *hi;
     int2 *alo, *ahi;
 };
+
+// ---- rows of 24 doubles per element, moved between lanes and memory ----
+// A wave's 64 consecutive elements own one contiguous stretch of a corner
+// array (24 doubles each).  Reading it one row per lane would make every
+// wave instruction touch 64 cache lines; instead the lanes load and store
+// the stretch as consecutive 16-byte pieces (coalesced) and pass rows
+// through a padded per-wave LDS region, 32 rows (13 x 16 B each) at a time.
+// e0: the wave's first element; nvalid: its elements inside the work item
+// (>= 1); buf: 32 x 13 double2 of this wave.
+constexpr int kRowBuf = 32 * 13;
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void wave_issue_rows(const double* src_, int e0, int nvalid,
+                                                double2 t[12])
+{
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const double2* src = reinterpret_cast<const double2*>(src_ + 24 * static_cast<size_t>(e0));
+#pragma unroll
+    for (int r = 0; r < 12; ++r) {
+        // pieces past the valid rows re-read piece 0: unconditional loads,
+        // whose values go unused
+        const int idx = 64 * r + lane;
+        t[r] = src[idx < 12 * nvalid ? idx : 0];
+    }
+}
+
+// the loaded pieces -> this lane's row
+__device__ __forceinline__ void wave_rows_to_lanes(const double2 t[12], double2* buf, double f[24])
+{
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const int idx = 64 * r + lane;
+            buf[(idx / 12) * 13 + idx % 12] = t[6 * h + r];
+        }
+        wave_sync();
+        if ((lane >> 5) == h) {
+            const double2* mine = buf + (lane & 31) * 13;
+#pragma unroll
+            for (int i = 0; i < 12; ++i) {
+                const double2 v = mine[i];
+                f[2 * i] = v.x;
+                f[2 * i + 1] = v.y;
+            }
+        }
+    }
+}
+
+// this lane's row -> memory (rows past nvalid are not stored)
+__device__ __forceinline__ void wave_store_rows(double* dst_, int e0, int nvalid, const double f[24],
+                                                double2* buf)
+{
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    double2* dst = reinterpret_cast<double2*>(dst_ + 24 * static_cast<size_t>(e0));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        wave_sync();
+        if ((lane >> 5) == h) {
+            double2* mine = buf + (lane & 31) * 13;
+#pragma unroll
+            for (int i = 0; i < 12; ++i) mine[i] = make_double2(f[2 * i], f[2 * i + 1]);
+        }
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            const int idx = 64 * r + lane;
+            const int row = 32 * h + idx / 12;
+            if (row < nvalid) dst[12 * row + idx % 12] = buf[(idx / 12) * 13 + idx % 12];
+        }
+    }
+}
 
 // e1 = e0 - alpha dt / M * P,  P = sum_c F_c . (v1+v0)/2: the element part
 // (e0-bar, M-bar and the scalar g = -alpha dt / M * e1-bar that the node
@@ -404,45 +494,73 @@ __global__ __launch_bounds__(kB) void k_adj_energy(DMesh m, DState s, const doub
 }
 
 // v1 = mask(v0 + alpha dt F/m), x1 = x0 + alpha dt (v1 + v0)/2, and the
-// energy update's corner terms: F_c-bar = g_e (v1+v0)/2 + alpha dt/m v1-bar,
-// and (v1+v0)/2-bar = sum_c g_e F_c over the node's corners (CSR order).
-__global__ __launch_bounds__(kB) void k_adj_node(DMesh m, DState s, const double* F, AdjView A,
+// energy update's corner terms: (v1+v0)/2-bar = sum_c g_e F_c over the
+// node's corners (list order).  Writes the node's h = (v1+v0)/2 and
+// kv = alpha dt/m v1-bar (masked), from which k_adj_force forms each
+// corner's F_c-bar = g_e h + kv.  The block's corner forces and their
+// elements' g are staged in LDS in sorted-corner order (coalesced loads, as
+// the forward velocity kernel stages its forces); a block over the capacity
+// gathers from global memory.  Sums in list order either way.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_adj_node(DMesh m, DState s, const double* F, AdjView A,
                                                  const double* dts, double alpha, int assign)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
-    const int n = bi.start + static_cast<int>(threadIdx.x);
-    if (n >= bi.end) return;
     const double k = alpha * dts[bi.inst];
-    const int q0 = m.cin_start[n];
-    int c8[8];
-    const int cnt = corners8(m, n, q0, c8);
-    double sh[3] = {0.0, 0.0, 0.0}, sf[3] = {0.0, 0.0, 0.0};
-    double g8[8];
-    {
-        double f8[8][3];
+    constexpr int CAPC = corner_stage_cap(BS);
+    __shared__ double sF[4 * CAPC];              // per sorted corner: F (3), g of its element
+    const int qb = m.cin_start[bi.start];
+    const int nc = m.cin_start[bi.end] - qb;
+    const bool staged = m.cs_cor != nullptr && nc <= CAPC;
+    if (staged) {
+        constexpr int R = CAPC / BS;
+        int cid[R];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            g8[j] = A.ge[c8[j] >> 3];
-            const double* f = F + 3 * static_cast<size_t>(c8[j]);
-            f8[j][0] = f[0];
-            f8[j][1] = f[1];
-            f8[j][2] = f[2];
+        for (int r = 0; r < R; ++r) {
+            const int j = r * BS + static_cast<int>(threadIdx.x);
+            cid[r] = j < nc ? m.cs_cor[qb + j] : -1;
+        }
+        double g[R][4];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int c = cid[r] < 0 ? 0 : cid[r];
+            const double* f = F + 3 * static_cast<size_t>(c);
+            g[r][0] = f[0];
+            g[r][1] = f[1];
+            g[r][2] = f[2];
+            g[r][3] = A.ge[c >> 3];
         }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if (j < cnt) {
+        for (int r = 0; r < R; ++r) {
+            const int j = r * BS + static_cast<int>(threadIdx.x);
+            if (cid[r] >= 0) {
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    sh[d] += g8[j] * f8[j][d];
-                    sf[d] += f8[j][d];
-                }
+                for (int d = 0; d < 4; ++d) sF[4 * j + d] = g[r][d];
             }
         }
+        __syncthreads();
     }
-    for (int q = q0 + 8; q < q0 + cnt; ++q) {
-        const int c = m.cin[q];
-        const double g = A.ge[c >> 3];
-        const double* f = F + 3 * static_cast<size_t>(c);
+    const int n = bi.start + static_cast<int>(threadIdx.x);
+    if (n >= bi.end) return;
+    const int q0 = m.cin_start[n];
+    const int cnt = m.cin_start[n + 1] - q0;
+    double sh[3] = {0.0, 0.0, 0.0}, sf[3] = {0.0, 0.0, 0.0};
+    for (int q = q0; q < q0 + cnt; ++q) {
+        double f[3], g;
+        if (staged) {
+            const double* p = sF + 4 * m.cs_pos[q];
+            f[0] = p[0];
+            f[1] = p[1];
+            f[2] = p[2];
+            g = p[3];
+        } else {
+            const int c = m.cin[q];
+            const double* p = F + 3 * static_cast<size_t>(c);
+            f[0] = p[0];
+            f[1] = p[1];
+            f[2] = p[2];
+            g = A.ge[c >> 3];
+        }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             sh[d] += g * f[d];
@@ -467,21 +585,10 @@ __global__ __launch_bounds__(kB) void k_adj_node(DMesh m, DState s, const double
     }
     A.mb[n] += -k * dot / (mass * mass);
     const double km = k / mass;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        if (j < cnt) {
-            double* fb = A.Fb + 3 * static_cast<size_t>(c8[j]);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) fb[d] = g8[j] * h[d] + km * vm[d];
-        }
-    }
-    for (int q = q0 + 8; q < q0 + cnt; ++q) {
-        const int c = m.cin[q];
-        const double g = A.ge[c >> 3];
-        double* fb = A.Fb + 3 * static_cast<size_t>(c);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) fb[d] = g * h[d] + km * vm[d];
-    }
+    double2* rec = reinterpret_cast<double2*>(A.hk + 6 * static_cast<size_t>(n));
+    rec[0] = make_double2(h[0], h[1]);
+    rec[1] = make_double2(h[2], km * vm[0]);
+    rec[2] = make_double2(km * vm[1], km * vm[2]);
 }
 
 // Shock-detector neighbour term: add to each element's divergence adjoint
@@ -529,12 +636,16 @@ __global__ __launch_bounds__(kB) void k_adj_ext_node(DMesh m, AdjView A)
 }
 
 // Reverse of force_elem (K/force_sgh.cpp:35-359) at the stage input state.
-__global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, const InstCtl* ctl,
+// One wave of 64 elements per workgroup; lanes past the work item compute on
+// the item's first element and store nothing (every lane takes part in the
+// row transposes).
+__global__ __launch_bounds__(64) void k_adj_force(DMesh m, DState s, AdjView A, const InstCtl* ctl,
                                                   const double* vol_a, const double* div_a)
 {
+    __shared__ double2 rbuf[kRowBuf];
     const BlockItem bi = m.eblk[blockIdx.x];
-    const int e = bi.start + static_cast<int>(threadIdx.x);
-    if (e >= bi.end) return;
+    const bool live = bi.start + static_cast<int>(threadIdx.x) < bi.end;
+    const int e = live ? bi.start + static_cast<int>(threadIdx.x) : bi.start;
     const double fuzz = ctl[bi.inst].fuzz, small = ctl[bi.inst].small;
     int32_t nd[8];
     ld_conn(m.conn, e, nd);
@@ -588,13 +699,18 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
     // ---- reverse ----
     double Bb[24], velb[3][8], Mb[8];
     double presb = 0.0, phib = 0.0, vsb[3] = {0.0, 0.0, 0.0};
+    const double ge = A.ge[e];
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        const size_t c = 8 * static_cast<size_t>(e) + a;
         double g = 0.0;
+        // the node's record (h, kv) from k_adj_node: three 16-byte loads
+        const double2* rec = reinterpret_cast<const double2*>(A.hk + 6 * static_cast<size_t>(nd[a]));
+        const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+        const double hh[3] = {r0.x, r0.y, r1.x}, kv[3] = {r1.y, r2.x, r2.y};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const double fb = A.Fb[3 * c + d];
+            // the corner's force adjoint g_e h + kv at its node
+            const double fb = ge * hh[d] + kv[d];
             presb += B[3 * a + d] * fb;
             Bb[3 * a + d] = pres * fb;
             g += fb * (vs[d] - vel[d][a]);
@@ -606,10 +722,11 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
     }
     if (riemann) {
         double Wb = 0.0, Sb[3];
+        const double iW = 1.0 / W;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            Sb[d] = vsb[d] / W;
-            Wb -= vsb[d] * vs[d] / W;
+            Sb[d] = vsb[d] * iW;
+            Wb -= vsb[d] * vs[d] * iW;
         }
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
@@ -628,16 +745,18 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
         const double mucb = Mb[a] * nb[a];
         const double nbb = Mb[a] * muc;
         if (nb[a] > 0.0) {
+            const double q = nbb / nb[a];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) Bb[3 * a + d] += nbb * B[3 * a + d] / nb[a];
+            for (int d = 0; d < 3; ++d) Bb[3 * a + d] += q * B[3 * a + d];
         }
         denb += mucb * (qa * sspd + qb * mv[a]);
         sspdb += mucb * den * qa;
         const double mvb = mucb * den * qb;
         if (mv[a] > 0.0) {
+            const double q = mvb / mv[a];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double db = mvb * (vel[d][a] - vbar[d]) / mv[a];
+                const double db = q * (vel[d][a] - vbar[d]);
                 velb[d][a] += db;
                 vbarb[d] -= db;
             }
@@ -661,22 +780,21 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
         extb = rb * 0.9 / bden;
         divb -= rb * r / bden;
     }
-    // reload the coordinates for the B-matrix VJP rather than keep them live
-    // through the reverse sweep (the compiler barrier forbids reusing them)
-    asm volatile("" ::: "memory");
-    g3(s.x1, s.y1, s.z1, nd, x, y, z);
-    double xb[8] = {0}, yb[8] = {0}, zb[8] = {0};
-    hex_bmatrix_vjp(x, y, z, Bb, xb, yb, zb);
+    // the B-matrix adjoint goes to k_adj_derived, which adds its own and
+    // applies the stage's one VJP (the map Bb -> x-bar is linear)
+    const int nvalid = bi.end - bi.start;
+    wave_store_rows(A.bbf, bi.start, nvalid, Bb, rbuf);
+    {
+        double row[24];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        double* cb = A.cb + 6 * (8 * static_cast<size_t>(e) + a);
-        cb[0] = xb[a];
-        cb[1] = yb[a];
-        cb[2] = zb[a];
-        cb[3] = velb[0][a];
-        cb[4] = velb[1][a];
-        cb[5] = velb[2][a];
+        for (int a = 0; a < 8; ++a) {
+            row[3 * a] = velb[0][a];
+            row[3 * a + 1] = velb[1][a];
+            row[3 * a + 2] = velb[2][a];
+        }
+        wave_store_rows(A.cbv, bi.start, nvalid, row, rbuf);
     }
+    if (!live) return;
     const size_t E = static_cast<size_t>(m.E);
     A.eadj[0 * E + e] = denb;
     A.eadj[1 * E + e] = presb;
@@ -690,15 +808,20 @@ __global__ __launch_bounds__(kB) void k_adj_force(DMesh m, DState s, AdjView A, 
 // Neighbour divergence adjoint, then the stage-input state (vol, den, EOS,
 // div).  setup != 0: the first stage of the run, whose density, pressure and
 // sound speed come from the initial conditions (den = rho0) and whose
-// coordinates are the fixed initial mesh.
-__global__ __launch_bounds__(kB) void k_adj_derived(DMesh m, DState s, AdjView A,
+// coordinates are the fixed initial mesh.  Adds its velocity terms to the
+// corner rows k_adj_force wrote, adds its B-matrix adjoint to the force
+// reverse's and writes the position part of the corner rows from the one
+// VJP.  Lanes past the work item compute on the item's first element and
+// store nothing.
+__global__ __launch_bounds__(64) void k_adj_derived(DMesh m, DState s, AdjView A,
                                                     const double* vol_a, const double* div_a,
                                                     const double* sie_in, double* sieb_in,
                                                     int setup, int assign)
 {
+    __shared__ double2 rbuf[kRowBuf];
     const BlockItem bi = m.eblk[blockIdx.x];
-    const int e = bi.start + static_cast<int>(threadIdx.x);
-    if (e >= bi.end) return;
+    const bool live = bi.start + static_cast<int>(threadIdx.x) < bi.end;
+    const int e = live ? bi.start + static_cast<int>(threadIdx.x) : bi.start;
     const size_t E = static_cast<size_t>(m.E);
     int32_t nd[8];
     ld_conn(m.conn, e, nd);
@@ -733,101 +856,151 @@ __global__ __launch_bounds__(kB) void k_adj_derived(DMesh m, DState s, AdjView A
     const double eb = presb * pr.d + sspdb * cs.d;
     eos(kind, Dual(gam, 1.0), csmin, Dual(den), Dual(sie), pr, cs);
     const double gb = presb * pr.d + sspdb * cs.d;
-    sieb_in[e] = assign ? eb : sieb_in[e] + eb;
-    A.gb[e] += gb;
+    if (live) {
+        sieb_in[e] = assign ? eb : sieb_in[e] + eb;
+        A.gb[e] += gb;
+    }
     // divergence (K/momentum.cpp:265-286): div = (sum_a v_a . B_a) / vol
     double B[24];
     hex_bmatrix(x, y, z, B);
     const double iv = 1.0 / vol;
+    // the force reverse's B-matrix adjoint
+    const int nvalid = bi.end - bi.start;
     double Bb[24];
+    {
+        const double2* rb = reinterpret_cast<const double2*>(A.bbf + 24 * static_cast<size_t>(e));
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            Bb[3 * a + d] = divb * iv * vel[d][a];
-            double* cb = A.cb + 6 * (8 * static_cast<size_t>(e) + a);
-            cb[3 + d] += divb * iv * B[3 * a + d];
+        for (int i = 0; i < 12; ++i) {
+            const double2 v = rb[i];
+            Bb[2 * i] = v.x;
+            Bb[2 * i + 1] = v.y;
         }
-    volb -= divb * div * iv;
-    if (setup) {
-        // den = rho0; vol and B belong to the fixed initial mesh
-        A.rhob[e] += dnb;
-        return;
     }
-    // den = M / vol
-    A.Mb[e] += dnb / vol;
-    volb -= dnb * den / vol;
-    // vol = V(x): dV/dx_a = B_a
-    double xb[8] = {0}, yb[8] = {0}, zb[8] = {0};
+    double2* rv = reinterpret_cast<double2*>(A.cbv + 24 * static_cast<size_t>(e));
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        xb[a] = volb * B[3 * a];
-        yb[a] = volb * B[3 * a + 1];
-        zb[a] = volb * B[3 * a + 2];
+    for (int i = 0; i < 12; ++i) {
+        // velocity part of the corner row, two values at a time
+        const double2 old = rv[i];
+        double o[2] = {old.x, old.y};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int a = (2 * i + h) / 3, d = (2 * i + h) % 3;
+            Bb[3 * a + d] += divb * iv * vel[d][a];
+            o[h] += divb * iv * B[3 * a + d];
+        }
+        if (live) rv[i] = make_double2(o[0], o[1]);
+    }
+    volb -= divb * div * iv;
+    double xb[8] = {0}, yb[8] = {0}, zb[8] = {0};
+    if (setup) {
+        // den = rho0; vol and B belong to the fixed initial mesh (its
+        // coordinate adjoint is not a gradient of the run)
+        if (live) A.rhob[e] += dnb;
+    } else {
+        // den = M / vol
+        if (live) A.Mb[e] += dnb / vol;
+        volb -= dnb * den / vol;
+        // vol = V(x): dV/dx_a = B_a
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            xb[a] = volb * B[3 * a];
+            yb[a] = volb * B[3 * a + 1];
+            zb[a] = volb * B[3 * a + 2];
+        }
     }
     hex_bmatrix_vjp(x, y, z, Bb, xb, yb, zb);
+    double row[24];
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        double* cb = A.cb + 6 * (8 * static_cast<size_t>(e) + a);
-        cb[0] += xb[a];
-        cb[1] += yb[a];
-        cb[2] += zb[a];
+        row[3 * a] = xb[a];
+        row[3 * a + 1] = yb[a];
+        row[3 * a + 2] = zb[a];
     }
+    wave_store_rows(A.cbx, bi.start, nvalid, row, rbuf);
 }
 
-// corner adjoints -> node adjoints, summed in corner order
-__global__ __launch_bounds__(kB) void k_adj_gather(DMesh m, const double* cb, double* xa0,
-                                                   double* xa1, double* xa2, double* va0,
-                                                   double* va1, double* va2, int assign)
+// corner adjoints -> node adjoints, summed in corner-list order (no
+// atomics).  The block's corner rows are staged in LDS in sorted-corner order
+// (coalesced loads), the position part, then the velocity part; a block over
+// the capacity gathers from global memory.  assign: written, else added to.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_adj_gather(DMesh m, const double* cbx, const double* cbv,
+                                                   double* xa0, double* xa1, double* xa2,
+                                                   double* va0, double* va1, double* va2,
+                                                   int assign)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
+    constexpr int CAPC = corner_stage_cap(BS);
+    constexpr int R = CAPC / BS;
+    __shared__ double sC[3 * CAPC];
+    const int qb = m.cin_start[bi.start];
+    const int nc = m.cin_start[bi.end] - qb;
+    const bool staged = m.cs_cor != nullptr && nc <= CAPC;
     const int n = bi.start + static_cast<int>(threadIdx.x);
-    if (n >= bi.end) return;
-    const int q0 = m.cin_start[n];
-    int c8[8];
-    const int cnt = corners8(m, n, q0, c8);
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    {
-        double v8[8][6];
+    const bool live = n < bi.end;
+    int cid[R];
+    if (staged) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const double2* p = reinterpret_cast<const double2*>(cb + 6 * static_cast<size_t>(c8[j]));
-            const double2 a = p[0], b = p[1], c = p[2];
-            v8[j][0] = a.x; v8[j][1] = a.y; v8[j][2] = b.x;
-            v8[j][3] = b.y; v8[j][4] = c.x; v8[j][5] = c.y;
+        for (int r = 0; r < R; ++r) {
+            const int j = r * BS + static_cast<int>(threadIdx.x);
+            cid[r] = j < nc ? m.cs_cor[qb + j] : -1;
         }
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (j < cnt)
-#pragma unroll
-                for (int k = 0; k < 6; ++k) s[k] += v8[j][k];
     }
-    for (int q = q0 + 8; q < q0 + cnt; ++q) {
-        const double* p = cb + 6 * static_cast<size_t>(m.cin[q]);
+    int q0 = 0, cnt = 0;
+    if (live) {
+        q0 = m.cin_start[n];
+        cnt = m.cin_start[n + 1] - q0;
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < 6; ++k) s[k] += p[k];
+    for (int part = 0; part < 2; ++part) {
+        const double* cb = part ? cbv : cbx;
+        if (staged) {
+            double g[R][3];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double* p = cb + 3 * static_cast<size_t>(cid[r] < 0 ? 0 : cid[r]);
+                g[r][0] = p[0];
+                g[r][1] = p[1];
+                g[r][2] = p[2];
+            }
+            if (part) __syncthreads();     // the position part's readers are done
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int j = r * BS + static_cast<int>(threadIdx.x);
+                if (cid[r] >= 0) {
+                    sC[3 * j] = g[r][0];
+                    sC[3 * j + 1] = g[r][1];
+                    sC[3 * j + 2] = g[r][2];
+                }
+            }
+            __syncthreads();
+        }
+        for (int q = q0; q < q0 + cnt; ++q) {
+            const double* p = staged ? sC + 3 * m.cs_pos[q] : cb + 3 * static_cast<size_t>(m.cin[q]);
+            acc[3 * part] += p[0];
+            acc[3 * part + 1] += p[1];
+            acc[3 * part + 2] += p[2];
+        }
     }
-    if (!assign) {
-        s[0] += xa0[n];
-        s[1] += xa1[n];
-        s[2] += xa2[n];
-        s[3] += va0[n];
-        s[4] += va1[n];
-        s[5] += va2[n];
-    }
-    xa0[n] = s[0];
-    xa1[n] = s[1];
-    xa2[n] = s[2];
-    va0[n] = s[3];
-    va1[n] = s[4];
-    va2[n] = s[5];
+    if (!live) return;
+    double* out[6] = {xa0, xa1, xa2, va0, va1, va2};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k][n] = assign ? acc[k] : acc[k] + out[k][n];
+}
+
+void launch_adj_gather(const DMesh& mn, const double* cbx, const double* cbv, double* const x[3],
+                       double* const v[3], int assign, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adj_gather<kAdjNodeBlock>, dim3(mn.n_nblk), dim3(kAdjNodeBlock), 0, st,
+                       mn, cbx, cbv, x[0], x[1], x[2], v[0], v[1], v[2], assign);
 }
 
 // Final element density den = M / V(x) as an objective term: its seed sd
 // gives M-bar += sd / V and, per corner, x-bar = -sd den / V dV/dx_a with
 // dV/dx_a = B_a (velocity part zero).
 __global__ __launch_bounds__(kB) void k_adj_seed_den(DMesh m, DState s, const double* sd,
-                                                     double* Mb, double* cb)
+                                                     double* Mb, double* cbx, double* cbv)
 {
     const BlockItem bi = m.eblk[blockIdx.x];
     const int e = bi.start + static_cast<int>(threadIdx.x);
@@ -843,13 +1016,14 @@ __global__ __launch_bounds__(kB) void k_adj_seed_den(DMesh m, DState s, const do
     const double k = -g * (M / vol) / vol;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        double* p = cb + 6 * (8 * static_cast<size_t>(e) + a);
-        p[0] = k * B[3 * a];
-        p[1] = k * B[3 * a + 1];
-        p[2] = k * B[3 * a + 2];
-        p[3] = 0.0;
-        p[4] = 0.0;
-        p[5] = 0.0;
+        double* px = cbx + 24 * static_cast<size_t>(e) + 3 * a;
+        double* pv = cbv + 24 * static_cast<size_t>(e) + 3 * a;
+        px[0] = k * B[3 * a];
+        px[1] = k * B[3 * a + 1];
+        px[2] = k * B[3 * a + 2];
+        pv[0] = 0.0;
+        pv[1] = 0.0;
+        pv[2] = 0.0;
     }
 }
 
@@ -1287,8 +1461,9 @@ void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, cons
     }
     hipLaunchKernelGGL(k_adj_energy, dim3(m.n_eblk), tb, 0, c->stream, m, s_out, F, A, dts,
                        alpha, assign);
-    hipLaunchKernelGGL(k_adj_node, dim3(m.n_nblk), tb, 0, c->stream, m, s_out, F, A, dts,
-                       alpha, assign);
+    const DMesh mn = adjoint_node_mesh(c);
+    hipLaunchKernelGGL(k_adj_node<kAdjNodeBlock>, dim3(mn.n_nblk), dim3(kAdjNodeBlock), 0,
+                       c->stream, mn, s_out, F, A, dts, alpha, assign);
     if (!ext)
         hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), tb, 0, c->stream, m,
                            static_cast<const double*>(s_in.div),
@@ -1302,9 +1477,7 @@ void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, cons
                        static_cast<const double*>(s_in.vol), static_cast<const double*>(s_in.div),
                        static_cast<const double*>(s_in.sie1), in.e,
                        (cyc == 0 && stage == 0) ? 1 : 0, assign);
-    hipLaunchKernelGGL(k_adj_gather, dim3(m.n_nblk), tb, 0, c->stream, m,
-                       static_cast<const double*>(A.cb), in.x[0], in.x[1], in.x[2], in.v[0],
-                       in.v[1], in.v[2], assign);
+    launch_adj_gather(mn, A.cbx, A.cbv, in.x, in.v, assign, c->stream);
 }
 
 int ensure_extreme_buffers(fsgh_ctx* c)
@@ -1373,6 +1546,7 @@ int ensure_reverse_buffers(fsgh_ctx* c)
                      2 * bank_bytes + other_bytes < mfree / 4 * 3;
     D.nbank = two ? 2 : 1;
     int rc = small_block_tables(c);
+    if (!rc) rc = adjoint_node_tables(c);
     if (!rc) rc = ensure_extreme_buffers(c);
     for (int b = 0; b < D.nbank && !rc; ++b) rc |= alloc_bank(c, D.bank[b], K, two);
     if (!rc && two) {
@@ -1389,8 +1563,10 @@ int ensure_reverse_buffers(fsgh_ctx* c)
     rc |= adj_alloc(c, &D.A.rhob, E);
     rc |= adj_alloc(c, &D.A.ge, E);
     rc |= adj_alloc(c, &D.grho, E);
-    rc |= adj_alloc(c, &D.A.Fb, 3 * C);
-    rc |= adj_alloc(c, &D.A.cb, 6 * C);
+    rc |= adj_alloc(c, &D.A.hk, 6 * N);
+    rc |= adj_alloc(c, &D.A.cbx, 3 * C);
+    rc |= adj_alloc(c, &D.A.cbv, 3 * C);
+    rc |= adj_alloc(c, &D.A.bbf, 3 * C);
     rc |= adj_alloc(c, &D.A.eadj, 6 * E);
     rc |= adj_alloc_t(c, &D.A.argext, E);
     rc |= adj_alloc(c, &D.A.nacc, 4 * N);
@@ -1626,10 +1802,8 @@ int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const
         const SetPtr O = set_ptr(D, D.set[iO]);
         DState sf = make_state(c, 1 - c->cur, c->cur);
         hipLaunchKernelGGL(k_adj_seed_den, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, sf,
-                           static_cast<const double*>(D.grho), D.A.Mb, D.A.cb);
-        hipLaunchKernelGGL(k_adj_gather, dim3(m.n_nblk), dim3(kB), 0, c->stream, m,
-                           static_cast<const double*>(D.A.cb), O.x[0], O.x[1], O.x[2], O.v[0],
-                           O.v[1], O.v[2], 0);
+                           static_cast<const double*>(D.grho), D.A.Mb, D.A.cbx, D.A.cbv);
+        launch_adj_gather(adjoint_node_mesh(c), D.A.cbx, D.A.cbv, O.x, O.v, 0, c->stream);
     }
     const int64_t K = D.every;
     const int64_t nseg = (D.ncyc + K - 1) / K;

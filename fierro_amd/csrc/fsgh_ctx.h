@@ -137,6 +137,13 @@ int ensure_table(fsgh_ctx* c, bool elem, int32_t bs, int32_t xcd, bool lists = t
 // registers per lane, spread a small mesh over every CU this way
 int small_block_tables(fsgh_ctx* c);
 DMesh small_block_mesh(fsgh_ctx* c);
+// the node table of the reverse pass's node kernels (kAdjNodeBlock nodes per
+// work item, chunks of kAdjNodeDeal items per XCD) with its sorted corner
+// lists, and the mesh view over it
+constexpr int32_t kAdjNodeBlock = 128;
+constexpr int32_t kAdjNodeDeal = 4;
+int adjoint_node_tables(fsgh_ctx* c);
+DMesh adjoint_node_mesh(fsgh_ctx* c);
 // per-block node lists of the element partition with `bs` elements per block
 // (3D device contexts; no-op otherwise), and the mesh view's pointers to the
 // lists of the partition its element table `m.eblk` has (ebs per block)

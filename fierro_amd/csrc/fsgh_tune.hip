@@ -229,6 +229,23 @@ DMesh fsgh::small_block_mesh(fsgh_ctx* c)
     return m;
 }
 
+int fsgh::adjoint_node_tables(fsgh_ctx* c)
+{
+    if (int rc = ensure_table(c, false, kAdjNodeBlock, kAdjNodeDeal, false)) return rc;
+    return build_corner_stage(c, kAdjNodeBlock);
+}
+
+DMesh fsgh::adjoint_node_mesh(fsgh_ctx* c)
+{
+    DMesh m = make_mesh(c);
+    const fsgh_ctx::WorkTable& wn = c->tables.at(table_key(false, kAdjNodeBlock, kAdjNodeDeal));
+    m.bs = kAdjNodeBlock;
+    m.nblk = wn.d;
+    m.n_nblk = wn.n;
+    attach_corner_stage(c, m, kAdjNodeBlock);
+    return m;
+}
+
 DMesh fsgh::dt_mesh(fsgh_ctx* c)
 {
     if (c->dims == 2 || c->d_stress[0]) return make_mesh(c);
