@@ -661,7 +661,7 @@ int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill*
     }
     for (int32_t f = 0; f < nfills; ++f) {
         if (fills[f].mat_id < 0 || fills[f].mat_id >= nmat) return FSGH_E_INVALID;
-        if (fills[f].velocity < 0 || fills[f].velocity > 4 || fills[f].volume < 0 || fills[f].volume > 3)
+        if (fills[f].velocity < 0 || fills[f].velocity > 5 || fills[f].volume < 0 || fills[f].volume > 3)
             return FSGH_E_UNSUPPORTED;
     }
     if (c->dims == 2) {
@@ -678,6 +678,7 @@ int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill*
     const int64_t n0 = c->L.node_off[inst], n1 = c->L.node_off[inst + 1];
     const size_t NE = static_cast<size_t>(e1 - e0), NN = static_cast<size_t>(n1 - n0);
     std::vector<double> vol(NE), den(NE), mass(NE), sie(NE), pres(NE), sspd(NE), gam(NE), csm(NE);
+    std::vector<double> sv4(NE, 0.0);
     std::vector<int32_t> mat(NE);
     std::vector<double> vel(3 * NN, 0.0), nmass(NN, 0.0);
     const double* X = c->coords.data();
@@ -725,9 +726,11 @@ int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill*
             const int32_t m = F.mat_id;
             if (mat_nsv[m] > 0) gam[k] = state_vars[static_cast<size_t>(m) * nsv + 0];
             if (mat_nsv[m] > 1) csm[k] = state_vars[static_cast<size_t>(m) * nsv + 1];
+            if (mat_nsv[m] > 4) sv4[k] = state_vars[static_cast<size_t>(m) * nsv + 4];
             // the material's EOS (K/setup.cpp:225-232)
             eos(c->mats[static_cast<size_t>(m)].eos, gam[k], csm[k], den[k], sie[k], pres[k],
                 sspd[k]);
+            if (F.velocity == 5) tg_vortex_state(ec, F.den, sv4[k], pres[k], sie[k]);
             for (int a = 0; a < 8; ++a) {
                 const int64_t n = c->conn[static_cast<size_t>(8 * e + a)];
                 double* v = &vel[3 * static_cast<size_t>(n - n0)];
@@ -735,6 +738,8 @@ int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill*
                     v[0] = F.u;
                     v[1] = F.v;
                     v[2] = F.w;
+                } else if (F.velocity == 5) {
+                    tg_vortex_velocity(X + 3 * n, true, v);
                 } else if (F.velocity == 1 || F.velocity == 2) {
                     const int nd = F.velocity == 1 ? 2 : 3;
                     double dir[3] = {0.0, 0.0, 0.0}, rv = 0.0;

@@ -18,6 +18,26 @@
 
 namespace fsgh {
 
+// Taylor-Green vortex initial condition (velocity fill kind 5,
+// K/setup.cpp:326-350), with pi as the reference defines it (K/mesh.h:8).
+// Node velocity from the node's coordinates; the element's pressure from its
+// centroid and the energy that pressure implies with the element's state
+// variable 4 taken as gamma (the reference reads elem_statev(e, 4), which is
+// zero unless the material sets at least five state variables).
+constexpr double kTgPi = 3.141592653589793;
+FSGH_HD inline void tg_vortex_velocity(const double* X, bool three_d, double v[3])
+{
+    v[0] = sin(kTgPi * X[0]) * cos(kTgPi * X[1]);
+    v[1] = -1.0 * cos(kTgPi * X[0]) * sin(kTgPi * X[1]);
+    if (three_d) v[2] = 0.0;
+}
+FSGH_HD inline void tg_vortex_state(const double* centroid, double den, double statev4,
+                                    double& pres, double& sie)
+{
+    pres = 0.25 * (cos(2.0 * kTgPi * centroid[0]) + cos(2.0 * kTgPi * centroid[1])) + 1.0;
+    sie = pres / (den * (statev4 - 1.0));
+}
+
 // velocity divergence from B, v and V (K/momentum.cpp:265-286)
 template <typename T>
 FSGH_HD inline T divergence(const T* B, const T* u, const T* v, const T* w, T vol)

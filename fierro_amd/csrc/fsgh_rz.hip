@@ -701,6 +701,7 @@ int rz_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill* f
     const int64_t n0 = c->L.node_off[inst], n1 = c->L.node_off[inst + 1];
     const size_t NE = static_cast<size_t>(e1 - e0), NN = static_cast<size_t>(n1 - n0);
     std::vector<double> vol(NE), den(NE), mass(NE), sie(NE), pres(NE), sspd(NE), gam(NE), csm(NE);
+    std::vector<double> sv4(NE, 0.0);
     std::vector<int32_t> mat(NE);
     std::vector<double> vel(2 * NN, 0.0), nmass(NN, 0.0);
     const double* X = c->coords.data();   // N x 3, z = 0
@@ -750,14 +751,18 @@ int rz_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill* f
             const int32_t m = F.mat_id;
             if (mat_nsv[m] > 0) gam[k] = state_vars[static_cast<size_t>(m) * nsv + 0];
             if (mat_nsv[m] > 1) csm[k] = state_vars[static_cast<size_t>(m) * nsv + 1];
+            if (mat_nsv[m] > 4) sv4[k] = state_vars[static_cast<size_t>(m) * nsv + 4];
             eos(c->mats[static_cast<size_t>(m)].eos, gam[k], csm[k], den[k], sie[k], pres[k],
                 sspd[k]);
+            if (F.velocity == 5) tg_vortex_state(ec, F.den, sv4[k], pres[k], sie[k]);
             for (int a = 0; a < 4; ++a) {
                 const int64_t n = c->conn[static_cast<size_t>(4 * e + a)];
                 double* v = &vel[2 * static_cast<size_t>(n - n0)];
                 if (F.velocity == 0) {
                     v[0] = F.u;
                     v[1] = F.v;
+                } else if (F.velocity == 5) {
+                    tg_vortex_velocity(X + 3 * n, false, v);
                 } else if (F.velocity == 1) {
                     double dir[2] = {0.0, 0.0}, rv = 0.0;
                     for (int d = 0; d < 2; ++d) {

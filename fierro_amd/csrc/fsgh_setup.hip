@@ -278,7 +278,7 @@ __global__ void k_fill_elem(const fsgh_fill* fills, int nfills, const int32_t* i
     ec[0] = ec[0] / 8;
     ec[1] = ec[1] / 8;
     ec[2] = ec[2] / 8;
-    double d = 0.0, ms = 0.0, ie = 0.0, g = 0.0, cm = 0.0, p = 0.0, cs = 0.0;
+    double d = 0.0, ms = 0.0, ie = 0.0, g = 0.0, cm = 0.0, p = 0.0, cs = 0.0, sv4 = 0.0;
     int32_t mt = 0;
     unsigned long long vm = 0;
     for (int f = 0; f < nfills; ++f) {
@@ -290,8 +290,10 @@ __global__ void k_fill_elem(const fsgh_fill* fills, int nfills, const int32_t* i
         mt = F.mat_id;
         if (mat_nsv[mt] > 0) g = state_vars[static_cast<int64_t>(mt) * nsv + 0];
         if (mat_nsv[mt] > 1) cm = state_vars[static_cast<int64_t>(mt) * nsv + 1];
+        if (mat_nsv[mt] > 4) sv4 = state_vars[static_cast<int64_t>(mt) * nsv + 4];
         eos(mats[mt].eos, g, cm, d, ie, p, cs);
-        if (F.velocity >= 0 && F.velocity <= 2) vm |= 1ull << f;
+        if (F.velocity == 5) tg_vortex_state(ec, F.den, sv4, p, ie);
+        if ((F.velocity >= 0 && F.velocity <= 2) || F.velocity == 5) vm |= 1ull << f;
     }
     den[e] = d;
     mass[e] = ms;
@@ -334,6 +336,8 @@ __global__ void k_fill_node(const fsgh_fill* fills, const int32_t* cin_start, co
             vel[0] = F.u;
             vel[1] = F.v;
             vel[2] = F.w;
+        } else if (F.velocity == 5) {
+            tg_vortex_velocity(X, true, vel);
         } else {
             const int nd = F.velocity == 1 ? 2 : 3;
             double dir[3] = {0.0, 0.0, 0.0}, rv = 0.0;

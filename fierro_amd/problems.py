@@ -393,6 +393,47 @@ def sod_ensemble(count=512, seed=0, cycles=500):
     return out
 
 
+TG_VORTEX = 5   # velocity fill kind (K/mesh.h:851)
+
+
+def taylor_green(n=16, nz=2, cycles=100, gamma=5.0 / 3.0, statev4=None, dims=3, dt_cfl=0.4):
+    """Taylor-Green vortex on the unit square (K/setup.cpp:326-350): velocity
+    (sin pi x cos pi y, -cos pi x sin pi y, 0), pressure
+    0.25 (cos 2 pi x + cos 2 pi y) + 1 by element centroid and energy
+    p / (rho (statev[4] - 1)); rho = 1.  The material carries 6 state
+    variables with gamma in slot 0 and ``statev4`` in slot 4 (default gamma,
+    the physical choice; 0 reproduces a material without that slot, for which
+    the reference's energy comes out negative).  3D: n x n x nz hexes on
+    [0,1] x [0,1] x [0, nz/n]; 2D: n x n quads.  Reflected on every side
+    (the vortex's normal velocity vanishes on the walls)."""
+    sv = _gas(gamma)
+    sv[0, 4] = gamma if statev4 is None else statev4
+    if dims == 2:
+        mesh = dict(kind="rect", nx=n, ny=n, lx=1.0, ly=1.0, lden=1.0)
+        bcs = [(X_PLANE, 0.0, REFLECTED), (X_PLANE, 1.0, REFLECTED),
+               (Y_PLANE, 0.0, REFLECTED), (Y_PLANE, 1.0, REFLECTED)]
+    else:
+        mesh = dict(kind="box", nx=n, ny=n, nz=nz, lx=float(n), ly=float(n), lz=float(nz),
+                    lden=float(n))
+        bcs = [(X_PLANE, 0.0, REFLECTED), (X_PLANE, 1.0, REFLECTED),
+               (Y_PLANE, 0.0, REFLECTED), (Y_PLANE, 1.0, REFLECTED),
+               (Z_PLANE, 0.0, REFLECTED), (Z_PLANE, nz / n, REFLECTED)]
+    spec = ProblemSpec(
+        name=f"taylor_green{n}" + ("_rz" if dims == 2 else ""),
+        mesh=mesh,
+        materials=[(1.0, 1.0, 1.3333, 0.0)],
+        state_vars=sv,
+        mat_num_state_vars=[6],
+        fills=[fill(GLOBAL, den=1.0, sie=1.0, velocity=TG_VORTEX)],
+        bcs=bcs,
+        dims=dims,
+    )
+    spec.run.update(time_final=1.0e16, dt_start=1.0e-5, dt_max=1.0e-2, dt_min=1.0e-8,
+                    dt_cfl=dt_cfl, graphics_dt_ival=1.0e16, graphics_cyc_ival=1000000000,
+                    cycle_stop=cycles)
+    return spec
+
+
 def noh(n=16, cycles=100, dt_cfl=0.4):
     """Noh implosion (K/input.cpp:297-340): rho=1, e=1e-9, spherical inward
     velocity of speed 1 on [0,1]^3 with n^3 hexes, reflected x=y=z=0 planes,

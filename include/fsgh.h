@@ -167,7 +167,8 @@ typedef struct fsgh_fill {
     int32_t mat_id;
     double x1, x2, y1, y2, z1, z2;     /* box */
     double radius1, radius2;           /* cylinder / sphere */
-    int32_t velocity;                  /* 0 cartesian, 1 radial, 2 spherical */
+    int32_t velocity;                  /* 0 cartesian, 1 radial, 2 spherical, 3 and 4 no-ops,
+                                          5 Taylor-Green vortex (K/setup.cpp:326-350) */
     int32_t pad_;
     double u, v, w, speed;
     double sie, den;
@@ -178,7 +179,11 @@ typedef struct fsgh_fill {
  * num_materials x num_state_vars; mat_num_state_vars[m] of them are copied.
  * 2D: V is the RZ volume per radian and the node mass is the areal mass
  * sum over the node's corners of (Barlow corner area x density)
- * (K/setup.cpp:365-420); velocity kinds 0 and 1 (2: FSGH_E_UNSUPPORTED). */
+ * (K/setup.cpp:365-420); velocity kinds 0, 1 and 5 (2: FSGH_E_UNSUPPORTED).
+ * Kind 5 also sets the element's pressure 0.25 (cos 2 pi x + cos 2 pi y) + 1
+ * and its energy p / (rho (statev[4] - 1)) (the material's state variable 4,
+ * zero when it has fewer than five), and leaves the sound speed of the
+ * fill's energy, as the reference does. */
 int fsgh_setup_fills(fsgh_ctx *ctx, int32_t instance, int32_t num_fills,
                      const fsgh_fill *fills, int32_t num_materials,
                      int32_t num_state_vars, const double *state_vars,

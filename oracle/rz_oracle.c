@@ -735,10 +735,13 @@ int64_t orz_build_bdy_sets(const double *xy, int64_t num_bdy_patches, const int6
 
 /* setup fills, 2D (K/setup.cpp:106-357): centroid of the 4 nodes (third
  * coordinate 0), velocities on 2 components; velocity kinds 0 (cartesian), 1
- * (radial), 3 and 4 (no-ops) */
+ * (radial), 3 and 4 (no-ops), 5 (Taylor-Green vortex, :326-350: the two
+ * in-plane components, and the element pressure and energy with gamma read
+ * from state variable 4) */
 void orz_setup_fills(const orz_mesh *m, orc_state *s, int64_t num_fills, const orc_fill *fills,
                      const double *state_vars, const int64_t *mat_num_state_vars)
 {
+    const double PI = 3.14159265358979323846;   /* K/mesh.h:8 */
     const int64_t N = m->num_nodes, E = m->num_elems;
     for (int64_t f = 0; f < num_fills; ++f) {
         const orc_fill *F = &fills[f];
@@ -789,7 +792,16 @@ void orz_setup_fills(const orz_mesh *m, orc_state *s, int64_t num_fills, const o
                     for (int d = 0; d < 2; ++d) dir[d] = rv > 1.0e-14 ? dir[d] / rv : 0.0;
                     VC(s, N, 1, n, 0) = F->speed * dir[0];
                     VC(s, N, 1, n, 1) = F->speed * dir[1];
+                } else if (F->velocity == 5) {
+                    VC(s, N, 1, n, 0) = sin(PI * XC(s, N, 1, n, 0)) * cos(PI * XC(s, N, 1, n, 1));
+                    VC(s, N, 1, n, 1) =
+                        -1.0 * cos(PI * XC(s, N, 1, n, 0)) * sin(PI * XC(s, N, 1, n, 1));
                 }
+            }
+            if (F->velocity == 5) {
+                s->elem_pres[e] = 0.25 * (cos(2.0 * PI * c[0]) + cos(2.0 * PI * c[1])) + 1.0;
+                double gamma = s->elem_statev[e * s->num_state_vars + 4];
+                s->elem_sie[E + e] = s->elem_pres[e] / (F->den * (gamma - 1.0));
             }
         }
     }

@@ -315,3 +315,33 @@ def test_user_eos_region(oracle):
         assert np.allclose(pr.elem_pres[ok], (g[ok] - 1) * pr.elem_sie[1][ok] * pr.elem_den[ok],
                            rtol=1e-15, atol=0)
         pr.stage(1e-4, 0.5)
+
+
+@pytest.mark.parametrize("dims", [3, 2])
+@pytest.mark.parametrize("statev4", [None, 0.0])
+def test_taylor_green_fill(oracle, dims, statev4):
+    """Velocity kind 5 (K/setup.cpp:326-350) against its closed form: node
+    velocity (sin pi x cos pi y, -cos pi x sin pi y, 0) with the reflected
+    walls' normal components zeroed, pressure 0.25 (cos 2 pi x_c + cos 2 pi
+    y_c) + 1 at the element centroid, energy p / (rho (statev[4] - 1)) --
+    negative when the material has no state variable 4, as in the reference
+    -- and the fill's sound speed left as the EOS gave it for e = 1."""
+    spec = P.taylor_green(6, dims=dims, statev4=statev4, cycles=1)
+    pr = (oracle.OracleProblem if dims == 3 else oracle.OracleRZ)(spec)
+    x = pr.node_coords[1]
+    pi = 3.141592653589793
+    v = np.zeros_like(x)
+    v[:, 0] = np.sin(pi * x[:, 0]) * np.cos(pi * x[:, 1])
+    v[:, 1] = -np.cos(pi * x[:, 0]) * np.sin(pi * x[:, 1])
+    v[np.isclose(x[:, 0], 0.0) | np.isclose(x[:, 0], 1.0), 0] = 0.0
+    v[np.isclose(x[:, 1], 0.0) | np.isclose(x[:, 1], 1.0), 1] = 0.0
+    assert np.allclose(pr.node_vel[1], v, rtol=0, atol=1e-15)
+    xc = x[pr.conn].mean(axis=1)
+    p = 0.25 * (np.cos(2 * pi * xc[:, 0]) + np.cos(2 * pi * xc[:, 1])) + 1.0
+    assert np.allclose(pr.elem_pres, p, rtol=1e-15, atol=0)
+    g4 = 5.0 / 3.0 if statev4 is None else statev4
+    assert np.allclose(pr.elem_sie[1], p / (1.0 * (g4 - 1.0)), rtol=1e-15, atol=0)
+    if statev4 == 0.0:
+        assert np.all(pr.elem_sie[1] < 0)
+    g = 5.0 / 3.0
+    assert np.allclose(pr.elem_sspd, np.sqrt(g * (g - 1) * 1.0), rtol=1e-15, atol=0)
