@@ -96,15 +96,24 @@ class InstanceStatus(C.Structure):
 _lib = None
 
 
-def load() -> C.CDLL:
-    """Load libfsgh.so; raise if it is not built."""
+# test-only variant of the library: the user strength hook runs the
+# hypoelastic test model (built with FSGH_TEST_STRENGTH_MODEL by build.py)
+STRENGTH_TEST_LIB_PATH = os.path.join(_HERE, "libfsgh_strength_test.so")
+_libs: dict = {}
+
+
+def load(path: str | None = None) -> C.CDLL:
+    """Load libfsgh.so (or the library at ``path``); raise if it is not built."""
     global _lib
-    if _lib is not None:
+    if path is None and _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"{LIB_PATH} is not built: run fierro_amd/build.py "
+    key = os.path.abspath(path or LIB_PATH)
+    if key in _libs:
+        return _libs[key]
+    if not os.path.exists(key):
+        raise RuntimeError(f"{key} is not built: run fierro_amd/build.py "
                            "(or __graft_entry__.build()) first")
-    L = C.CDLL(LIB_PATH)
+    L = C.CDLL(key)
     vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
     sig = {
         "fsgh_create": [C.POINTER(MeshDesc), vp, vp, C.c_int, C.POINTER(vp)],
@@ -161,7 +170,9 @@ def load() -> C.CDLL:
     L.fsgh_device_count.restype = C.c_int
     L.fsgh_stream.argtypes = [vp]
     L.fsgh_stream.restype = vp
-    _lib = L
+    _libs[key] = L
+    if path is None:
+        _lib = L
     return L
 
 

@@ -75,5 +75,16 @@ def build(verbose: bool = False, out: str = LIB, defines=(), sources=None) -> st
     return out
 
 
+STRENGTH_TEST_LIB = os.path.join(HERE, "libfsgh_strength_test.so")
+
+
+def build_test_variants(verbose: bool = False) -> list:
+    """Test-only builds of the same sources: libfsgh_strength_test.so, whose
+    user strength hook runs the hypoelastic test model (the product library
+    ships the reference's empty body)."""
+    return [build(verbose, out=STRENGTH_TEST_LIB, defines=("FSGH_TEST_STRENGTH_MODEL",))]
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
+    build_test_variants(verbose="-v" in sys.argv)

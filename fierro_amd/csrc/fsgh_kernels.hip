@@ -1166,9 +1166,10 @@ __global__ __launch_bounds__(kBlock) void k_force_stress(DMesh m, DState s, cons
     if (mt.strength == kStrengthHypo) {
         double L[9];
         velgrad(B, u, v, w, vol, L);
-        double p = pres, c = sspd;
+        double p = pres, c = sspd, stn[9];
+        load_stress(s.sig0, E, e, stn);
         user_strength(mat, s.sparams + kStrengthParams * mat, den, s.sie0[e], L, vol, dt, alpha,
-                      st, p, c);
+                      stn, st, p, c);
         s.pres[e] = p;
         s.sspd[e] = c;
     }
@@ -1212,8 +1213,10 @@ __global__ __launch_bounds__(kBlock) void k_energy_state_div_stress(DMesh m, DSt
         if (mt.strength == kStrengthHyper) {
             double L[9];
             velgrad(B, u1, v1, w1, vol, L);
+            double stn[9];
+            load_stress(s.sig0, E, e, stn);
             user_strength(mat, s.sparams + kStrengthParams * mat, den, s.sie0[e], L, vol, c.dt,
-                          alpha, st, p, cs);
+                          alpha, stn, st, p, cs);
             store_stress(s.sig1, E, e, st);
         }
         eos(mt.eos, s.gamma[e], s.csmin[e], den, sie, p, cs, st);
@@ -1254,8 +1257,10 @@ __global__ __launch_bounds__(kBlock) void k_update_state_stress(DMesh m, DState 
         gather3(s.u1, s.v1, s.w1, nd, u, v, w);
         hex_bmatrix(x, y, z, B);
         velgrad(B, u, v, w, s.vol[e], L);
+        double stn[9];
+        load_stress(s.sig0, E, e, stn);
         user_strength(mat, s.sparams + kStrengthParams * mat, den, s.sie0[e], L, s.vol[e], dt,
-                      alpha, st, p, c);
+                      alpha, stn, st, p, c);
         store_stress(s.sig1, E, e, st);
     }
     eos(mt.eos, s.gamma[e], s.csmin[e], den, s.sie1[e], p, c, st);

@@ -228,9 +228,11 @@ __global__ __launch_bounds__(kB) void k_rz_force(DMesh m, DState s, const InstCt
         if (mt.strength == kStrengthHypo) {
             double L[9];
             rz_velgrad(B, u, v, s.vol[e], area, L);
-            double p = s.pres[e], c = s.sspd[e];
+            double p = s.pres[e], c = s.sspd[e], stn[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) stn[k] = s.sig0[static_cast<size_t>(k) * E + e];
             user_strength(mat, s.sparams + kStrengthParams * mat, s.den[e], s.sie0[e], L, s.vol[e],
-                          dt, alpha, st, p, c);
+                          dt, alpha, stn, st, p, c);
             s.pres[e] = p;
             s.sspd[e] = c;
         }

@@ -60,6 +60,7 @@ class ProblemSpec:
     eos: list = None                # per material: EOS_IDEAL_GAS (default) or EOS_USER
     dims: int = 3                   # 2: axisymmetric RZ (mesh kinds "rect", "geo2")
     strength: list = None           # per material: STRENGTH_NONE (default), _HYPO or _HYPER
+    strength_params: list = None    # per material: up to 8 parameters of its strength model
 
     def mesh_arrays(self, oracle_gen=None, oracle_read=None):
         """(coords N x dims, nodes_in_elem E x 2^dims).  The oracle passes its

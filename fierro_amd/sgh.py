@@ -72,9 +72,11 @@ def pack(meshes):
 
 
 class SGHSolver:
-    def __init__(self, specs, device: int = 0, renumber: bool = True, meshes=None):
-        self.L = _abi.load()
-        if _abi.load().fsgh_device_count() < 1:
+    def __init__(self, specs, device: int = 0, renumber: bool = True, meshes=None,
+                 library: str | None = None):
+        # library: another build of the ABI (tests load the strength-model test variant)
+        self.L = _abi.load(library)
+        if self.L.fsgh_device_count() < 1:
             raise RuntimeError("no HIP device visible to libfsgh.so")
         if not isinstance(specs, (list, tuple)):
             specs = [specs]
@@ -114,6 +116,9 @@ class SGHSolver:
                     check(self.L.fsgh_set_material_models(self.ctx, index[i] + k, int(kind),
                                                           int(st)),
                           self.ctx, "set_material_models")
+            for k, params in enumerate(getattr(s, "strength_params", None) or []):
+                if len(params):
+                    self.set_strength_params(index[i] + k, params)
         for i, s in enumerate(self.specs):
             bcs = (_abi.Boundary * max(len(s.bcs), 1))(
                 *[_abi.Boundary(int(b[0]), int(b[2]), float(b[1])) for b in s.bcs])

@@ -38,7 +38,7 @@ class OrcState(C.Structure):
                 ("elem_sie", P), ("elem_vol", P), ("elem_div", P),
                 ("elem_mass", P), ("elem_mat_id", P), ("elem_statev", P),
                 ("num_state_vars", C.c_int64), ("corner_force", P), ("mat_eos", P),
-                ("elem_stress", P), ("mat_strength", P)]
+                ("elem_stress", P), ("mat_strength", P), ("mat_strength_params", P)]
 
 
 class OrcMaterial(C.Structure):
@@ -131,8 +131,11 @@ def load_library(path: str) -> C.CDLL:
     return L
 
 
-def lib(fast: bool = False) -> C.CDLL:
-    name = "liboracle_sgh_fast.so" if fast else "liboracle_sgh.so"
+def lib(fast: bool = False, strength_model: bool = False) -> C.CDLL:
+    """The parity build, the timing build (fast) or the parity build whose
+    user strength hook runs the hypoelastic test model (strength_model)."""
+    name = ("liboracle_sgh_strength.so" if strength_model else
+            "liboracle_sgh_fast.so" if fast else "liboracle_sgh.so")
     if name not in _LIBS:
         path = os.path.join(_HERE, name)
         if not os.path.exists(path):
@@ -222,8 +225,9 @@ def bdy_sets(coords, cx, bcs):
 class OracleProblem:
     """One SGH problem in the reference layout, set up as K/driver.cpp does."""
 
-    def __init__(self, spec, fast: bool = False, coords=None, conn=None):
-        self.L = lib(fast)
+    def __init__(self, spec, fast: bool = False, coords=None, conn=None,
+                 strength_model: bool = False):
+        self.L = lib(fast, strength_model)
         self.spec = spec
         if coords is None:
             coords, conn = spec.mesh_arrays(oracle_gen=gen_box, oracle_read=read_ensight)
@@ -255,7 +259,8 @@ class OracleProblem:
                                _ptr(self.elem_sie), _ptr(self.elem_vol), _ptr(self.elem_div),
                                _ptr(self.elem_mass), _ptr(self.elem_mat_id),
                                _ptr(self.elem_statev), nsv, _ptr(self.corner_force), None, None,
-                               None)
+                               None, None)
+        self._set_strength_params(spec)
         mats = spec.materials
         if spec.eos:
             self.mat_eos = np.ascontiguousarray(spec.eos, np.int64)
@@ -265,6 +270,16 @@ class OracleProblem:
             self.enable_stress()
         self._mat = (OrcMaterial * len(mats))(*[OrcMaterial(*m) for m in mats])
         self.setup()
+
+    def _set_strength_params(self, spec):
+        """Per-material strength parameters (ProblemSpec.strength_params)."""
+        sp = getattr(spec, "strength_params", None)
+        if sp:
+            a = np.zeros((len(spec.materials), 8))
+            for k, row in enumerate(sp):
+                a[k, :len(row)] = row
+            self.mat_strength_params = a
+            self._state.mat_strength_params = _ptr(a)
 
     def enable_stress(self):
         """Carry a stress field [2][E][3][3] (zero, as K/setup.cpp:214-220
@@ -389,8 +404,8 @@ class OrzMesh(C.Structure):
                 ("nodes_in_node_start", P), ("nodes_in_node", P)]
 
 
-def _rz_lib(fast=False):
-    L = lib(fast)
+def _rz_lib(fast=False, strength_model=False):
+    L = lib(fast, strength_model)
     if not getattr(L, "_rz_ready", False):
         L.orz_get_vol_quad.restype = C.c_double
         L.orz_get_area_quad.restype = C.c_double
@@ -484,8 +499,9 @@ class OracleRZ:
     """One 2D RZ problem in the reference layout (num_dims = 2), set up and
     run as K/driver.cpp and K/sgh_solve.cpp do on that branch."""
 
-    def __init__(self, spec, fast: bool = False, coords=None, conn=None):
-        self.L = _rz_lib(fast)
+    def __init__(self, spec, fast: bool = False, coords=None, conn=None,
+                 strength_model: bool = False):
+        self.L = _rz_lib(fast, strength_model)
         self.spec = spec
         if coords is None:
             coords, conn = spec.mesh_arrays(oracle_gen=gen_rect, oracle_read=read_ensight_2d)
@@ -520,7 +536,8 @@ class OracleRZ:
                                _ptr(self.elem_sie), _ptr(self.elem_vol), _ptr(self.elem_div),
                                _ptr(self.elem_mass), _ptr(self.elem_mat_id),
                                _ptr(self.elem_statev), nsv, _ptr(self.corner_force), None, None,
-                               None)
+                               None, None)
+        OracleProblem._set_strength_params(self, spec)
         if spec.eos:
             self.mat_eos = np.ascontiguousarray(spec.eos, np.int64)
             self._state.mat_eos = _ptr(self.mat_eos)

@@ -231,9 +231,9 @@ void orz_get_force(const orz_mesh *m, orc_state *s, const orc_material *mat, dou
                 v[a] = vel[a][1];
             }
             orz_get_velgrad2D(L, u, v, B, s->elem_vol[e], elem_area);
-            orc_user_strength(s->elem_mat_id[e], den, s->elem_sie[m->num_elems + e], L,
-                              s->elem_vol[e], dt, rk_alpha, stress, &s->elem_pres[e],
-                              &s->elem_sspd[e]);
+            orc_user_strength(s->elem_mat_id[e], orc_strength_params(s, e), den, s->elem_sie[e],
+                              L, s->elem_vol[e], dt, rk_alpha, s->elem_stress + e * 9, stress,
+                              &s->elem_pres[e], &s->elem_sspd[e]);
         }
     }
 }

@@ -179,12 +179,23 @@ void orc_get_divergence(const orc_mesh *m, orc_state *s)
     }
 }
 
-void orc_user_strength(int64_t mat_id, double den, double sie, const double vel_grad[9],
-                       double vol, double dt, double rk_alpha, double stress[9], double *pres,
-                       double *sspd)
+void orc_user_strength(int64_t mat_id, const double *params, double den, double sie,
+                       const double vel_grad[9], double vol, double dt, double rk_alpha,
+                       const double stress_n[9], double stress[9], double *pres, double *sspd)
 {
-    (void)mat_id; (void)den; (void)sie; (void)vel_grad; (void)vol; (void)dt; (void)rk_alpha;
-    (void)stress; (void)pres; (void)sspd;
+    (void)mat_id; (void)den; (void)sie; (void)vol; (void)pres; (void)sspd;
+#ifdef ORC_TEST_STRENGTH_MODEL
+    const double G = params[0];
+    const double tr = vel_grad[0] + vel_grad[4] + vel_grad[8];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double d = 0.5 * (vel_grad[3 * i + j] + vel_grad[3 * j + i]) -
+                             (i == j ? tr / 3.0 : 0.0);
+            stress[3 * i + j] = stress_n[3 * i + j] + rk_alpha * dt * 2.0 * G * d;
+        }
+#else
+    (void)params; (void)vel_grad; (void)dt; (void)rk_alpha; (void)stress_n; (void)stress;
+#endif
 }
 
 void orc_get_velgrad(double L[9], const double u[8], const double v[8], const double w[8],
@@ -301,8 +312,9 @@ void orc_get_force_sgh(const orc_mesh *m, orc_state *s, const orc_material *mat,
                 w[a] = vel[a][2];
             }
             orc_get_velgrad(L, u, v, w, B, vol);
-            orc_user_strength(s->elem_mat_id[e], den, s->elem_sie[m->num_elems + e], L, vol, dt,
-                              rk_alpha, stress, &s->elem_pres[e], &s->elem_sspd[e]);
+            orc_user_strength(s->elem_mat_id[e], orc_strength_params(s, e), den, s->elem_sie[e], L,
+                              vol, dt, rk_alpha, s->elem_stress + e * 9, stress, &s->elem_pres[e],
+                              &s->elem_sspd[e]);
         }
     }
 }
@@ -420,8 +432,9 @@ void orc_update_state(const orc_mesh *m, orc_state *s, double dt, double rk_alph
                 w[a] = s->node_vel[(N + n) * 3 + 2];
             }
             orc_get_velgrad(L, u, v, w, B, s->elem_vol[e]);
-            orc_user_strength(s->elem_mat_id[e], s->elem_den[e], s->elem_sie[E + e], L,
-                              s->elem_vol[e], dt, rk_alpha, s->elem_stress + (E + e) * 9,
+            orc_user_strength(s->elem_mat_id[e], orc_strength_params(s, e), s->elem_den[e],
+                              s->elem_sie[e], L, s->elem_vol[e], dt, rk_alpha,
+                              s->elem_stress + e * 9, s->elem_stress + (E + e) * 9,
                               &s->elem_pres[e], &s->elem_sspd[e]);
         }
         ideal_gas(s, E, e, s->elem_den[e], s->elem_sie[E + e]);

@@ -60,7 +60,19 @@ typedef struct orc_state {
     const int64_t *mat_eos; /* per material: 0 ideal gas, 1 user model; NULL = all ideal gas */
     double *elem_stress;   /* 2 x E x 3 x 3 Cauchy stress (K/state.h:64), or NULL = zero */
     const int64_t *mat_strength; /* per material: 0 none, 1 hypo, 2 hyper; NULL = none */
+    const double *mat_strength_params; /* per material ORC_STRENGTH_PARAMS values, or NULL */
 } orc_state;
+
+#define ORC_STRENGTH_PARAMS 8
+
+/* the strength parameters of element e's material (zeros when none are set) */
+static inline const double *orc_strength_params(const orc_state *s, int64_t e)
+{
+    static const double zeros[ORC_STRENGTH_PARAMS] = {0.0};
+    return s->mat_strength_params
+               ? s->mat_strength_params + s->elem_mat_id[e] * ORC_STRENGTH_PARAMS
+               : zeros;
+}
 
 /* material_t (K/state.h:130-178), ideal-gas EOS, no strength model */
 typedef struct orc_material {
@@ -119,10 +131,15 @@ typedef struct orc_run_result {
 } orc_run_result;
 
 /* ---- strength model hook (K/user_mat.cpp:47-90) ---- */
-/* user_strength_model restated: the reference ships an empty body. */
-void orc_user_strength(int64_t mat_id, double den, double sie, const double vel_grad[9],
-                       double vol, double dt, double rk_alpha, double stress[9], double *pres,
-                       double *sspd);
+/* user_strength_model restated: the reference ships an empty body.  sie is
+ * the energy at t_n (the reference passes elem_sie(elem_gid), bin 0 of the
+ * [2][E] array), stress_n the t_n stress (bin 0), stress the current stress,
+ * updated in place.  Built with ORC_TEST_STRENGTH_MODEL (the oracle's test
+ * variant) the body is the hypoelastic update
+ * stress = stress_n + rk_alpha dt 2 G dev((L + L^T)/2), G = params[0]. */
+void orc_user_strength(int64_t mat_id, const double *params, double den, double sie,
+                       const double vel_grad[9], double vol, double dt, double rk_alpha,
+                       const double stress_n[9], double stress[9], double *pres, double *sspd);
 /* get_velgrad (K/momentum.cpp:59-144): L_ij = sum_p u_p^i B_pj / V with the
  * outward B */
 void orc_get_velgrad(double L[9], const double u[8], const double v[8], const double w[8],

@@ -99,3 +99,54 @@ def test_stressed_run_conserves_energy(make):
     q = O.OracleRZ(make()) if spec.dims == 2 else O.OracleProblem(make())
     q.solve()
     assert not np.allclose(q.node_coords[1], pr.node_coords[1])
+
+
+def hypo_spec(G):
+    spec = P.noh(4, cycles=1)
+    spec.strength = [P.STRENGTH_HYPO]
+    spec.strength_params = [[G]]
+    return spec
+
+
+@pytest.mark.parametrize("rotation", [False, True])
+def test_hypoelastic_test_model_closed_form(rotation):
+    """The oracle's test strength model (its ORC_TEST_STRENGTH_MODEL build)
+    through the hypo call site of get_force_sgh (K/force_sgh.cpp:369-391):
+    on a linear velocity field u = A x + b the velocity gradient is exactly A,
+    so every element's new stress is sigma_n + alpha dt 2 G dev(sym A) --
+    built from the t_n stress (bin 0), not the current one (bin 1) -- and a
+    rigid rotation (A antisymmetric) leaves sigma_n."""
+    G, dt, alpha = 0.7, 1e-3, 0.5
+    pr = O.OracleProblem(hypo_spec(G), strength_model=True)
+    A = RNG.standard_normal((3, 3))
+    if rotation:
+        A = 0.5 * (A - A.T)
+    X = pr.node_coords[1] + 0.03 * RNG.standard_normal(pr.node_coords[1].shape)
+    pr.node_coords[1] = X
+    pr.node_vel[1] = X @ A.T + RNG.standard_normal(3)
+    pr.get_vol()
+    pr.get_divergence()
+    s_n, s_cur = sym(), sym()
+    pr.elem_stress[0] = s_n
+    pr.elem_stress[1] = s_cur
+    pr.get_force_sgh(dt, alpha)
+    D = 0.5 * (A + A.T)
+    expect = s_n + alpha * dt * 2.0 * G * (D - np.trace(D) / 3.0 * np.eye(3))
+    assert np.allclose(pr.elem_stress[1], expect[None], rtol=0, atol=1e-12)
+    assert np.array_equal(pr.elem_stress[0], np.broadcast_to(s_n, pr.elem_stress[0].shape))
+    if rotation:
+        assert np.allclose(pr.elem_stress[1], s_n[None], rtol=0, atol=1e-12)
+
+
+def test_hypoelastic_run_conserves_energy():
+    """A stressed run with the test model (the stress now evolving) still
+    conserves total energy to round-off: the stress does work only through
+    the corner forces the energy update uses."""
+    spec = P.sedov(6, cycles=60)
+    spec.strength = [P.STRENGTH_HYPO]
+    spec.strength_params = [[0.05]]
+    pr = O.OracleProblem(spec, strength_model=True)
+    r = pr.solve()
+    assert r["cycles"] == 60
+    assert np.abs(pr.elem_stress[1]).max() > 0       # the model did act
+    assert abs(r["te1"] - r["te0"]) <= 1e-13 * abs(r["te0"])
