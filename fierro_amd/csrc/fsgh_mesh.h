@@ -63,8 +63,10 @@ void block_table(const std::vector<int64_t>& off, std::vector<BlockItem>& table,
                  std::vector<int32_t>& inst_begin, int32_t bs = kBlock, int32_t deal = 1);
 
 // Boundary node sets of one instance (K/setup.cpp:431-665): nodes of the
-// boundary faces whose 4 nodes all satisfy the surface test.  Each set is
-// returned sorted by caller node id.
+// boundary faces whose 4 nodes all satisfy the surface test.  Each set holds
+// its nodes in the reference's first-seen order: boundary faces by ascending
+// caller element id, then face, then the face's node order (K/mesh.h:353-376,
+// K/setup.cpp:591-642).
 int boundary_sets(const Layout& L, int32_t inst, const int64_t* nodes_in_elem,
                   const double* coords, const std::vector<int64_t>& cstart,
                   const std::vector<int64_t>& corners, int32_t nsets, const fsgh_boundary* bc,

@@ -182,11 +182,15 @@ int boundary_sets(const Layout& L, int32_t inst, const int64_t* conn, const doub
                     shared = hit == fnodes - 1;
                 }
                 if (shared) continue;
-                for (int q = 0; q < fnodes; ++q) mark[static_cast<size_t>(fn[q] - n0)] = 1;
+                // nodes in first-seen order over (element, face, face node), the
+                // order build_boundry_node_sets stores them in (K/setup.cpp:591-642)
+                for (int q = 0; q < fnodes; ++q)
+                    if (!mark[static_cast<size_t>(fn[q] - n0)]) {
+                        mark[static_cast<size_t>(fn[q] - n0)] = 1;
+                        out[static_cast<size_t>(s)].push_back(fn[q]);
+                    }
             }
         }
-        for (int64_t n = n0; n < n1; ++n)
-            if (mark[static_cast<size_t>(n - n0)]) out[static_cast<size_t>(s)].push_back(n);
     }
     return 0;
 }

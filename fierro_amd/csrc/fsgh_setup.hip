@@ -186,9 +186,14 @@ __global__ void k_bc_on(const double* xyz, const int32_t* on_of_nn, int64_t nn0,
 }
 
 // faces with all four nodes on the surface and held by no other element
-// mark their nodes (node lids of the faces: K/mesh.h:353-358)
+// mark their nodes (node lids of the faces: K/mesh.h:353-358).  The mark is
+// the smallest position at which the reference's walk meets the node,
+// (caller element - first caller element of the instance) * 24 + face * 4 +
+// face node, so that sorting a set by it gives the reference's first-seen
+// order (K/setup.cpp:591-642); unmarked nodes keep ~0.
 __global__ void k_bc_faces(const int32_t* iconn, const int32_t* cin_start, const int32_t* cin,
-                           const uint8_t* on, int64_t ne0, int64_t ne1, uint8_t* mark)
+                           const uint8_t* on, const int32_t* oe_of_ne, int64_t e0,
+                           int64_t ne0, int64_t ne1, unsigned long long* mark)
 {
     const int64_t e = ne0 + static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     if (e >= ne1) return;
@@ -217,19 +222,23 @@ __global__ void k_bc_faces(const int32_t* iconn, const int32_t* cin_start, const
             shared = hit == 3;
         }
         if (shared) continue;
-        for (int q = 0; q < 4; ++q) mark[fn[q]] = 1;
+        const unsigned long long pos =
+            static_cast<unsigned long long>(oe_of_ne[e] - e0) * 24ull + 4ull * f;
+        for (int q = 0; q < 4; ++q) atomicMin(mark + fn[q], pos + q);
     }
 }
 
 // flag per original node id of one instance (scattered from the internal
 // ids), and the original ids themselves
-__global__ void k_bc_flag(const uint8_t* mark, const int32_t* on_of_nn, int64_t n0, int64_t n1,
-                          uint8_t* flag, int64_t* ids)
+__global__ void k_bc_flag(const unsigned long long* mark, const int32_t* on_of_nn, int64_t n0,
+                          int64_t n1, uint8_t* flag, int64_t* ids, unsigned long long* pos)
 {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     const int64_t nn = n0 + k;
     if (nn >= n1) return;
-    flag[on_of_nn[nn] - n0] = mark[nn];
+    const int64_t o = on_of_nn[nn] - n0;
+    flag[o] = mark[nn] != ~0ull;
+    pos[o] = mark[nn];
     ids[k] = nn;
 }
 
@@ -559,7 +568,8 @@ int gpu_boundary_sets(fsgh_ctx* c, int32_t inst, int32_t nsets, const fsgh_bound
     out.assign(static_cast<size_t>(nsets), {});
     if (nsets == 0) return FSGH_OK;
     DBuf<double> xyz;
-    DBuf<uint8_t> on, mark, flag;
+    DBuf<uint8_t> on, flag;
+    DBuf<unsigned long long> mark, pos, selpos;
     DBuf<int64_t> ids, sel;
     DBuf<unsigned> nsel;
     SETUP_HIP(xyz.alloc(static_cast<size_t>(3 * N)));
@@ -568,31 +578,48 @@ int gpu_boundary_sets(fsgh_ctx* c, int32_t inst, int32_t nsets, const fsgh_bound
     SETUP_HIP(flag.alloc(static_cast<size_t>(NI)));
     SETUP_HIP(ids.alloc(static_cast<size_t>(NI)));
     SETUP_HIP(sel.alloc(static_cast<size_t>(NI)));
+    SETUP_HIP(pos.alloc(static_cast<size_t>(NI)));
+    SETUP_HIP(selpos.alloc(static_cast<size_t>(NI)));
     SETUP_HIP(nsel.alloc(1));
     SETUP_HIP(hipMemcpyAsync(xyz.p, c->coords.data(), static_cast<size_t>(3 * N) * 8,
                              hipMemcpyHostToDevice, st));
-    size_t sb = 0;
+    size_t sb = 0, sb2 = 0;
     SETUP_HIP(rocprim::select(nullptr, sb, ids.p, flag.p, sel.p, nsel.p, static_cast<size_t>(NI),
                               st));
+    SETUP_HIP(rocprim::select(nullptr, sb2, pos.p, flag.p, selpos.p, nsel.p,
+                              static_cast<size_t>(NI), st));
+    sb = std::max(sb, sb2);
     DBuf<char> tmp;
     SETUP_HIP(tmp.alloc(sb));
     for (int32_t s = 0; s < nsets; ++s) {
         hipLaunchKernelGGL(k_bc_on, dim3(grid(static_cast<size_t>(NI))), dim3(kB), 0, st, xyz.p,
                            c->d_on_of_nn, nn0, nn1, bc[s].surface, bc[s].value, on.p);
-        SETUP_HIP(hipMemsetAsync(mark.p, 0, static_cast<size_t>(N), st));
+        SETUP_HIP(hipMemsetAsync(mark.p, 0xff, static_cast<size_t>(N) * 8, st));
         hipLaunchKernelGGL(k_bc_faces, dim3(grid(static_cast<size_t>(ne1 - ne0))), dim3(kB), 0, st,
-                           c->d_conn, c->d_cin_start, c->d_cin, on.p, ne0, ne1, mark.p);
+                           c->d_conn, c->d_cin_start, c->d_cin, on.p, c->d_oe_of_ne, e0, ne0, ne1,
+                           mark.p);
         hipLaunchKernelGGL(k_bc_flag, dim3(grid(static_cast<size_t>(NI))), dim3(kB), 0, st, mark.p,
-                           c->d_on_of_nn, n0, n1, flag.p, ids.p);
+                           c->d_on_of_nn, n0, n1, flag.p, ids.p, pos.p);
         SETUP_HIP(rocprim::select(tmp.p, sb, ids.p, flag.p, sel.p, nsel.p, static_cast<size_t>(NI),
                                   st));
+        SETUP_HIP(rocprim::select(tmp.p, sb, pos.p, flag.p, selpos.p, nsel.p,
+                                  static_cast<size_t>(NI), st));
         unsigned count = 0;
         SETUP_HIP(hipMemcpyAsync(&count, nsel.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
         SETUP_HIP(hipStreamSynchronize(st));
+        std::vector<int64_t> node(count);
+        std::vector<unsigned long long> at(count);
+        SETUP_HIP(hipMemcpyAsync(node.data(), sel.p, count * 8, hipMemcpyDeviceToHost, st));
+        SETUP_HIP(hipMemcpyAsync(at.data(), selpos.p, count * 8, hipMemcpyDeviceToHost, st));
+        SETUP_HIP(hipStreamSynchronize(st));
+        // first-seen order: by the position of each node's first appearance
+        std::vector<uint32_t> order(count);
+        for (uint32_t k = 0; k < count; ++k) order[k] = k;
+        std::sort(order.begin(), order.end(),
+                  [&](uint32_t a, uint32_t b) { return at[a] < at[b]; });
         auto& v = out[static_cast<size_t>(s)];
         v.resize(count);
-        SETUP_HIP(hipMemcpyAsync(v.data(), sel.p, count * 8, hipMemcpyDeviceToHost, st));
-        SETUP_HIP(hipStreamSynchronize(st));
+        for (uint32_t k = 0; k < count; ++k) v[k] = node[order[k]];
     }
     return FSGH_OK;
 }

@@ -359,6 +359,24 @@ int fsgh_autotune(fsgh_ctx* c, int32_t reps, fsgh_tuning* out)
         ~Guard() { hipFree(p); }
     } guard{S.buf};
     if (int rc = copy_parts(c, S, true)) return rc;
+    // every exit from here on, an error return included, puts the fields and
+    // the control blocks back (the trial launches run on the live state);
+    // declared after guard, so it runs before the copy is freed
+    struct Restore {
+        fsgh_ctx* c;
+        Snapshot* S;
+        bool done;
+        ~Restore() {
+            if (done) return;
+            size_t off = 0;
+            for (auto& p : S->parts) {
+                hipMemcpyAsync(p.first, S->buf + off, p.second, hipMemcpyDeviceToDevice,
+                               c->stream);
+                off += (p.second + 255) / 256 * 256;
+            }
+            hipStreamSynchronize(c->stream);
+        }
+    } restore{c, &S, false};
 
     // trial control block: every instance live, a small positive dt
     std::vector<InstCtl> h(static_cast<size_t>(c->L.ninst));
@@ -437,6 +455,7 @@ int fsgh_autotune(fsgh_ctx* c, int32_t reps, fsgh_tuning* out)
         best_ms[k] = t[bi];
         def_ms[k] = t[0];   // candidates start with the default (256, 1, 1)
     }
+    restore.done = true;
     if (int rc = copy_parts(c, S, false)) return rc;
     for (int k = 0; k < kNumTuned; ++k)
         if (int rc = apply(c, k, best[k])) return rc;

@@ -7,7 +7,10 @@
 
 The fields come from ``SGHSolver.download`` (reference layout, caller
 numbering), so a run on the GPU produces the same files as the reference's
-``write_outputs`` at the same state.
+``write_outputs`` at the same state.  A 2D RZ state (coordinates [2][N][2],
+4-node connectivity) is written as the reference writes it: quad4 elements,
+a zero third coordinate and velocity component, and element means over the
+element's own 4 nodes (K/outputs.cpp:131-157,184-199,232-267,459-476).
 """
 from __future__ import annotations
 
@@ -37,15 +40,31 @@ def _lines(values):
     return "".join("%12.5e\n" % v for v in values)
 
 
-def elem_speed(node_vel1, conn):
-    """|mean of the element's node velocities| (K/outputs.cpp:131-157)."""
-    v = node_vel1[conn]                      # E x 8 x 3
+def _pad3(x):
+    """N x dims -> N x 3 with a zero third column for 2D (K/outputs.cpp:186-199)."""
+    x = np.asarray(x, np.float64)
+    if x.shape[1] == 3:
+        return x
+    return np.concatenate([x, np.zeros((x.shape[0], 3 - x.shape[1]))], axis=1)
+
+
+def elem_mean(node_field1, conn):
+    """Mean of a node field over each element's nodes, summed in ascending
+    local order (K/outputs.cpp:131-150, :459-476)."""
+    v = _pad3(node_field1)[conn]             # E x nodes x 3
     acc = np.zeros((conn.shape[0], 3))
-    for a in range(8):                       # ascending node order, as the reference sums
+    for a in range(conn.shape[1]):
         acc = acc + v[:, a, :]
-    acc = acc / 8
+    return acc / conn.shape[1]
+
+
+def elem_speed(node_vel1, conn):
+    """|mean of the element's node velocities| over the mesh's num_dims
+    components (K/outputs.cpp:131-157)."""
+    dims = np.asarray(node_vel1).shape[1]
+    acc = elem_mean(node_vel1, conn)
     s = np.zeros(conn.shape[0])
-    for d in range(3):
+    for d in range(dims):
         s = s + acc[:, d] * acc[:, d]
     return np.sqrt(s)
 
@@ -64,10 +83,11 @@ class EnsightWriter:
         os.makedirs(data, exist_ok=True)
         gid = self.graphics_id
         conn = np.asarray(fields["nodes_in_elem"], np.int64)
-        x1 = fields["node_coords"][1]
-        v1 = fields["node_vel"][1]
+        speed = elem_speed(fields["node_vel"][1], conn)
+        x1 = _pad3(fields["node_coords"][1])
+        v1 = _pad3(fields["node_vel"][1])
         E, N = conn.shape[0], x1.shape[0]
-        speed = elem_speed(v1, conn)
+        etype = "hexa8\n" if conn.shape[1] == 8 else "quad4\n"
         switch = np.where(np.arange(E) % 2 == 0, 1.0, -1.0)
         scal = {"den": fields["elem_den"], "pres": fields["elem_pres"],
                 "sie": fields["elem_sie"][1], "vol": fields["elem_vol"],
@@ -86,7 +106,7 @@ class EnsightWriter:
             f.write("%10d\n" % N)
             for d in range(3):
                 f.write(_lines(x1[:, d]))
-            f.write("hexa8\n")
+            f.write(etype)
             f.write("%10d\n" % E)
             f.write("".join("".join("%10d\t" % (n + 1) for n in row) + "\n"
                             for row in conn.tolist()))
@@ -95,7 +115,7 @@ class EnsightWriter:
                 f.write("Per_elem scalar values\n")
                 f.write("part\n")
                 f.write("%10d\n" % 1)
-                f.write("hexa8\n")
+                f.write(etype)
                 f.write(_lines(scal[name]))
         vec = {"pos": x1, "vel": v1}
         for name in VECTORS:
@@ -133,11 +153,7 @@ def write_state(fields, time_value, root="."):
     os.makedirs(os.path.join(root, "state"), exist_ok=True)
     path = os.path.join(root, "state", "elem_state_t%6.5e.txt" % time_value)
     conn = np.asarray(fields["nodes_in_elem"], np.int64)
-    x1 = fields["node_coords"][1]
-    c = np.zeros((conn.shape[0], 3))
-    for a in range(8):
-        c = c + x1[conn[:, a]]
-    c = c / 8
+    c = elem_mean(fields["node_coords"][1], conn)
     rad2 = np.sqrt(c[:, 0] * c[:, 0] + c[:, 1] * c[:, 1])
     rad3 = np.sqrt(c[:, 0] * c[:, 0] + c[:, 1] * c[:, 1] + c[:, 2] * c[:, 2])
     cols = [c[:, 0], c[:, 1], c[:, 2], rad2, rad3, fields["elem_den"], fields["elem_pres"],

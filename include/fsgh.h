@@ -155,9 +155,10 @@ typedef struct fsgh_boundary {
 int fsgh_set_bcs(fsgh_ctx *ctx, int32_t instance, int32_t num_sets,
                  const fsgh_boundary *sets);
 /* number of nodes in boundary set `set` of `instance`, and the nodes (caller
- * ids, ascending).  The reference stores the same set in first-seen order of
- * its patch numbering (K/setup.cpp:575-665); the order has no effect on the
- * velocity condition, which only zeroes components. */
+ * ids) in the reference's order: first seen over the boundary patches in
+ * patch order -- ascending caller element, then face, then the face's node
+ * order -- as build_boundry_node_sets stores bdy_nodes_in_set
+ * (K/mesh.h:353-376,435-513; K/setup.cpp:591-642). */
 int fsgh_bc_set_size(fsgh_ctx *ctx, int32_t instance, int32_t set, int64_t *count);
 int fsgh_get_bc_set(fsgh_ctx *ctx, int32_t instance, int32_t set, int64_t *nodes);
 

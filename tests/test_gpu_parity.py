@@ -70,6 +70,9 @@ def test_setup_matches_oracle(oracle, Solver, case, renumber):
     assert np.array_equal(corners, pr.cx["corners_in_node"])
     sizes = np.diff(pr.bset_start)
     assert g.bc_set_sizes() == list(sizes)
+    for i in range(len(spec.bcs)):
+        # the reference's first-seen order, not only the same nodes
+        assert np.array_equal(g.bc_set(0, i), pr.bset_nodes[pr.bset_start[i]:pr.bset_start[i + 1]])
     for k, v in oracle_fields(pr).items():
         if k in ("corner_force", "elem_div"):
             continue

@@ -77,6 +77,50 @@ def test_connectivity_closed_forms(oracle, dims):
         assert set(got) == brute and len(got) == len(brute)
 
 
+FACE_NODES = [(0, 4, 7, 3), (1, 2, 6, 5), (0, 1, 5, 4), (2, 3, 7, 6), (0, 3, 2, 1), (4, 5, 6, 7)]
+
+
+def first_seen_sets(x, conn, bcs):
+    """Boundary-node sets by brute force: a face is on the boundary when its
+    node set occurs once in the mesh; walking the boundary faces by element,
+    then face (K/mesh.h:353-358 face node order), a node joins a set the first
+    time a face on that set's plane holds it."""
+    from collections import Counter
+
+    key = lambda e, f: frozenset(int(conn[e, a]) for a in FACE_NODES[f])  # noqa: E731
+    count = Counter(key(e, f) for e in range(len(conn)) for f in range(6))
+    out = []
+    for surf, val, _ in bcs:
+        seen, order = set(), []
+        for e in range(len(conn)):
+            for f in range(6):
+                fn = [int(conn[e, a]) for a in FACE_NODES[f]]
+                if count[key(e, f)] != 1 or not all(abs(x[n, surf] - val) <= 1e-7 for n in fn):
+                    continue
+                for n in fn:
+                    if n not in seen:
+                        seen.add(n)
+                        order.append(n)
+        out.append(np.array(order, np.int64))
+    return out
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2), (3, 2, 4), (5, 1, 2)])
+def test_bdy_set_order_first_seen(oracle, dims):
+    """The oracle's boundary sets (patch numbering through hash keys and
+    element neighbours, K/mesh.h:395-513) hold their nodes in the order the
+    brute-force walk meets them; the order is not ascending."""
+    spec = P.sod_tube(dims[0], 0.125, 0.1, 1.4, ny=dims[1], nz=dims[2])
+    pr = oracle.OracleProblem(spec)
+    exp = first_seen_sets(pr.coords0, pr.conn, spec.bcs)
+    unsorted = 0
+    for s in range(len(spec.bcs)):
+        got = pr.bset_nodes[pr.bset_start[s]:pr.bset_start[s + 1]]
+        assert np.array_equal(got, exp[s]), s
+        unsorted += int(np.any(np.diff(got) < 0))
+    assert unsorted > 0
+
+
 def test_bdy_sets_geometry(oracle):
     spec = P.reference_sedov(os.path.join(GOLD, "mesh_Sedov_8.geo"))
     pr = oracle.OracleProblem(spec)
@@ -101,7 +145,8 @@ def test_library_maps_match_oracle(oracle, dims, renumber):
     assert np.array_equal(cin, cx["corners_in_node"])
     bs, bn, _ = oracle.bdy_sets(mm.coords0, cx, spec.bcs)
     for s in range(len(spec.bcs)):
-        assert np.array_equal(mm.bc_set(0, s), np.sort(bn[bs[s]:bs[s + 1]]))
+        # same nodes in the same (first-seen) order
+        assert np.array_equal(mm.bc_set(0, s), bn[bs[s]:bs[s + 1]])
     e, n = mm.renumbering()
     assert np.array_equal(np.sort(e), np.arange(mm.E))
     assert np.array_equal(np.sort(n), np.arange(mm.N))
@@ -139,6 +184,10 @@ def test_batched_maps(oracle):
         assert np.array_equal(np.sort(e[e0:e1]), np.arange(e0, e1))
         assert np.array_equal(np.sort(n[n0:n1]), np.arange(n0, n1))
         x = mm.coords0[n0:n1]
+        conn_i = mm.conn[e0:e1] - n0
+        cx = oracle.connectivity(x, conn_i)
+        bs, bn, _ = oracle.bdy_sets(x, cx, specs[i].bcs)
         for s, (surf, val, _) in enumerate(specs[i].bcs):
-            exp = n0 + np.nonzero(np.abs(x[:, surf] - val) <= 1e-7)[0]
-            assert np.array_equal(mm.bc_set(i, s), exp)
+            exp = np.nonzero(np.abs(x[:, surf] - val) <= 1e-7)[0]
+            assert np.array_equal(np.sort(mm.bc_set(i, s)) - n0, exp)
+            assert np.array_equal(mm.bc_set(i, s) - n0, bn[bs[s]:bs[s + 1]])

@@ -58,3 +58,37 @@ def test_state_dump(oracle, tmp_path):
     assert np.allclose(rows[:, :3], cen, atol=1e-6)
     assert np.allclose(rows[:, 4], np.linalg.norm(cen, axis=1), atol=1e-6)
     assert np.allclose(rows[:, 5], pr.elem_den, atol=1e-6)
+
+
+def test_ensight_and_state_rz(oracle, tmp_path):
+    """A 2D RZ dump: quad4 elements with 4 node ids, a zero third coordinate
+    and velocity component, speed and centroid over the 4 nodes
+    (K/outputs.cpp:131-157,184-199,232-267,459-476)."""
+    pr = oracle.OracleRZ(P.sedov_rz(4, cycles=5))
+    pr.solve()
+    f = oracle_fields(pr)
+    base = outputs.EnsightWriter(str(tmp_path)).write(f, 0.0)
+    geo = open(base + ".geo").read().split("\n")
+    N, E = pr.N, pr.E
+    assert int(geo[8]) == N
+    xyz = np.array(geo[9:9 + 3 * N], float).reshape(3, N)
+    assert np.allclose(xyz[:2].T, pr.node_coords[1], rtol=1e-5, atol=1e-12)
+    assert np.all(xyz[2] == 0.0)
+    assert geo[9 + 3 * N] == "quad4" and int(geo[10 + 3 * N]) == E
+    ids = np.array([list(map(int, ln.split())) for ln in geo[11 + 3 * N:11 + 3 * N + E]])
+    assert np.array_equal(ids, pr.conn + 1)
+    assert open(base + ".den").read().split("\n")[3] == "quad4"
+    v = np.array(open(base + ".vel").read().split("\n")[4:4 + 3 * N], float).reshape(3, N)
+    assert np.allclose(v[:2].T, pr.node_vel[1], rtol=1e-5, atol=1e-12) and np.all(v[2] == 0.0)
+    speed = np.array(open(base + ".speed").read().split("\n")[4:4 + E], float)
+    vmean = pr.node_vel[1][pr.conn].mean(1)
+    assert np.allclose(speed, np.linalg.norm(vmean, axis=1), rtol=1e-5, atol=1e-12)
+    assert speed.max() > 0
+
+    path = outputs.write_state(f, 0.5, str(tmp_path))
+    rows = np.array([list(map(float, ln.split()))
+                     for ln in open(path).read().split("\n")[2:2 + E]])
+    cen = pr.node_coords[1][pr.conn].mean(1)
+    assert np.allclose(rows[:, :2], cen, atol=1e-6) and np.all(rows[:, 2] == 0.0)
+    assert np.allclose(rows[:, 3], np.linalg.norm(cen, axis=1), atol=1e-6)
+    assert np.allclose(rows[:, 4], rows[:, 3])
