@@ -146,6 +146,11 @@ TRAFFIC_KERNELS = {
               "vel_bc_pos": "k_vel_bc_pos<", "energy_state_div": "k_energy_state_div<"},
     "sedov-rz": {"rz_div_extremes": "k_div_extremes<", "rz_force": "k_rz_force",
                  "rz_node": "k_rz_node", "rz_energy_state_div": "k_rz_energy_state_div"},
+    # the reverse pass of the calibration gradient (every launch covers the batch)
+    "calibration": {"adj_energy": "k_adj_energy", "adj_node": "k_adj_node<",
+                    "arg_extremes": "k_arg_extremes", "adj_force": "k_adj_force",
+                    "adj_ext_node": "k_adj_ext_node", "adj_derived": "k_adj_derived",
+                    "adj_gather": "k_adj_gather<"},
 }
 
 
@@ -177,15 +182,21 @@ def traffic_record(workload, n, fetch_dir, write_dir, copy_fetch_dir, copy_write
     from fierro_amd import bytemodel, problems as P
     from fierro_amd.sgh import SGHSolver
 
-    spec = P.sedov_rz(n, cycles=1) if workload == "sedov-rz" else P.sedov(n, cycles=1)
-    g = SGHSolver(spec, renumber=True)
-    if workload == "sedov-rz":
-        alg = {k: fn(g.E, g.N) for k, fn in bytemodel.RZ_KERNELS.items()}
+    if workload == "calibration":
+        specs, _, _, _ = P.blast_ensemble(64, seed=0, cycles=150)
+        g = SGHSolver(specs, renumber=True)
+        alg = {k: fn(g.E, g.N) for k, fn in bytemodel.ADJ_KERNELS.items()}
+        name = "blast_calibration64"
     else:
-        kernels, _ = kernel_table(g, {}, tune(g, argparse.Namespace(tune="table")))
-        alg = {k: v["alg_bytes"] for k, v in kernels.items()}
-    rec = {"workload": f"{workload.replace('-', '_')}{n}", "elements": g.E, "nodes": g.N,
-           "calibration": cal, "kernels": {}}
+        spec = P.sedov_rz(n, cycles=1) if workload == "sedov-rz" else P.sedov(n, cycles=1)
+        g = SGHSolver(spec, renumber=True)
+        if workload == "sedov-rz":
+            alg = {k: fn(g.E, g.N) for k, fn in bytemodel.RZ_KERNELS.items()}
+        else:
+            kernels, _ = kernel_table(g, {}, tune(g, argparse.Namespace(tune="table")))
+            alg = {k: v["alg_bytes"] for k, v in kernels.items()}
+        name = f"{workload.replace('-', '_')}{n}"
+    rec = {"workload": name, "elements": g.E, "nodes": g.N, "calibration": cal, "kernels": {}}
     for key, prefix in TRAFFIC_KERNELS[workload].items():
         names = [k for k in fetch if k.startswith(prefix)]
         if len(names) != 1:
@@ -213,7 +224,7 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--workload",
                     choices=["sedov", "sedov-rz", "sod-ensemble", "anvil-adjoint",
-                             "calibration", "sedov-operators"],
+                             "calibration", "calibration-reverse", "sedov-operators"],
                     default="sedov")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--checkpoint-every", type=int, default=4,
@@ -234,8 +245,10 @@ def main():
     if args.traffic_from:
         name = "sedov_rz" if args.workload == "sedov-rz" else "sedov"
         n = args.n if args.workload == "sedov" else (args.n if args.n != 256 else 4096)
+        out = (f"{name}{n}_traffic.json" if args.workload != "calibration"
+               else "blast_calibration64_traffic.json")
         return traffic_record(args.workload, n, *args.traffic_from,
-                              os.path.join(ROOT, "profiles", f"{name}{n}_traffic.json"))
+                              os.path.join(ROOT, "profiles", out))
     if args.copy_calib:
         return copy_calibration(args.copy_calib)
     if args.gpus != 1:
@@ -249,6 +262,8 @@ def main():
         return emit(adjoint(args))
     if args.workload == "calibration":
         return emit(calibration(args))
+    if args.workload == "calibration-reverse":
+        return emit(calibration_reverse(args))
     if args.workload == "sedov-rz":
         return sedov_rz(args)
     if args.workload == "sedov-operators":
@@ -715,6 +730,55 @@ def calibration(args):
         "cpu_baseline": None,
     }
     return out
+
+
+def calibration_reverse(args):
+    """The reverse kernels of the calibration gradient (64 blasts, 150
+    cycles, --checkpoint-every), each timed alone: with per-kernel HIP events
+    on, the reverse pass keeps one checkpoint bank on the context stream (no
+    recompute overlapped with it, no captured segments) and finds the
+    divergence extremes itself (k_arg_extremes).  Per kernel: average launch
+    time, algorithmic bytes per launch (fierro_amd/bytemodel.py) and their
+    rate against 8 TB/s."""
+    from fierro_amd import bytemodel, calibration as CAL, problems as P
+    from fierro_amd.sgh import SGHSolver
+
+    cycles, every = 150, args.checkpoint_every
+    specs, params, tspec, _ = P.blast_ensemble(64, seed=0, cycles=cycles)
+    target = CAL.target_profile(SGHSolver, tspec, cycles)
+    g = SGHSolver(specs, renumber=True)
+    E, N = g.E, g.N
+    log(f"calibration reverse: {len(specs)} instances E={E} N={N}")
+    tune(g, args)
+    layout = CAL.Layout(g)
+    g.profile(True)
+    t = time.perf_counter()
+    CAL.gradient(g, params, target, cycles, every, layout=layout)
+    wall = time.perf_counter() - t
+    prof = g.profile_read()
+    g.profile(False)
+    g.close()
+    kernels = {}
+    for name, fn in bytemodel.ADJ_KERNELS.items():
+        if name not in prof:
+            continue
+        ms, calls = prof[name]
+        avg = ms / calls
+        b = fn(E, N)
+        kernels[name] = {"calls": calls, "avg_ms": avg, "alg_bytes": b,
+                         "alg_GBps": b / avg / 1e6, "frac": b / avg / 1e6 / HBM_PEAK_GBS}
+        log(f"  {name:13s} {calls:5d} x {1e3 * avg:9.1f} us  {b / 1e9:6.2f} GB  "
+            f"{b / avg / 1e6:7.0f} GB/s  {100 * b / avg / 1e6 / HBM_PEAK_GBS:5.1f} %")
+    stage = sum(k["avg_ms"] for k in kernels.values())
+    return {"metric": "reverse-pass kernels of the calibration gradient (each timed alone)",
+            "value": bytemodel.adj_stage(E, N) / stage / 1e6, "unit": "GB/s (one reversed "
+            "stage: algorithmic bytes / sum of its kernels' average times)", "n_gpus": 1,
+            "steps": cycles, "warmup": 0, "ms_per_step": 2 * stage, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded blast ensemble)",
+            "config": {"workload": "blast-calibration64-reverse", "elements": E, "nodes": N,
+                       "cycles": cycles, "checkpoint_every": every},
+            "gradient_s_one_bank": wall, "kernels": kernels, "cpu_baseline": None}
 
 
 if __name__ == "__main__":

@@ -46,6 +46,66 @@ KERNELS = {"div_extremes": div_extremes, "force": force, "vel_bc_pos": vel_bc_po
            "energy_state_div": energy_state_div}
 
 
+# ---- reverse pass of one RK stage (fsgh_adjoint.hip), accumulating form ----
+# (the first stage reversed in a cycle writes some accumulators instead of
+# adding to them; counted here as read + write throughout)
+def adj_energy(E, N):
+    # e-bar 8E + e0-bar 16E (rw) + M 8E + g 8E (w) + conn 32E + F 24C + M-bar 16E (rw)
+    # + v0, v1 48N
+    return 8 * E + 16 * E + 8 * E + 8 * E + 32 * E + 192 * E + 16 * E + 48 * N
+
+
+def adj_node(E, N):
+    # corner CSR starts 4(N+1) + sorted corner list 4C + positions 2C + F 24C + g 8E
+    # + m 8N + BC mask 1N + v0, v1 48N + x-bar, v-bar of the stage output 48N
+    # + x0-bar, v0-bar 96N (rw) + m-bar 16N (rw) + the (h, kv) records 48N (w)
+    return (4 * (N + 1) + 32 * E + 16 * E + 192 * E + 8 * E + 8 * N + N + 48 * N + 48 * N
+            + 96 * N + 16 * N + 48 * N)
+
+
+def arg_extremes(E, N):
+    # corner CSR (4C + 4(N+1)) + div 8E + the extreme pairs 32N and their
+    # elements 16N (w)
+    return 32 * E + 4 * (N + 1) + 8 * E + 48 * N
+
+
+def adj_force(E, N):
+    # conn 32E + x, v 48N + vol, div, den, sspd, pres 40E + extreme pairs and their
+    # elements 48N + g 8E + (h, kv) records 48N + B-bar rows 24C (w) + corner
+    # velocity adjoint rows 24C (w) + element adjoints 48E (w) + chosen extreme 4E (w)
+    return 32 * E + 48 * N + 40 * E + 48 * N + 8 * E + 48 * N + 192 * E + 192 * E + 48 * E + 4 * E
+
+
+def adj_ext_node(E, N):
+    # corner CSR (4C + 4(N+1)) + chosen extreme 4E + ext-bar 8E + per-slot sums 32N (w)
+    return 32 * E + 4 * (N + 1) + 4 * E + 8 * E + 32 * N
+
+
+def adj_derived(E, N):
+    # conn 32E + den, pres, sspd, vol, div adjoints 40E + extreme elements 16N
+    # + per-slot sums 32N + x, v 48N + vol, div 16E + gamma, csmin 16E + e 8E + den 8E
+    # + e-bar 16E (rw) + gamma-bar 16E (rw) + B-bar rows 24C + corner velocity rows
+    # 24C (rw) + M-bar (or rho0-bar) 16E (rw) + corner position rows 24C (w)
+    return (32 * E + 40 * E + 16 * N + 32 * N + 48 * N + 16 * E + 16 * E + 8 * E + 8 * E
+            + 16 * E + 16 * E + 192 * E + 384 * E + 16 * E + 192 * E)
+
+
+def adj_gather(E, N):
+    # corner CSR starts 4(N+1) + sorted corner list 4C + positions 2C + corner
+    # position and velocity rows 48C + x-bar, v-bar 96N (rw)
+    return 4 * (N + 1) + 32 * E + 16 * E + 384 * E + 96 * N
+
+
+ADJ_KERNELS = {"adj_energy": adj_energy, "adj_node": adj_node, "arg_extremes": arg_extremes,
+               "adj_force": adj_force, "adj_ext_node": adj_ext_node,
+               "adj_derived": adj_derived, "adj_gather": adj_gather}
+
+
+def adj_stage(E, N):
+    """Bytes of the reverse of one RK stage (every reverse kernel once)."""
+    return sum(f(E, N) for f in ADJ_KERNELS.values())
+
+
 # ---- 2D axisymmetric (RZ) path: quads, 4 corners per element, 2 components ----
 def rz_div_extremes(E, N):
     # corner CSR (16E + 4(N+1)) + div 8E + the two extreme pairs 32N (w)

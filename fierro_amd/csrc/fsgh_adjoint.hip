@@ -1459,25 +1459,39 @@ void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, cons
         A.alo = ext->alo;
         A.ahi = ext->ahi;
     }
-    hipLaunchKernelGGL(k_adj_energy, dim3(m.n_eblk), tb, 0, c->stream, m, s_out, F, A, dts,
-                       alpha, assign);
+    timed(c, "adj_energy", [&] {
+        hipLaunchKernelGGL(k_adj_energy, dim3(m.n_eblk), tb, 0, c->stream, m, s_out, F, A, dts,
+                           alpha, assign);
+    });
     const DMesh mn = adjoint_node_mesh(c);
-    hipLaunchKernelGGL(k_adj_node<kAdjNodeBlock>, dim3(mn.n_nblk), dim3(kAdjNodeBlock), 0,
-                       c->stream, mn, s_out, F, A, dts, alpha, assign);
+    timed(c, "adj_node", [&] {
+        hipLaunchKernelGGL(k_adj_node<kAdjNodeBlock>, dim3(mn.n_nblk), dim3(kAdjNodeBlock), 0,
+                           c->stream, mn, s_out, F, A, dts, alpha, assign);
+    });
     if (!ext)
-        hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), tb, 0, c->stream, m,
+        timed(c, "arg_extremes", [&] {
+            hipLaunchKernelGGL(k_arg_extremes, dim3(m.n_nblk), tb, 0, c->stream, m,
+                               static_cast<const double*>(s_in.div),
+                               static_cast<const double*>(nullptr), A.lo, A.hi, A.alo, A.ahi,
+                               static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
+        });
+    timed(c, "adj_force", [&] {
+        hipLaunchKernelGGL(k_adj_force, dim3(m.n_eblk), tb, 0, c->stream, m, s_in, A,
+                           static_cast<const InstCtl*>(c->d_ctl),
+                           static_cast<const double*>(s_in.vol),
+                           static_cast<const double*>(s_in.div));
+    });
+    timed(c, "adj_ext_node", [&] {
+        hipLaunchKernelGGL(k_adj_ext_node, dim3(m.n_nblk), tb, 0, c->stream, m, A);
+    });
+    timed(c, "adj_derived", [&] {
+        hipLaunchKernelGGL(k_adj_derived, dim3(m.n_eblk), tb, 0, c->stream, m, s_in, A,
+                           static_cast<const double*>(s_in.vol),
                            static_cast<const double*>(s_in.div),
-                           static_cast<const double*>(nullptr), A.lo, A.hi, A.alo, A.ahi,
-                           static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
-    hipLaunchKernelGGL(k_adj_force, dim3(m.n_eblk), tb, 0, c->stream, m, s_in, A,
-                       static_cast<const InstCtl*>(c->d_ctl), static_cast<const double*>(s_in.vol),
-                       static_cast<const double*>(s_in.div));
-    hipLaunchKernelGGL(k_adj_ext_node, dim3(m.n_nblk), tb, 0, c->stream, m, A);
-    hipLaunchKernelGGL(k_adj_derived, dim3(m.n_eblk), tb, 0, c->stream, m, s_in, A,
-                       static_cast<const double*>(s_in.vol), static_cast<const double*>(s_in.div),
-                       static_cast<const double*>(s_in.sie1), in.e,
-                       (cyc == 0 && stage == 0) ? 1 : 0, assign);
-    launch_adj_gather(mn, A.cbx, A.cbv, in.x, in.v, assign, c->stream);
+                           static_cast<const double*>(s_in.sie1), in.e,
+                           (cyc == 0 && stage == 0) ? 1 : 0, assign);
+    });
+    timed(c, "adj_gather", [&] { launch_adj_gather(mn, A.cbx, A.cbv, in.x, in.v, assign, c->stream); });
 }
 
 int ensure_extreme_buffers(fsgh_ctx* c)
@@ -1538,11 +1552,14 @@ int ensure_reverse_buffers(fsgh_ctx* c)
     // a second bank (recompute overlapped with the reverse pass) when there
     // is more than one segment and both banks, with the reverse pass's other
     // buffers (~110 doubles per element), take under three quarters of the
-    // free device memory
+    // free device memory; with per-kernel timing on (fsgh_profile_enable) the
+    // pass keeps one bank on the context stream, so that each reverse kernel's
+    // events time it alone
     const size_t bank_bytes = 8 * ((K + 1) * (6 * N + 6 * E) + K * (18 * N + 54 * E));
     const size_t other_bytes = 8 * (3 * (6 * N + E) + 6 * N + 90 * E);
     size_t mfree = 0, mtotal = 0;
-    const bool two = D.ncyc > D.every && hipMemGetInfo(&mfree, &mtotal) == hipSuccess &&
+    const bool two = !c->prof && D.ncyc > D.every &&
+                     hipMemGetInfo(&mfree, &mtotal) == hipSuccess &&
                      2 * bank_bytes + other_bytes < mfree / 4 * 3;
     D.nbank = two ? 2 : 1;
     int rc = small_block_tables(c);

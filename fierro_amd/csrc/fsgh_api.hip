@@ -93,24 +93,6 @@ int dalloc(fsgh_ctx* c, T** p, size_t n)
     return FSGH_OK;
 }
 
-// Launch `fn` with optional per-kernel event timing on the context stream.
-template <typename Fn>
-void timed(fsgh_ctx* c, const char* name, Fn fn)
-{
-    if (!c->prof) {
-        fn();
-        return;
-    }
-    fsgh_ctx::Rec r;
-    r.name = name;
-    hipEventCreate(&r.a);
-    hipEventCreate(&r.b);
-    hipEventRecord(r.a, c->stream);
-    fn();
-    hipEventRecord(r.b, c->stream);
-    c->recs.push_back(r);
-}
-
 void resolve_profile(fsgh_ctx* c)
 {
     if (c->recs.empty()) return;

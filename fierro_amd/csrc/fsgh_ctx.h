@@ -167,6 +167,25 @@ int gpu_boundary_sets(fsgh_ctx* c, int32_t inst, int32_t nsets, const fsgh_bound
                       std::vector<std::vector<int64_t>>& out);
 int gpu_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill* fills,
                     int32_t nmat, int32_t nsv, const double* state_vars, const int32_t* mat_nsv);
+// Launch `fn` with optional per-kernel event timing on the context stream
+// (fsgh_profile_enable); the records are resolved by fsgh_profile_read.
+template <typename Fn>
+void timed(fsgh_ctx* c, const char* name, Fn fn)
+{
+    if (!c->prof) {
+        fn();
+        return;
+    }
+    fsgh_ctx::Rec r;
+    r.name = name;
+    hipEventCreate(&r.a);
+    hipEventCreate(&r.b);
+    hipEventRecord(r.a, c->stream);
+    fn();
+    hipEventRecord(r.b, c->stream);
+    c->recs.push_back(r);
+}
+
 }  // namespace fsgh
 
 #define CHECK_CTX(c)                                   \
