@@ -81,9 +81,9 @@ def compare_instance(tag, ro, pr, rg, fields):
         assert rel(fields[k], v) <= 1e-9, (tag, k, rel(fields[k], v))
 
 
-@pytest.mark.parametrize("which", ["baseline_blast", "reference_sedov3d"])
+@pytest.mark.parametrize("which", ["point_deposit_blast", "reference_sedov3d"])
 def test_sedov32_200_cycles(oracle, Solver, which):
-    spec = (P.sedov(32, cycles=200) if which == "baseline_blast"
+    spec = (P.sedov(32, cycles=200) if which == "point_deposit_blast"
             else P.reference_sedov(n=32, cycles=200))
     print(f"[{which}] 32^3, 200 cycles", flush=True)
     pr = oracle.OracleProblem(spec, fast=True)
