@@ -150,8 +150,17 @@ __device__ __forceinline__ void rz_force_elem(const double x[4], const double y[
         }
         const double node_radius = y[a];
         if (node_radius > 1e-14) {
-            f0 += tau10 * A[a] / node_radius;
-            f1 += (tau11 - tau22) * A[a] / node_radius;
+            if (stress != nullptr) {
+                f0 += tau10 * A[a] / node_radius;
+                f1 += (tau11 - tau22) * A[a] / node_radius;
+            } else {
+                // tau10 and tau11 - tau22 are exactly zero without a stress
+                // field, and 0 A / R = 0 (A / R) for every finite A / R (the
+                // same signed zero, NaN alike): one division for both terms
+                const double q = A[a] / node_radius;
+                f0 += tau10 * q;
+                f1 += (tau11 - tau22) * q;
+            }
         }
         f[2 * a] = f0;
         f[2 * a + 1] = f1;
@@ -433,9 +442,17 @@ __global__ __launch_bounds__(kB) void k_rz_axis(DMesh m, DState s, const int2* l
     s.nmass[n] = nm;
 }
 
+// corners of a node block staged in LDS by the RZ node kernel: 6 per node
+// (4 on a structured quad mesh); blocks with more gather from global memory
+constexpr int kRzCornerCap = 6 * kB;
+
 // fused stage, node part: velocity (K/momentum.cpp:23-49), boundary
 // (K/boundary.cpp:16-55), position (K/geometry.cpp:17-24), the stage-input
-// radius for the energy kernel and the RZ areal mass of the new position
+// radius for the energy kernel and the RZ areal mass of the new position.
+// The block's corner forces are loaded into LDS in ascending corner order
+// (its sorted corner list: consecutive lanes read consecutive 16-byte
+// pieces of few elements' rows) and each node sums its own in
+// corners_in_node order, as rz_node_force does from global memory.
 __global__ __launch_bounds__(kB) void k_rz_node(DMesh m, DState s, const InstCtl* ctl,
                                                 double alpha, const double* rsrc, double* rin,
                                                 const double* ext)
@@ -443,11 +460,43 @@ __global__ __launch_bounds__(kB) void k_rz_node(DMesh m, DState s, const InstCtl
     const BlockItem bi = m.nblk[blockIdx.x];
     const InstCtl& c = ctl[bi.inst];
     if (c.stopped) return;
+    __shared__ double2 sF[kRzCornerCap];
+    const int qb = m.cin_start[bi.start];
+    const int nc = m.cin_start[bi.end] - qb;
+    const bool staged = m.cs_cor != nullptr && nc <= kRzCornerCap;   // block-uniform
+    const double2* __restrict__ F2 = reinterpret_cast<const double2*>(s.F);
+    if (staged) {
+        constexpr int R = kRzCornerCap / kB;
+        int cid[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k = r * kB + static_cast<int>(threadIdx.x);
+            cid[r] = k < nc ? m.cs_cor[qb + k] : -1;
+        }
+        double2 g[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) g[r] = F2[cid[r] < 0 ? 0 : cid[r]];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (cid[r] >= 0) sF[r * kB + static_cast<int>(threadIdx.x)] = g[r];
+        __syncthreads();
+    }
     const int n = bi.start + static_cast<int>(threadIdx.x);
     if (n >= bi.end) return;
     const double dt = c.dt;
     double f0, f1;
-    rz_node_force(m, s.F, n, f0, f1);
+    if (staged) {
+        f0 = 0.0;
+        f1 = 0.0;
+        const int q1 = m.cin_start[n + 1];
+        for (int q = m.cin_start[n]; q < q1; ++q) {
+            const double2 t = sF[m.cs_pos[q]];
+            f0 += t.x;
+            f1 += t.y;
+        }
+    } else {
+        rz_node_force(m, s.F, n, f0, f1);
+    }
     const double mass = s.nmass[n];
     const uint8_t mk = s.bcmask[n];
     const double u0 = s.u0[n], v0 = s.v0[n];

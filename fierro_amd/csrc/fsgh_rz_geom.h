@@ -41,22 +41,25 @@ FSGH_HD inline double volume(const double x[4], const double y[4])
            6.0;
 }
 
-// triangle area from its side lengths  (K/geometry.cpp:472-498)
-FSGH_HD inline double heron(double x1, double y1, double x2, double y2, double x3, double y3)
+// triangle area from its side lengths a, b, c, summed in the reference's
+// order  (K/geometry.cpp:472-498)
+FSGH_HD inline double heron(double a, double b, double c)
 {
     double S = 0.0;
-    const double a = sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
     S += a;
-    const double b = sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2));
     S += b;
-    const double c = sqrt((x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1));
     S += c;
     S *= 0.5;
     return sqrt(S * (S - a) * (S - b) * (S - c));
 }
 
 // corner areas by Barlow's partition of the four node-centroid-node
-// triangles  (K/geometry.cpp:503-543)
+// triangles  (K/geometry.cpp:503-543).  The reference's triangle (node i,
+// centroid, node i+1) has sides |centroid - node i|, |node i+1 - centroid|
+// and |node i+1 - node i|; each node-centroid length appears in two
+// triangles with opposite signs of the differences, whose squares are
+// bitwise equal (fl(a - b) = -fl(b - a)), so each is formed once: 12 square
+// roots instead of 16, with the same bits.
 FSGH_HD inline void corner_areas(const double x[4], const double y[4], double A[4])
 {
     double rc = 0.0, zc = 0.0;
@@ -64,10 +67,16 @@ FSGH_HD inline void corner_areas(const double x[4], const double y[4], double A[
         rc += 0.25 * y[a];
         zc += 0.25 * x[a];
     }
-    const double A12 = heron(x[0], y[0], zc, rc, x[1], y[1]);
-    const double A23 = heron(x[1], y[1], zc, rc, x[2], y[2]);
-    const double A34 = heron(x[2], y[2], zc, rc, x[3], y[3]);
-    const double A41 = heron(x[3], y[3], zc, rc, x[0], y[0]);
+    double d[4], e[4];
+    for (int a = 0; a < 4; ++a) {
+        const int b = (a + 1) & 3;
+        d[a] = sqrt((x[a] - zc) * (x[a] - zc) + (y[a] - rc) * (y[a] - rc));
+        e[a] = sqrt((x[b] - x[a]) * (x[b] - x[a]) + (y[b] - y[a]) * (y[b] - y[a]));
+    }
+    const double A12 = heron(d[0], d[1], e[0]);
+    const double A23 = heron(d[1], d[2], e[1]);
+    const double A34 = heron(d[2], d[3], e[2]);
+    const double A41 = heron(d[3], d[0], e[3]);
     A[0] = (5. * A41 + 5. * A12 + A23 + A34) / 12.;
     A[1] = (A41 + 5. * A12 + 5. * A23 + A34) / 12.;
     A[2] = (A41 + A12 + 5. * A23 + 5. * A34) / 12.;

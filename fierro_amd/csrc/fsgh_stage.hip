@@ -254,7 +254,7 @@ int build_node_stage(fsgh_ctx* c, int32_t bs)
 
 int build_corner_stage(fsgh_ctx* c, int32_t bs)
 {
-    if (c->host_only || c->dims != 3 || c->cstages.count(bs)) return FSGH_OK;
+    if (c->host_only || c->cstages.count(bs)) return FSGH_OK;
     if (bs != 64 && bs != 128 && bs != 256 && bs != 512) return FSGH_OK;
     std::vector<BlockItem> logical;
     std::vector<int32_t> ib;
