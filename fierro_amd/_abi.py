@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfsgh.so")
 
 FSGH_OK = 0
-RUN_GRAPH, RUN_STOP_ON_OUTPUT = 1, 2
+RUN_GRAPH, RUN_STOP_ON_OUTPUT, RUN_PERSISTENT = 1, 2, 4
 ERRORS = {-1: "FSGH_E_INVALID", -2: "FSGH_E_HIP", -3: "FSGH_E_NOMEM", -4: "FSGH_E_NONFINITE",
           -5: "FSGH_E_STATE", -6: "FSGH_E_UNSUPPORTED", -7: "FSGH_E_NOTBUILT"}
 

@@ -1274,6 +1274,16 @@ extern "C" {
 
 int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t flags)
 {
+    CHECK_CTX(c);
+    if (flags & FSGH_RUN_PERSISTENT) {
+        // switched off: the first cooperative launch of a one-launch cycle
+        // ended the process with SIGABRT and no runtime message on the
+        // MI355X, and no cause was found; the implementation was taken out
+        set_error(c, "FSGH_RUN_PERSISTENT is switched off in this build (a one-launch cycle's "
+                     "first cooperative launch aborted on the MI355X with no runtime message; "
+                     "cause not found)");
+        return FSGH_E_UNSUPPORTED;
+    }
     OP_PROLOGUE(c);
     if (!c->run_active) {
         set_error(c, "fsgh_run before fsgh_run_begin");

@@ -303,9 +303,12 @@ class SGHSolver:
     def run_begin(self):
         check(self.L.fsgh_run_begin(self.ctx, self._run_params()), self.ctx, "run_begin")
 
-    def run(self, ncycles, use_graph=True, stop_on_output=False):
+    def run(self, ncycles, use_graph=True, stop_on_output=False, persistent=False):
+        """fsgh_run; persistent: one cooperative launch per cycle
+        (FSGH_RUN_PERSISTENT), which ignores use_graph."""
         flags = (_abi.RUN_GRAPH if use_graph else 0) | \
-            (_abi.RUN_STOP_ON_OUTPUT if stop_on_output else 0)
+            (_abi.RUN_STOP_ON_OUTPUT if stop_on_output else 0) | \
+            (_abi.RUN_PERSISTENT if persistent else 0)
         check(self.L.fsgh_run(self.ctx, int(ncycles), flags), self.ctx, "run")
 
     def status(self, inst=0):
@@ -313,7 +316,7 @@ class SGHSolver:
         check(self.L.fsgh_run_status(self.ctx, inst, C.byref(st)), self.ctx, "status")
         return {k: getattr(st, k) for k, _ in _abi.InstanceStatus._fields_}
 
-    def sgh_solve(self, cycles=None, use_graph=True, output=None):
+    def sgh_solve(self, cycles=None, use_graph=True, output=None, persistent=False):
         """The cycle loop of K/sgh_solve.cpp:125-634 (fused kernels).  With
         ``output(solver, time)`` (single instance) the dumps are written at
         t=0 and at every cycle meeting the output condition, as
@@ -322,13 +325,13 @@ class SGHSolver:
         self.run_begin()
         n = cycles if cycles is not None else max(int(s.run["cycle_stop"]) for s in self.specs)
         if output is None:
-            self.run(n, use_graph)
+            self.run(n, use_graph, persistent=persistent)
         else:
             output(self, 0.0)
             done = 0
             while done < n:
                 before = self.status(0)["cycles"]
-                self.run(n - done, use_graph, stop_on_output=True)
+                self.run(n - done, use_graph, stop_on_output=True, persistent=persistent)
                 st = self.status(0)
                 done += st["cycles"] - before
                 if st["write_pending"]:

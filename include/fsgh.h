@@ -299,10 +299,16 @@ int fsgh_run_begin(fsgh_ctx *ctx, const fsgh_run_params *params);
  * per cycle; FSGH_RUN_STOP_ON_OUTPUT returns after the first cycle in which an
  * instance met the output condition (K/sgh_solve.cpp:594-606; see
  * fsgh_instance_status.write_pending) so the caller can write its dump.
+ * FSGH_RUN_PERSISTENT (each cycle as ONE cooperative launch, its phases
+ * separated by grid-wide barriers; no counterpart in the reference) is
+ * switched off in this build and returns FSGH_E_UNSUPPORTED before anything
+ * runs: its first cooperative launch on the MI355X ended the process with no
+ * runtime message, and the cause has not been found.
  * Returns when all instances stopped, on such an output cycle, or after
  * ncycles; synchronises. */
 #define FSGH_RUN_GRAPH 1
 #define FSGH_RUN_STOP_ON_OUTPUT 2
+#define FSGH_RUN_PERSISTENT 4
 int fsgh_run(fsgh_ctx *ctx, int64_t ncycles, int32_t flags);
 int fsgh_run_status(fsgh_ctx *ctx, int32_t instance, fsgh_instance_status *out);
 

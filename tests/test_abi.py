@@ -117,3 +117,17 @@ def test_tuning_entry_points_on_host_context():
         assert L.fsgh_get_tuning(ctx, None, 1) == -1
     finally:
         assert L.fsgh_destroy(ctx) == 0
+
+
+def test_persistent_flag_is_switched_off():
+    """FSGH_RUN_PERSISTENT stays off until the abort of its first cooperative
+    launch is explained: fsgh_run returns FSGH_E_UNSUPPORTED with the reason
+    before anything runs (here on a host-only context, which no other run
+    flag gets past)."""
+    from fierro_amd.sgh import MeshMaps
+
+    mm = MeshMaps(P.sedov(2, cycles=1))
+    L = _abi.load()
+    assert L.fsgh_run(mm.ctx, 1, _abi.RUN_PERSISTENT) == -6
+    assert b"switched off" in L.fsgh_last_error(mm.ctx)
+    assert L.fsgh_run(mm.ctx, 1, _abi.RUN_GRAPH) == -5
