@@ -271,8 +271,9 @@ def main():
 
     out = plain(args)
     if not args.no_extras:
-        # the metric's other parts, each on its own workload (BASELINE configs
-        # 3, 4 and 5) and the drop-in operator path at the plain mesh
+        # the metric's other parts, each on its own workload (the 512-tube
+        # ensemble, the anvil adjoint, the 64-blast calibration) and the
+        # drop-in operator path at the plain mesh
         out["batched"] = part(ensemble(args, steps=500, warmup=5),
                               ("batched_over_loop", "loop_plain", "roofline", "kernels"))
         out["adjoint"] = part(adjoint(args), ("plain_forward_s", "checkpointed_forward_s",
