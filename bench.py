@@ -191,7 +191,7 @@ def traffic_record(workload, n, fetch_dir, write_dir, copy_fetch_dir, copy_write
         spec = P.sedov_rz(n, cycles=1) if workload == "sedov-rz" else P.sedov(n, cycles=1)
         g = SGHSolver(spec, renumber=True)
         if workload == "sedov-rz":
-            alg = {k: fn(g.E, g.N) for k, fn in bytemodel.RZ_KERNELS.items()}
+            alg = rz_alg_bytes(g)
         else:
             kernels, _ = kernel_table(g, {}, tune(g, argparse.Namespace(tune="table")))
             alg = {k: v["alg_bytes"] for k, v in kernels.items()}
@@ -373,6 +373,19 @@ def plain(args):
     return out
 
 
+def rz_alg_bytes(g):
+    """Algorithmic bytes per launch of the fused RZ kernels of context g; the
+    element kernels' count the node lists of their 256-quad blocks."""
+    from fierro_amd import bytemodel
+
+    entries = g.stage_info(256)[0]
+    out = {}
+    for name, fn in bytemodel.RZ_KERNELS.items():
+        staged = name in ("rz_force", "rz_energy_state_div") and entries
+        out[name] = fn(g.E, g.N, entries) if staged else fn(g.E, g.N)
+    return out
+
+
 def sedov_rz(args):
     """2D axisymmetric blast (the reference's SedovRZ input) on n x n quads
     of [0,1.2]^2 (default 4096^2 = 16.8M zones, the zone count of the 3D
@@ -399,11 +412,12 @@ def sedov_rz(args):
     st = g.status(0)
     log(f"timed loop: {args.steps} cycles in {wall:.3f} s (graph loop {graph_s:.3f} s); {st}")
     E, N = g.E, g.N
+    alg = rz_alg_bytes(g)
     kernels = {}
-    for name, fn in bytemodel.RZ_KERNELS.items():
+    for name in bytemodel.RZ_KERNELS:
         ms, calls = prof.get(name, (0.0, 0))
         avg = ms / max(calls, 1)
-        b = fn(E, N)
+        b = alg[name]
         kernels[name] = {"avg_ms": avg, "calls": calls, "alg_bytes": b,
                          "GBs": b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0}
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["calls"])

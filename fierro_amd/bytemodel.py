@@ -112,9 +112,14 @@ def rz_div_extremes(E, N):
     return (16 * E + 4 * (N + 1)) + 8 * E + 32 * N
 
 
-def rz_force(E, N):
-    # conn 16E + x,y,u,v 32N + den,pres,sspd,div 32E + one extreme pair 16N + F 64E (w)
-    return 16 * E + 32 * N + 32 * E + 16 * N + 64 * E
+def rz_force(E, N, list_entries=None):
+    # connectivity + x,y,u,v 32N + den,pres,sspd,div 32E + the divergence-extreme
+    # pairs + F 64E (w).  With the blocks' nodes staged in LDS: 16-bit local
+    # connectivity 8E + the node lists 4 x list_entries, and both extreme pairs
+    # are staged (32N); otherwise conn 16E and one pair per node (16N).
+    if list_entries is None:
+        return 16 * E + 32 * N + 32 * E + 16 * N + 64 * E
+    return 8 * E + 4 * list_entries + 32 * N + 32 * E + 32 * N + 64 * E
 
 
 def rz_node(E, N):
@@ -123,10 +128,13 @@ def rz_node(E, N):
     return 64 * E + (16 * E + 4 * (N + 1)) + 16 * N + N + 32 * N + 8 * N + 32 * N + 8 * N + 8 * N
 
 
-def rz_energy_state_div(E, N):
-    # F 64E + conn 16E + u1,v1,u0,v0 32N + radius 8N + x1,y1 16N + M,e0,gamma,csmin 32E
-    # + e1,V,rho,p,c,div 48E (w)
-    return 64 * E + 16 * E + 32 * N + 8 * N + 16 * N + 32 * E + 48 * E
+def rz_energy_state_div(E, N, list_entries=None):
+    # F 64E + connectivity + u1,v1,u0,v0 32N + radius 8N + x1,y1 16N
+    # + M,e0,gamma,csmin 32E + e1,V,rho,p,c,div 48E (w).  Connectivity: conn 16E,
+    # or with the blocks' nodes staged in LDS the 16-bit local connectivity 8E
+    # + the node lists 4 x list_entries.
+    conn = 16 * E if list_entries is None else 8 * E + 4 * list_entries
+    return 64 * E + conn + 32 * N + 8 * N + 16 * N + 32 * E + 48 * E
 
 
 RZ_KERNELS = {"rz_div_extremes": rz_div_extremes, "rz_force": rz_force, "rz_node": rz_node,

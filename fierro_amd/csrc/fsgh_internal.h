@@ -63,8 +63,8 @@ struct DMesh {
     int32_t pad_;
     // per logical element block of this partition (bs elements): its distinct
     // nodes (sn_nodes[sn_start[b] .. sn_start[b+1]), ascending) and each
-    // corner's position in that list (sn_lconn, 8 per element); null when not
-    // built (2D, or a partition without lists)
+    // corner's position in that list (sn_lconn, 8 per hex, 4 per RZ quad);
+    // null when not built (a partition without lists)
     const int32_t* sn_start;
     const int32_t* sn_nodes;
     const uint16_t* sn_lconn;

@@ -21,6 +21,7 @@
 
 #define FSGH_HD __host__ __device__
 #include "fsgh_ctx.h"
+#include "fsgh_lds.h"
 #include "fsgh_physics.h"
 #include "fsgh_rz_geom.h"
 
@@ -247,6 +248,144 @@ __global__ __launch_bounds__(kB) void k_rz_force(DMesh m, DState s, const InstCt
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) s.sig1w[static_cast<size_t>(k) * E + e] = st[k];
+    }
+}
+
+// Strength-free RZ corner forces with the block's nodes staged in LDS.
+// Each workgroup of kB quads (a 16 x 16 Morton square: 289 distinct nodes)
+// loads its distinct nodes' x, y, u, v and both divergence-extreme pairs
+// once, consecutive lanes on consecutive list entries (lists from
+// fsgh_stage.hip), then every quad reads its four nodes from LDS through its
+// 16-bit local connectivity.  The list indices are requested first, then the
+// quad's own inputs, then every node value, and only then is LDS written
+// (vector loads retire in issue order).  The corner forces go through the
+// same LDS buffer (row stride 5 x 16 B) and leave as contiguous 16-byte
+// streaming stores.  A block whose list exceeds the capacity (e.g. a long
+// row of a lexicographic order) gathers from global memory instead: a
+// block-uniform branch with the same arithmetic, so the same bits.
+constexpr int kRzStageCap = 2 * kB;   // distinct nodes per block held in LDS
+
+__global__ __launch_bounds__(kB) void k_rz_force_staged(DMesh m, DState s, const InstCtl* ctl,
+                                                        double fuzz, double small)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    if (ctl != nullptr) {
+        if (ctl[bi.inst].stopped) return;
+        fuzz = ctl[bi.inst].fuzz;
+        small = ctl[bi.inst].small;
+    }
+    constexpr int CAP = kRzStageCap;
+    constexpr int R = CAP / kB;
+    // x, y, u, v as [4][CAP] doubles, then the extreme pairs as [2][CAP]
+    // double2; reused for the corner-force rows
+    constexpr int kWords = 4 * CAP / 2 + 2 * CAP > kB * 5 ? 4 * CAP / 2 + 2 * CAP : kB * 5;
+    __shared__ double2 smem[kWords];
+    double* sd = reinterpret_cast<double*>(smem);
+    double2* sx = smem + 2 * CAP;
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    const bool live = e < bi.end;
+    const int ec = live ? e : bi.start;   // in bounds; results of idle lanes unused
+    int q0, nn;
+    stage_range(m, bi, CAP, q0, nn);
+    const bool staged = nn > 0 && nn <= CAP;
+    uint32_t nid[R];
+    double g[R][8];   // x, y, u, v, ext_lo pair, ext_hi pair
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * kB + static_cast<int>(threadIdx.x);
+            nid[r] = static_cast<uint32_t>(m.sn_nodes[q0 + (i < nn ? i : nn - 1)]);
+        }
+    }
+    const double div = s.div[ec];
+    const double den = s.den[ec];
+    const double pres = s.pres[ec];
+    const double sspd = s.sspd[ec];
+    uint2 lraw;
+    int4 craw;
+    if (staged) lraw = *reinterpret_cast<const uint2*>(m.sn_lconn + 4 * static_cast<size_t>(ec));
+    else craw = reinterpret_cast<const int4*>(m.conn)[ec];
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (wave_in_list(r * kB, nn)) {
+                const uint32_t n = nid[r];
+                g[r][0] = s.x1[n];
+                g[r][1] = s.y1[n];
+                g[r][2] = s.u1[n];
+                g[r][3] = s.v1[n];
+                const double2 lo = s.ext_lo[n], hi = s.ext_hi[n];
+                g[r][4] = lo.x;
+                g[r][5] = lo.y;
+                g[r][6] = hi.x;
+                g[r][7] = hi.y;
+            }
+        }
+    }
+    const Material mt = s.nmat > 1 ? s.mats[s.mat[ec]] : s.mats[0];
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * kB + static_cast<int>(threadIdx.x);
+            if (i < nn) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) sd[k * CAP + i] = g[r][k];
+                sx[i] = make_double2(g[r][4], g[r][5]);
+                sx[CAP + i] = make_double2(g[r][6], g[r][7]);
+            }
+        }
+        __syncthreads();
+    }
+    double f[8];
+    if (live) {
+        int32_t nd[4];
+        double x[4], y[4], u[4], v[4];
+        if (staged) {
+            nd[0] = static_cast<int32_t>(lraw.x & 0xffffu);
+            nd[1] = static_cast<int32_t>(lraw.x >> 16);
+            nd[2] = static_cast<int32_t>(lraw.y & 0xffffu);
+            nd[3] = static_cast<int32_t>(lraw.y >> 16);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                x[a] = sd[nd[a]];
+                y[a] = sd[CAP + nd[a]];
+                u[a] = sd[2 * CAP + nd[a]];
+                v[a] = sd[3 * CAP + nd[a]];
+            }
+        } else {
+            nd[0] = craw.x;
+            nd[1] = craw.y;
+            nd[2] = craw.z;
+            nd[3] = craw.w;
+            gather2(s.x1, s.y1, nd, x, y);
+            gather2(s.u1, s.v1, nd, u, v);
+        }
+        // neighbour extreme of the shock detector (as rz_neighbour_extreme)
+        const bool lo = div + small >= 0.0;
+        double ext = lo ? HUGE_VAL : -HUGE_VAL;
+        const double2* __restrict__ xs = staged ? (lo ? sx : sx + CAP) : (lo ? s.ext_lo : s.ext_hi);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const double2 t = xs[nd[a]];
+            const double other = (t.x == div) ? t.y : t.x;
+            ext = lo ? fmin(ext, other) : fmax(ext, other);
+        }
+        double B[8], area;
+        rz_force_elem(x, y, u, v, den, pres, sspd, div, ext, mt, fuzz, small, nullptr, B, area, f);
+    }
+    if (staged) __syncthreads();   // every node read is done before smem is reused
+    if (live) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            smem[threadIdx.x * 5 + a] = make_double2(f[2 * a], f[2 * a + 1]);
+    }
+    __syncthreads();
+    const int cnt = 4 * (bi.end - bi.start);
+    double2* out = reinterpret_cast<double2*>(s.F + 8 * static_cast<size_t>(bi.start));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int idx = r * kB + static_cast<int>(threadIdx.x);
+        if (idx < cnt) stream_store(out + idx, smem[(idx >> 2) * 5 + (idx & 3)]);
     }
 }
 
@@ -570,6 +709,141 @@ __global__ __launch_bounds__(kB) void k_rz_energy_state_div(DMesh m, DState s, c
     }
 }
 
+// The same fused element pass for a context without a stress field, with the
+// block's nodes staged in LDS (lists as in k_rz_force_staged): per distinct
+// node the stage-output coordinates and velocities, the mean stage velocity
+// (v1 + v0) * 0.5 of each component (formed once per node, the same
+// operations the power takes per corner) and the stage-input radius.  The
+// list indices are requested first, then the quad's force row and inputs,
+// then the node values; LDS is written after every load is issued.  Blocks
+// over the capacity gather from global memory (block-uniform, same bits).
+__global__ __launch_bounds__(kB) void k_rz_energy_state_div_staged(DMesh m, DState s,
+                                                                   const InstCtl* ctl, double alpha,
+                                                                   const double* rin,
+                                                                   double* dt_partial)
+{
+    const BlockItem bi = m.eblk[blockIdx.x];
+    const InstCtl& c = ctl[bi.inst];
+    if (c.stopped) return;
+    constexpr int CAP = kRzStageCap;
+    constexpr int R = CAP / kB;
+    constexpr int NV = 7;   // x1, y1, u1, v1, mean u, mean v, radius
+    __shared__ double sd[NV * CAP];
+    const int e = bi.start + static_cast<int>(threadIdx.x);
+    const bool live = e < bi.end;
+    const int ec = live ? e : bi.start;
+    int q0, nn;
+    stage_range(m, bi, CAP, q0, nn);
+    const bool staged = nn > 0 && nn <= CAP;
+    uint32_t nid[R];
+    if (staged) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * kB + static_cast<int>(threadIdx.x);
+            nid[r] = static_cast<uint32_t>(m.sn_nodes[q0 + (i < nn ? i : nn - 1)]);
+        }
+    }
+    double f[8];
+    rz_load_forces(s, ec, f);
+    const double emass = s.emass[ec];
+    const double sie0 = s.sie0[ec];
+    const double gamma = s.gamma[ec];
+    const double csmin = s.csmin[ec];
+    uint2 lraw;
+    int4 craw;
+    if (staged) lraw = *reinterpret_cast<const uint2*>(m.sn_lconn + 4 * static_cast<size_t>(ec));
+    else craw = reinterpret_cast<const int4*>(m.conn)[ec];
+    if (staged) {
+        double g[R][NV];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (wave_in_list(r * kB, nn)) {
+                const uint32_t n = nid[r];
+                g[r][0] = s.x1[n];
+                g[r][1] = s.y1[n];
+                g[r][2] = s.u1[n];
+                g[r][3] = s.v1[n];
+                g[r][4] = s.u0[n];
+                g[r][5] = s.v0[n];
+                g[r][6] = rin[n];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r * kB + static_cast<int>(threadIdx.x);
+            if (i < nn) {
+                sd[i] = g[r][0];
+                sd[CAP + i] = g[r][1];
+                sd[2 * CAP + i] = g[r][2];
+                sd[3 * CAP + i] = g[r][3];
+                sd[4 * CAP + i] = (g[r][2] + g[r][4]) * 0.5;
+                sd[5 * CAP + i] = (g[r][3] + g[r][5]) * 0.5;
+                sd[6 * CAP + i] = g[r][6];
+            }
+        }
+        __syncthreads();
+    }
+    double dtl = DBL_MAX;
+    if (live) {
+        double x[4], y[4], u1[4], v1[4], hu[4], hv[4], r[4], B[8];
+        if (staged) {
+            int32_t nd[4];
+            nd[0] = static_cast<int32_t>(lraw.x & 0xffffu);
+            nd[1] = static_cast<int32_t>(lraw.x >> 16);
+            nd[2] = static_cast<int32_t>(lraw.y & 0xffffu);
+            nd[3] = static_cast<int32_t>(lraw.y >> 16);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                x[a] = sd[nd[a]];
+                y[a] = sd[CAP + nd[a]];
+                u1[a] = sd[2 * CAP + nd[a]];
+                v1[a] = sd[3 * CAP + nd[a]];
+                hu[a] = sd[4 * CAP + nd[a]];
+                hv[a] = sd[5 * CAP + nd[a]];
+                r[a] = sd[6 * CAP + nd[a]];
+            }
+        } else {
+            const int32_t nd[4] = {craw.x, craw.y, craw.z, craw.w};
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                x[a] = s.x1[nd[a]];
+                y[a] = s.y1[nd[a]];
+                u1[a] = s.u1[nd[a]];
+                v1[a] = s.v1[nd[a]];
+                hu[a] = (u1[a] + s.u0[nd[a]]) * 0.5;
+                hv[a] = (v1[a] + s.v0[nd[a]]) * 0.5;
+                r[a] = rin[nd[a]];
+            }
+        }
+        // power in rz_power's order: per corner the axial then the radial term
+        double pw = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            pw += f[2 * a] * r[a] * hu[a];
+            pw += f[2 * a + 1] * r[a] * hv[a];
+        }
+        const double sie = sie0 - alpha * c.dt / emass * pw;
+        const double vol = rz::volume(x, y);
+        const double den = emass / vol;
+        double p, cs;
+        eos(s.mats[s.nmat > 1 ? s.mat[ec] : 0].eos, gamma, csmin, den, sie, p, cs);
+        rz::bmatrix(x, y, B);
+        const double div = rz::divergence(B, u1, v1, rz::area(x, y), vol);
+        s.sie1[e] = sie;
+        s.vol[e] = vol;
+        s.den[e] = den;
+        s.pres[e] = p;
+        s.sspd[e] = cs;
+        s.div_out[e] = div;
+        if (dt_partial != nullptr)
+            dtl = rz_elem_dt(rz::min_pair_distance(x, y), cs, c.dt_cfl, c.dt_max, c.dt_min, c.fuzz);
+    }
+    if (dt_partial != nullptr) {
+        dtl = rz_block_min(dtl);
+        if (threadIdx.x == 0) dt_partial[bi.lid] = dtl;
+    }
+}
+
 unsigned grid_of(int32_t n) { return static_cast<unsigned>((n + kB - 1) / kB); }
 
 }  // namespace
@@ -590,6 +864,9 @@ void launch_rz_force(const DMesh& m, const DState& s, const InstCtl* ctl, double
     if (s.sig1 != nullptr)
         hipLaunchKernelGGL(k_rz_force<true>, dim3(m.n_eblk), dim3(kB), 0, st, m, s, ctl, fuzz, small,
                            dt, alpha);
+    else if (m.bs == kB)
+        hipLaunchKernelGGL(k_rz_force_staged, dim3(m.n_eblk), dim3(kB), 0, st, m, s, ctl, fuzz,
+                           small);
     else
         hipLaunchKernelGGL(k_rz_force<false>, dim3(m.n_eblk), dim3(kB), 0, st, m, s, ctl, fuzz,
                            small, dt, alpha);
@@ -645,7 +922,11 @@ void launch_rz_node(const DMesh& m, const DState& s, const InstCtl* ctl, double 
 void launch_rz_energy_state_div(const DMesh& m, const DState& s, const InstCtl* ctl, double alpha,
                                 const double* rin, double* dt_partial, hipStream_t st)
 {
-    if (m.n_eblk)
+    if (!m.n_eblk) return;
+    if (s.sig1 == nullptr && m.bs == kB)
+        hipLaunchKernelGGL(k_rz_energy_state_div_staged, dim3(m.n_eblk), dim3(kB), 0, st, m, s, ctl,
+                           alpha, rin, dt_partial);
+    else
         hipLaunchKernelGGL(k_rz_energy_state_div, dim3(m.n_eblk), dim3(kB), 0, st, m, s, ctl, alpha,
                            rin, dt_partial);
 }

@@ -15,9 +15,10 @@
 // blocks numbered as BlockItem::lid), this file builds
 //   start[b] .. start[b+1]  the block's distinct internal node ids, ascending
 //                           (nodes[]),
-//   lconn[8 e + l]          the position of node l of element e in its
-//                           block's list (16 bits),
-// on the GPU: one workgroup per block sorts the block's 8 BS node ids in LDS
+//   lconn[npe e + l]        the position of node l of element e in its
+//                           block's list (16 bits; npe = 8 nodes per hex,
+//                           4 per RZ quad),
+// on the GPU: one workgroup per block sorts the block's npe BS node ids in LDS
 // (bitonic), marks the first of each run, ranks them by a block scan, and
 // finds each corner's rank by binary search.  The kernels that read them fall
 // back to global gathers for a block whose list exceeds their LDS capacity,
@@ -43,8 +44,9 @@ namespace {
 
 constexpr int kStageThreads = 256;
 
-// KMAX = power of two >= 8 x block size; one workgroup of 256 threads per block
-template <int KMAX>
+// KMAX = power of two >= NPE x block size (NPE nodes per element: 8 for
+// hexes, 4 for RZ quads); one workgroup of 256 threads per block
+template <int KMAX, int NPE>
 __global__ __launch_bounds__(kStageThreads) void k_stage_lists(const BlockItem* blocks,
                                                                const int32_t* conn,
                                                                int32_t* counts,
@@ -54,8 +56,8 @@ __global__ __launch_bounds__(kStageThreads) void k_stage_lists(const BlockItem* 
     __shared__ int32_t key[KMAX];
     __shared__ int32_t part[kStageThreads + 1];
     const BlockItem b = blocks[blockIdx.x];
-    const int nk = 8 * (b.end - b.start);
-    const int32_t* src = conn + 8 * static_cast<size_t>(b.start);
+    const int nk = NPE * (b.end - b.start);
+    const int32_t* src = conn + NPE * static_cast<size_t>(b.start);
     for (int i = threadIdx.x; i < KMAX; i += kStageThreads) key[i] = i < nk ? src[i] : INT32_MAX;
     __syncthreads();
     // bitonic sort, ascending
@@ -114,7 +116,7 @@ __global__ __launch_bounds__(kStageThreads) void k_stage_lists(const BlockItem* 
     for (int k = 0; k < kChunk; ++k)
         if (mine[k] >= 0) key[mine[k]] = val[k];
     __syncthreads();
-    uint16_t* lc = lconn + 8 * static_cast<size_t>(b.start);
+    uint16_t* lc = lconn + NPE * static_cast<size_t>(b.start);
     for (int i = threadIdx.x; i < nk; i += kStageThreads) {
         const int32_t v = src[i];
         int lo = 0, hi = nu - 1;
@@ -168,19 +170,19 @@ __global__ __launch_bounds__(kStageThreads) void k_corner_stage(const BlockItem*
     }
 }
 
-template <int KMAX>
+template <int KMAX, int NPE>
 void launch_stage(int32_t nb, const BlockItem* blocks, const int32_t* conn, int32_t* counts,
                   const int32_t* start, int32_t* nodes, uint16_t* lconn, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_stage_lists<KMAX>, dim3(nb), dim3(kStageThreads), 0, st, blocks, conn,
-                       counts, start, nodes, lconn);
+    hipLaunchKernelGGL((k_stage_lists<KMAX, NPE>), dim3(nb), dim3(kStageThreads), 0, st, blocks,
+                       conn, counts, start, nodes, lconn);
 }
 
 }  // namespace
 
 int build_node_stage(fsgh_ctx* c, int32_t bs)
 {
-    if (c->host_only || c->dims != 3 || c->stages.count(bs)) return FSGH_OK;
+    if (c->host_only || c->stages.count(bs)) return FSGH_OK;
     if (bs != 64 && bs != 128 && bs != 256 && bs != 512) return FSGH_OK;
     std::vector<BlockItem> logical;
     std::vector<int32_t> ib;
@@ -204,17 +206,28 @@ int build_node_stage(fsgh_ctx* c, int32_t bs)
         return fail("stage counts");
     if (hipMalloc(reinterpret_cast<void**>(&ns.start), (nb + 1) * sizeof(int32_t)) != hipSuccess)
         return fail("stage start");
-    if (hipMalloc(reinterpret_cast<void**>(&ns.lconn), 8 * static_cast<size_t>(c->E) * sizeof(uint16_t)) !=
+    const int npe = c->dims == 2 ? 4 : 8;   // nodes per element
+    if (hipMalloc(reinterpret_cast<void**>(&ns.lconn), npe * static_cast<size_t>(c->E) * sizeof(uint16_t)) !=
         hipSuccess)
         return fail("stage lconn");
     HIPCHK(c, hipMemcpyAsync(d_blocks, logical.data(), nb * sizeof(BlockItem), hipMemcpyHostToDevice,
                              c->stream));
     auto run = [&](int32_t* counts, const int32_t* start, int32_t* nodes) {
+        const int32_t* cn = c->d_conn;
+        if (npe == 4) {
+            switch (bs) {
+            case 64: launch_stage<256, 4>(nb, d_blocks, cn, counts, start, nodes, ns.lconn, c->stream); break;
+            case 128: launch_stage<512, 4>(nb, d_blocks, cn, counts, start, nodes, ns.lconn, c->stream); break;
+            case 256: launch_stage<1024, 4>(nb, d_blocks, cn, counts, start, nodes, ns.lconn, c->stream); break;
+            default: launch_stage<2048, 4>(nb, d_blocks, cn, counts, start, nodes, ns.lconn, c->stream); break;
+            }
+            return;
+        }
         switch (bs) {
-        case 64: launch_stage<512>(nb, d_blocks, c->d_conn, counts, start, nodes, ns.lconn, c->stream); break;
-        case 128: launch_stage<1024>(nb, d_blocks, c->d_conn, counts, start, nodes, ns.lconn, c->stream); break;
-        case 256: launch_stage<2048>(nb, d_blocks, c->d_conn, counts, start, nodes, ns.lconn, c->stream); break;
-        default: launch_stage<4096>(nb, d_blocks, c->d_conn, counts, start, nodes, ns.lconn, c->stream); break;
+        case 64: launch_stage<512, 8>(nb, d_blocks, cn, counts, start, nodes, ns.lconn, c->stream); break;
+        case 128: launch_stage<1024, 8>(nb, d_blocks, cn, counts, start, nodes, ns.lconn, c->stream); break;
+        case 256: launch_stage<2048, 8>(nb, d_blocks, cn, counts, start, nodes, ns.lconn, c->stream); break;
+        default: launch_stage<4096, 8>(nb, d_blocks, cn, counts, start, nodes, ns.lconn, c->stream); break;
         }
     };
     run(d_counts, nullptr, nullptr);

@@ -102,7 +102,7 @@ int fsgh_get_device_csr(fsgh_ctx *ctx, int32_t *start, int32_t *corners);
  * the LDS-staged element kernels for element blocks of `elem_block`
  * elements (64, 128, 256 or 512): total list entries (sum over blocks of
  * their distinct nodes) and the largest block's count; zeros when not built
- * (2D contexts, or a block size no kernel configuration uses yet). */
+ * (a block size no kernel configuration uses yet; 2D contexts build 256). */
 int fsgh_get_stage_info(fsgh_ctx *ctx, int32_t elem_block, int64_t *list_entries,
                         int32_t *max_block_nodes);
 

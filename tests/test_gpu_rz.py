@@ -139,6 +139,31 @@ def test_rz_fused_cycles(oracle, Solver, case, use_graph):
         assert rel(g.download(k), fields(pr)[k]) <= 1e-9, k
 
 
+def test_rz_long_rows(oracle, Solver):
+    """Rows of 600 quads in the caller's (lexicographic) order give element
+    blocks of 256 quads whose distinct nodes (514) exceed the element
+    kernels' LDS capacity (512): those blocks gather from global memory,
+    next to blocks that fit; the Morton order stages every block.  Both runs
+    against the oracle after 60 fused cycles, and bitwise equal to each
+    other (each node keeps its caller corner order, so only where the node
+    values are read from differs)."""
+    spec = P.sod_rz(600, 3, cycles=60)
+    pr = oracle.OracleRZ(spec)
+    ro = pr.solve()
+    out = {}
+    for renumber in (False, True):
+        g = Solver(spec, renumber=renumber)
+        rg = g.sgh_solve(use_graph=True)[0]
+        assert rg["cycles"] == ro["cycles"] == 60
+        assert abs(rg["time_value"] - ro["time_value"]) <= 1e-14 * ro["time_value"]
+        assert abs(rg["te1"] - ro["te1"]) <= 1e-12 * abs(ro["te1"])
+        out[renumber] = {k: g.download(k) for k in STATE}
+        for k in STATE:
+            assert rel(out[renumber][k], fields(pr)[k]) <= 1e-9, (renumber, k)
+    for k in STATE:
+        assert np.array_equal(out[False][k], out[True][k]), k
+
+
 def test_rz_batched_equal_plain(Solver):
     specs = [P.sedov_rz(9, cycles=40), P.noh_rz(6, cycles=30), P.sod_rz(30, 3, cycles=50)]
     batch = Solver(specs, renumber=True)
