@@ -560,8 +560,10 @@ __device__ __forceinline__ void insert_max2(double& a, double& b, double v)
 // Per node, the two smallest and two largest divergences of the elements
 // around it, counted with multiplicity (feeds the shock detector).  A NaN
 // divergence is left out, as fmin/fmax leave it out of the reference's
-// neighbour minimum.
-template <int BS>
+// neighbour minimum.  NC: corners read unconditionally (clamped to the list)
+// before the loop over longer lists: 8 for hexes, 4 for RZ quads (a node of
+// a quad mesh has at most 4 corners unless elements are repeated).
+template <int BS, int NC = 8>
 __global__ __launch_bounds__(BS) void k_div_extremes(DMesh m, DState s, const InstCtl* ctl)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
@@ -572,16 +574,16 @@ __global__ __launch_bounds__(BS) void k_div_extremes(DMesh m, DState s, const In
     double mn1 = HUGE_VAL, mn2 = HUGE_VAL, mx1 = -HUGE_VAL, mx2 = -HUGE_VAL;
     if (cnt > 0) {
         const int last = cnt - 1;
-        double d[8];
+        double d[NC];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) d[j] = s.div[m.cin[q0 + (j < cnt ? j : last)] >> m.cshift];
+        for (int j = 0; j < NC; ++j) d[j] = s.div[m.cin[q0 + (j < cnt ? j : last)] >> m.cshift];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < NC; ++j) {
             const bool ok = j < cnt && !isnan(d[j]);
             insert_min2(mn1, mn2, ok ? d[j] : HUGE_VAL);
             insert_max2(mx1, mx2, ok ? d[j] : -HUGE_VAL);
         }
-        for (int q = q0 + 8; q < q0 + cnt; ++q) {
+        for (int q = q0 + NC; q < q0 + cnt; ++q) {
             const double v = s.div[m.cin[q] >> m.cshift];
             if (!isnan(v)) {
                 insert_min2(mn1, mn2, v);
@@ -1422,6 +1424,11 @@ void launch_force(const DMesh& m, const DState& s, const InstCtl* ctl, double fu
 void launch_div_extremes(const DMesh& m, const DState& s, const InstCtl* ctl, hipStream_t st)
 {
     if (!m.n_nblk) return;
+    if (m.cshift == 2 && m.bs == kBlock) {   // RZ quads: at most 4 corners per node
+        hipLaunchKernelGGL((k_div_extremes<kBlock, 4>), dim3(m.n_nblk), dim3(kBlock), 0, st, m,
+                           s, ctl);
+        return;
+    }
 #define FSGH_VL(B)                                                                          \
     if (m.bs == B) {                                                                       \
         hipLaunchKernelGGL((k_div_extremes<B>), dim3(m.n_nblk), dim3(B), 0, st, m, s, ctl);  \
