@@ -16,7 +16,7 @@
 //                  the node (CSR order)                  -> x0-bar, v0-bar, F-bar, m-bar
 //   k_arg_extremes per node, the two smallest / largest divergences with the
 //                  elements holding them
-//   k_adj_force    corner forces (MARS + shock detector) -> per-corner (x-bar, v-bar),
+//   k_adj_force_pair corner forces (MARS + shock detector) -> per-corner (x-bar, v-bar),
 //                  element (den, pres, sspd, vol, div, ext) adjoints, ext's element
 //   k_adj_ext_node per node and extreme slot, the ext-adjoints of the elements
 //                  that chose it (each element then adds the sums of the slots
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(kB) void k_adj_energy(DMesh m, DState s, const doub
 // v1 = mask(v0 + alpha dt F/m), x1 = x0 + alpha dt (v1 + v0)/2, and the
 // energy update's corner terms: (v1+v0)/2-bar = sum_c g_e F_c over the
 // node's corners (list order).  Writes the node's h = (v1+v0)/2 and
-// kv = alpha dt/m v1-bar (masked), from which k_adj_force forms each
+// kv = alpha dt/m v1-bar (masked), from which k_adj_force_pair forms each
 // corner's F_c-bar = g_e h + kv.  The block's corner forces and their
 // elements' g are staged in LDS in sorted-corner order (coalesced loads, as
 // the forward velocity kernel stages its forces); a block over the capacity
@@ -635,23 +635,72 @@ __global__ __launch_bounds__(kB) void k_adj_ext_node(DMesh m, AdjView A)
     out[1] = make_double2(acc[2], acc[3]);
 }
 
-// Reverse of force_elem (K/force_sgh.cpp:35-359) at the stage input state.
-// One wave of 64 elements per workgroup; lanes past the work item compute on
-// the item's first element and store nothing (every lane takes part in the
-// row transposes).
-__global__ __launch_bounds__(64) void k_adj_force(DMesh m, DState s, AdjView A, const InstCtl* ctl,
-                                                  const double* vol_a, const double* div_a)
+// ---- one element over a pair of lanes ----
+// Lanes 2i and 2i + 1 of a wave share element i; lane half h = lane & 1
+// holds corners 4h .. 4h + 3.  A sum over the element's corners is formed
+// per half and the two halves added (a + b == b + a, so both lanes hold the
+// same value).
+__device__ __forceinline__ double pair_sum(double v) { return v + __shfl_xor(v, 1, 64); }
+
+// hi when h, else lo, by bit masks: the same value as h ? hi : lo, written so
+// the compiler cannot turn a choice between two array elements into a
+// lane-dependent index (which would move the array to scratch memory)
+__device__ __forceinline__ double pick(int h, double lo, double hi)
 {
-    __shared__ double2 rbuf[kRowBuf];
+    const long long m = -static_cast<long long>(h);
+    return __longlong_as_double((__double_as_longlong(lo) & ~m) | (__double_as_longlong(hi) & m));
+}
+__device__ __forceinline__ int32_t pick(int h, int32_t lo, int32_t hi)
+{
+    const int32_t m = -h;
+    return (lo & ~m) | (hi & m);
+}
+
+// Half rows (12 doubles: corners 4h .. 4h + 3 of element i) held by the lane
+// pair (2i, 2i + 1) of a wave -> memory.  The wave's 32 elements own one
+// contiguous stretch of 6 KB of the corner array, of which lane l holds bytes
+// [96 l, 96 l + 96); the lanes store it as consecutive 16-byte pieces
+// (coalesced) through a padded LDS region of 64 x 7 pieces.  e0: the wave's
+// first element; nvalid: its elements inside the work item (rows past it are
+// not stored).
+__device__ __forceinline__ void wave_store_half_rows(double* dst_, int e0, int nvalid,
+                                                     const double f[12], double2* buf)
+{
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    double2* dst = reinterpret_cast<double2*>(dst_ + 24 * static_cast<size_t>(e0));
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 6; ++k) buf[lane * 7 + k] = make_double2(f[2 * k], f[2 * k + 1]);
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        const int p = 64 * r + lane;   // piece within the wave's stretch
+        if (p / 12 < nvalid) dst[p] = buf[(p / 6) * 7 + p % 6];
+    }
+}
+
+// Reverse of force_elem (K/force_sgh.cpp:35-359) at the stage input state,
+// with each element's corners split over a lane pair (two waves of 32
+// elements per 64-element work item): a lane holds the per-corner values of
+// four corners, so the kernel runs two waves per SIMD (one element per lane
+// needed 450 registers, one wave).  The B matrix and the mean velocity are
+// formed from all eight nodes in both lanes (the mean velocity in the forward
+// kernel's order); the other sums over corners are formed per half and the
+// halves added.  Lanes past the work item compute on the item's first
+// element and store nothing (every lane takes part in the row transposes).
+__global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, AdjView A,
+                                                           const InstCtl* ctl, const double* vol_a,
+                                                           const double* div_a)
+{
+    __shared__ double2 hbuf[2 * 64 * 7];
     const BlockItem bi = m.eblk[blockIdx.x];
-    const bool live = bi.start + static_cast<int>(threadIdx.x) < bi.end;
-    const int e = live ? bi.start + static_cast<int>(threadIdx.x) : bi.start;
+    const int h = static_cast<int>(threadIdx.x) & 1;
+    const int el = static_cast<int>(threadIdx.x) >> 1;
+    const bool live = bi.start + el < bi.end;
+    const int e = live ? bi.start + el : bi.start;
     const double fuzz = ctl[bi.inst].fuzz, small = ctl[bi.inst].small;
     int32_t nd[8];
     ld_conn(m.conn, e, nd);
-    double x[8], y[8], z[8], vel[3][8];
-    g3(s.x1, s.y1, s.z1, nd, x, y, z);
-    g3(s.u1, s.v1, s.w1, nd, vel[0], vel[1], vel[2]);
     const double vol = vol_a[e], div = div_a[e], den = s.den[e], sspd = s.sspd[e],
                  pres = s.pres[e];
     const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
@@ -660,29 +709,50 @@ __global__ __launch_bounds__(64) void k_adj_force(DMesh m, DState s, AdjView A, 
     int code;
     const double ext = neighbour_extreme(nd, e, low, A.lo, A.hi, A.alo, A.ahi, arg, nullptr,
                                          nullptr, nullptr, &code);
+    int32_t n4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) n4[k] = pick(h, nd[k], nd[4 + k]);
     // ---- forward recompute ----
-    double B[24];
-    hex_bmatrix(x, y, z, B);
-    double vbar[3] = {0.0, 0.0, 0.0};
+    double B4[12];
+    {
+        double x[8], y[8], z[8], B[24];
+        g3(s.x1, s.y1, s.z1, nd, x, y, z);
+        hex_bmatrix(x, y, z, B);
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 12; ++i) B4[i] = pick(h, B[i], B[12 + i]);
+    }
+    double vbar[3] = {0.0, 0.0, 0.0}, vel4[3][4];
+    {
+        double vel[3][8];
+        g3(s.u1, s.v1, s.w1, nd, vel[0], vel[1], vel[2]);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) vbar[d] += 0.125 * vel[d][a];
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) vbar[d] += 0.125 * vel[d][a];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) vel4[d][k] = pick(h, vel[d][k], vel[d][4 + k]);
+    }
     const double qa = div < 0.0 ? mt.q1 : mt.q1ex;
     const double qb = div < 0.0 ? mt.q2 : mt.q2ex;
-    // kept per corner: |v - v_bar|, |B_a|, mu |B_a|; the rest is recomputed
-    double mv[8], nb[8], Mt[8], S[3] = {0.0, 0.0, 0.0}, W = 0.0;
+    double mv[4], nb[4], Mt[4], S[3] = {0.0, 0.0, 0.0}, W = 0.0;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const double d0 = vel[0][a] - vbar[0], d1 = vel[1][a] - vbar[1], d2 = vel[2][a] - vbar[2];
-        mv[a] = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-        const double muc = den * (qa * sspd + qb * mv[a]);
-        nb[a] = sqrt(B[3 * a] * B[3 * a] + B[3 * a + 1] * B[3 * a + 1] + B[3 * a + 2] * B[3 * a + 2]);
-        Mt[a] = muc * nb[a];
+    for (int k = 0; k < 4; ++k) {
+        const double d0 = vel4[0][k] - vbar[0], d1 = vel4[1][k] - vbar[1],
+                     d2 = vel4[2][k] - vbar[2];
+        mv[k] = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        const double muc = den * (qa * sspd + qb * mv[k]);
+        nb[k] = sqrt(B4[3 * k] * B4[3 * k] + B4[3 * k + 1] * B4[3 * k + 1] +
+                     B4[3 * k + 2] * B4[3 * k + 2]);
+        Mt[k] = muc * nb[k];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) S[d] += Mt[a] * vel[d][a];
-        W += Mt[a];
+        for (int d = 0; d < 3; ++d) S[d] += Mt[k] * vel4[d][k];
+        W += Mt[k];
     }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) S[d] = pair_sum(S[d]);
+    W = pair_sum(W);
     const bool riemann = W > fuzz;
     double vs[3];
 #pragma unroll
@@ -697,29 +767,31 @@ __global__ __launch_bounds__(64) void k_adj_force(DMesh m, DState s, AdjView A, 
     const double alpha = fmin(1.0, argv);
     const double phi = alpha * phi0;
     // ---- reverse ----
-    double Bb[24], velb[3][8], Mb[8];
+    double Bb4[12], velb4[3][4], Mb[4];
     double presb = 0.0, phib = 0.0, vsb[3] = {0.0, 0.0, 0.0};
     const double ge = A.ge[e];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
+    for (int k = 0; k < 4; ++k) {
         double g = 0.0;
-        // the node's record (h, kv) from k_adj_node: three 16-byte loads
-        const double2* rec = reinterpret_cast<const double2*>(A.hk + 6 * static_cast<size_t>(nd[a]));
+        const double2* rec = reinterpret_cast<const double2*>(A.hk + 6 * static_cast<size_t>(n4[k]));
         const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
         const double hh[3] = {r0.x, r0.y, r1.x}, kv[3] = {r1.y, r2.x, r2.y};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            // the corner's force adjoint g_e h + kv at its node
             const double fb = ge * hh[d] + kv[d];
-            presb += B[3 * a + d] * fb;
-            Bb[3 * a + d] = pres * fb;
-            g += fb * (vs[d] - vel[d][a]);
-            vsb[d] += phi * Mt[a] * fb;
-            velb[d][a] = -phi * Mt[a] * fb;
+            presb += B4[3 * k + d] * fb;
+            Bb4[3 * k + d] = pres * fb;
+            g += fb * (vs[d] - vel4[d][k]);
+            vsb[d] += phi * Mt[k] * fb;
+            velb4[d][k] = -phi * Mt[k] * fb;
         }
-        phib += Mt[a] * g;
-        Mb[a] = phi * g;
+        phib += Mt[k] * g;
+        Mb[k] = phi * g;
     }
+    presb = pair_sum(presb);
+    phib = pair_sum(phib);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vsb[d] = pair_sum(vsb[d]);
     if (riemann) {
         double Wb = 0.0, Sb[3];
         const double iW = 1.0 / W;
@@ -729,43 +801,47 @@ __global__ __launch_bounds__(64) void k_adj_force(DMesh m, DState s, AdjView A, 
             Wb -= vsb[d] * vs[d] * iW;
         }
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
+        for (int k = 0; k < 4; ++k) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                velb[d][a] += Mt[a] * Sb[d];
-                Mb[a] += Sb[d] * vel[d][a];
+                velb4[d][k] += Mt[k] * Sb[d];
+                Mb[k] += Sb[d] * vel4[d][k];
             }
-            Mb[a] += Wb;
+            Mb[k] += Wb;
         }
     }
     double denb = 0.0, sspdb = 0.0, vbarb[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const double muc = den * (qa * sspd + qb * mv[a]);
-        const double mucb = Mb[a] * nb[a];
-        const double nbb = Mb[a] * muc;
-        if (nb[a] > 0.0) {
-            const double q = nbb / nb[a];
+    for (int k = 0; k < 4; ++k) {
+        const double muc = den * (qa * sspd + qb * mv[k]);
+        const double mucb = Mb[k] * nb[k];
+        const double nbb = Mb[k] * muc;
+        if (nb[k] > 0.0) {
+            const double q = nbb / nb[k];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) Bb[3 * a + d] += q * B[3 * a + d];
+            for (int d = 0; d < 3; ++d) Bb4[3 * k + d] += q * B4[3 * k + d];
         }
-        denb += mucb * (qa * sspd + qb * mv[a]);
+        denb += mucb * (qa * sspd + qb * mv[k]);
         sspdb += mucb * den * qa;
         const double mvb = mucb * den * qb;
-        if (mv[a] > 0.0) {
-            const double q = mvb / mv[a];
+        if (mv[k] > 0.0) {
+            const double q = mvb / mv[k];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double db = q * (vel[d][a] - vbar[d]);
-                velb[d][a] += db;
+                const double db = q * (vel4[d][k] - vbar[d]);
+                velb4[d][k] += db;
                 vbarb[d] -= db;
             }
         }
     }
+    denb = pair_sum(denb);
+    sspdb = pair_sum(sspdb);
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+    for (int d = 0; d < 3; ++d) vbarb[d] = pair_sum(vbarb[d]);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) velb[d][a] += 0.125 * vbarb[d];
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) velb4[d][k] += 0.125 * vbarb[d];
     double divb = 0.0, volb = 0.0, extb = 0.0;
     const double alphab = phib * phi0;
     const double phi0b = phib * alpha;
@@ -780,21 +856,22 @@ __global__ __launch_bounds__(64) void k_adj_force(DMesh m, DState s, AdjView A, 
         extb = rb * 0.9 / bden;
         divb -= rb * r / bden;
     }
-    // the B-matrix adjoint goes to k_adj_derived, which adds its own and
-    // applies the stage's one VJP (the map Bb -> x-bar is linear)
-    const int nvalid = bi.end - bi.start;
-    wave_store_rows(A.bbf, bi.start, nvalid, Bb, rbuf);
-    {
-        double row[24];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int nvalid = bi.end - bi.start - 32 * wave;   // this wave's elements in the item
+    const int e0 = bi.start + 32 * wave;
+    double2* buf = hbuf + wave * 64 * 7;
+    if (nvalid > 0) {   // wave-uniform
+        wave_store_half_rows(A.bbf, e0, nvalid, Bb4, buf);
+        double row[12];
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            row[3 * a] = velb[0][a];
-            row[3 * a + 1] = velb[1][a];
-            row[3 * a + 2] = velb[2][a];
+        for (int k = 0; k < 4; ++k) {
+            row[3 * k] = velb4[0][k];
+            row[3 * k + 1] = velb4[1][k];
+            row[3 * k + 2] = velb4[2][k];
         }
-        wave_store_rows(A.cbv, bi.start, nvalid, row, rbuf);
+        wave_store_half_rows(A.cbv, e0, nvalid, row, buf);
     }
-    if (!live) return;
+    if (!live || h != 0) return;
     const size_t E = static_cast<size_t>(m.E);
     A.eadj[0 * E + e] = denb;
     A.eadj[1 * E + e] = presb;
@@ -809,7 +886,7 @@ __global__ __launch_bounds__(64) void k_adj_force(DMesh m, DState s, AdjView A, 
 // div).  setup != 0: the first stage of the run, whose density, pressure and
 // sound speed come from the initial conditions (den = rho0) and whose
 // coordinates are the fixed initial mesh.  Adds its velocity terms to the
-// corner rows k_adj_force wrote, adds its B-matrix adjoint to the force
+// corner rows k_adj_force_pair wrote, adds its B-matrix adjoint to the force
 // reverse's and writes the position part of the corner rows from the one
 // VJP.  Lanes past the work item compute on the item's first element and
 // store nothing.
@@ -1476,7 +1553,8 @@ void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, cons
                                static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
         });
     timed(c, "adj_force", [&] {
-        hipLaunchKernelGGL(k_adj_force, dim3(m.n_eblk), tb, 0, c->stream, m, s_in, A,
+        // two lanes per element of the 64-element work items (small_block_mesh)
+        hipLaunchKernelGGL(k_adj_force_pair, dim3(m.n_eblk), dim3(128), 0, c->stream, m, s_in, A,
                            static_cast<const InstCtl*>(c->d_ctl),
                            static_cast<const double*>(s_in.vol),
                            static_cast<const double*>(s_in.div));
