@@ -381,84 +381,13 @@ struct AdjView {
     int2 *alo, *ahi;
 };
 
-// ---- rows of 24 doubles per element, moved between lanes and memory ----
-// A wave's 64 consecutive elements own one contiguous stretch of a corner
-// array (24 doubles each).  Reading it one row per lane would make every
-// wave instruction touch 64 cache lines; instead the lanes load and store
-// the stretch as consecutive 16-byte pieces (coalesced) and pass rows
-// through a padded per-wave LDS region, 32 rows (13 x 16 B each) at a time.
-// e0: the wave's first element; nvalid: its elements inside the work item
-// (>= 1); buf: 32 x 13 double2 of this wave.
-constexpr int kRowBuf = 32 * 13;
-
+// Wave-local synchronisation around the per-wave LDS transposes of corner
+// rows (release/acquire at wavefront scope and a wave barrier).
 __device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ void wave_issue_rows(const double* src_, int e0, int nvalid,
-                                                double2 t[12])
-{
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-    const double2* src = reinterpret_cast<const double2*>(src_ + 24 * static_cast<size_t>(e0));
-#pragma unroll
-    for (int r = 0; r < 12; ++r) {
-        // pieces past the valid rows re-read piece 0: unconditional loads,
-        // whose values go unused
-        const int idx = 64 * r + lane;
-        t[r] = src[idx < 12 * nvalid ? idx : 0];
-    }
-}
-
-// the loaded pieces -> this lane's row
-__device__ __forceinline__ void wave_rows_to_lanes(const double2 t[12], double2* buf, double f[24])
-{
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        wave_sync();
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            const int idx = 64 * r + lane;
-            buf[(idx / 12) * 13 + idx % 12] = t[6 * h + r];
-        }
-        wave_sync();
-        if ((lane >> 5) == h) {
-            const double2* mine = buf + (lane & 31) * 13;
-#pragma unroll
-            for (int i = 0; i < 12; ++i) {
-                const double2 v = mine[i];
-                f[2 * i] = v.x;
-                f[2 * i + 1] = v.y;
-            }
-        }
-    }
-}
-
-// this lane's row -> memory (rows past nvalid are not stored)
-__device__ __forceinline__ void wave_store_rows(double* dst_, int e0, int nvalid, const double f[24],
-                                                double2* buf)
-{
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-    double2* dst = reinterpret_cast<double2*>(dst_ + 24 * static_cast<size_t>(e0));
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        wave_sync();
-        if ((lane >> 5) == h) {
-            double2* mine = buf + (lane & 31) * 13;
-#pragma unroll
-            for (int i = 0; i < 12; ++i) mine[i] = make_double2(f[2 * i], f[2 * i + 1]);
-        }
-        wave_sync();
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            const int idx = 64 * r + lane;
-            const int row = 32 * h + idx / 12;
-            if (row < nvalid) dst[12 * row + idx % 12] = buf[(idx / 12) * 13 + idx % 12];
-        }
-    }
 }
 
 // e1 = e0 - alpha dt / M * P,  P = sum_c F_c . (v1+v0)/2: the element part
@@ -890,110 +819,158 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
 // reverse's and writes the position part of the corner rows from the one
 // VJP.  Lanes past the work item compute on the item's first element and
 // store nothing.
-__global__ __launch_bounds__(64) void k_adj_derived(DMesh m, DState s, AdjView A,
-                                                    const double* vol_a, const double* div_a,
-                                                    const double* sie_in, double* sieb_in,
-                                                    int setup, int assign)
+// Each element's corners are split over a lane pair, as in k_adj_force_pair:
+// a lane holds its four corners' B-matrix adjoint, velocity terms and corner
+// rows; the B matrix is formed in both lanes and the one B-matrix VJP of the
+// stage runs in each lane on its own rows (the other half zero), the two
+// coordinate-adjoint partials then added across the pair for each lane's
+// four nodes (one element per lane needed 404 registers, one wave per SIMD).
+__global__ __launch_bounds__(128, 2) void k_adj_derived_pair(DMesh m, DState s, AdjView A,
+                                                             const double* vol_a,
+                                                             const double* div_a,
+                                                             const double* sie_in,
+                                                             double* sieb_in, int setup,
+                                                             int assign)
 {
-    __shared__ double2 rbuf[kRowBuf];
+    __shared__ double2 hbuf[2 * 64 * 7];
     const BlockItem bi = m.eblk[blockIdx.x];
-    const bool live = bi.start + static_cast<int>(threadIdx.x) < bi.end;
-    const int e = live ? bi.start + static_cast<int>(threadIdx.x) : bi.start;
+    const int h = static_cast<int>(threadIdx.x) & 1;
+    const int el = static_cast<int>(threadIdx.x) >> 1;
+    const bool live = bi.start + el < bi.end;
+    const int e = live ? bi.start + el : bi.start;
     const size_t E = static_cast<size_t>(m.E);
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int nvalid = bi.end - bi.start - 32 * wave;   // this wave's elements in the item
+    const int e0 = bi.start + 32 * wave;
+    double2* buf = hbuf + wave * 64 * 7;
     int32_t nd[8];
     ld_conn(m.conn, e, nd);
-    // divergence adjoint: own term plus the shock-detector neighbour term,
-    // the per-slot sums (k_adj_ext_node) of the slots this element occupies
-    double divb = A.eadj[4 * E + e];
+    int32_t n4[4];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const int2 il = A.alo[nd[a]], ih = A.ahi[nd[a]];
-        const double2* p = reinterpret_cast<const double2*>(A.nacc + 4 * static_cast<size_t>(nd[a]));
+    for (int k = 0; k < 4; ++k) n4[k] = pick(h, nd[k], nd[4 + k]);
+    // divergence adjoint: own term plus the shock-detector neighbour term, the
+    // per-slot sums (k_adj_ext_node) of the slots this element occupies
+    double divb = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int2 il = A.alo[n4[k]], ih = A.ahi[n4[k]];
+        const double2* p = reinterpret_cast<const double2*>(A.nacc + 4 * static_cast<size_t>(n4[k]));
         const double2 s01 = p[0], s23 = p[1];
         divb += (il.x == e ? s01.x : 0.0) + (il.y == e ? s01.y : 0.0) +
                 (ih.x == e ? s23.x : 0.0) + (ih.y == e ? s23.y : 0.0);
     }
+    divb = A.eadj[4 * E + e] + pair_sum(divb);
     const double denb = A.eadj[0 * E + e], presb = A.eadj[1 * E + e], sspdb = A.eadj[2 * E + e];
     double volb = A.eadj[3 * E + e];
-    double x[8], y[8], z[8], vel[3][8];
-    g3(s.x1, s.y1, s.z1, nd, x, y, z);
-    g3(s.u1, s.v1, s.w1, nd, vel[0], vel[1], vel[2]);
     const double vol = vol_a[e], div = div_a[e];
     const double gam = s.gamma[e], csmin = s.csmin[e];
     const double sie = sie_in[e];
     const double den = s.den[e];
-    // the material's EOS (K/eos.cpp:35-43 for the ideal gas): its partial
-    // derivatives in density, energy and gamma by three dual-number
-    // evaluations, so any model of fsgh::eos is transposed exactly
+    // the material's EOS: its partial derivatives in density, energy and gamma
+    // by three dual-number evaluations (both lanes; the first stores)
     const int32_t kind = s.mats[s.nmat > 1 ? s.mat[e] : 0].eos;
     Dual pr, cs;
     eos(kind, Dual(gam), csmin, Dual(den, 1.0), Dual(sie), pr, cs);
-    double dnb = denb + presb * pr.d + sspdb * cs.d;
+    const double dnb = denb + presb * pr.d + sspdb * cs.d;
     eos(kind, Dual(gam), csmin, Dual(den), Dual(sie, 1.0), pr, cs);
     const double eb = presb * pr.d + sspdb * cs.d;
     eos(kind, Dual(gam, 1.0), csmin, Dual(den), Dual(sie), pr, cs);
     const double gb = presb * pr.d + sspdb * cs.d;
-    if (live) {
+    if (live && h == 0) {
         sieb_in[e] = assign ? eb : sieb_in[e] + eb;
         A.gb[e] += gb;
     }
-    // divergence (K/momentum.cpp:265-286): div = (sum_a v_a . B_a) / vol
-    double B[24];
-    hex_bmatrix(x, y, z, B);
-    const double iv = 1.0 / vol;
-    // the force reverse's B-matrix adjoint
-    const int nvalid = bi.end - bi.start;
-    double Bb[24];
+    // this lane's four rows of the B matrix
+    double x[8], y[8], z[8], B4[12];
+    g3(s.x1, s.y1, s.z1, nd, x, y, z);
     {
-        const double2* rb = reinterpret_cast<const double2*>(A.bbf + 24 * static_cast<size_t>(e));
+        double B[24];
+        hex_bmatrix(x, y, z, B);
 #pragma unroll
-        for (int i = 0; i < 12; ++i) {
-            const double2 v = rb[i];
-            Bb[2 * i] = v.x;
-            Bb[2 * i + 1] = v.y;
+        for (int i = 0; i < 12; ++i) B4[i] = pick(h, B[i], B[12 + i]);
+    }
+    // this lane's half of the force reverse's B-matrix adjoint row and of the
+    // velocity part of its corner row (per-lane 16-byte pieces)
+    const int e_row = live ? e : bi.start;
+    double Bb4[12], rv4[12];
+    {
+        const double2* rb =
+            reinterpret_cast<const double2*>(A.bbf + 24 * static_cast<size_t>(e_row) + 12 * h);
+        const double2* rr =
+            reinterpret_cast<const double2*>(A.cbv + 24 * static_cast<size_t>(e_row) + 12 * h);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const double2 u = rb[i], w = rr[i];
+            Bb4[2 * i] = u.x;
+            Bb4[2 * i + 1] = u.y;
+            rv4[2 * i] = w.x;
+            rv4[2 * i + 1] = w.y;
         }
     }
-    double2* rv = reinterpret_cast<double2*>(A.cbv + 24 * static_cast<size_t>(e));
+    const double iv = 1.0 / vol;
+    // divergence (K/momentum.cpp:265-286): div = (sum_a v_a . B_a) / vol
 #pragma unroll
-    for (int i = 0; i < 12; ++i) {
-        // velocity part of the corner row, two values at a time
-        const double2 old = rv[i];
-        double o[2] = {old.x, old.y};
+    for (int k = 0; k < 4; ++k) {
+        const double vk[3] = {s.u1[n4[k]], s.v1[n4[k]], s.w1[n4[k]]};
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int a = (2 * i + h) / 3, d = (2 * i + h) % 3;
-            Bb[3 * a + d] += divb * iv * vel[d][a];
-            o[h] += divb * iv * B[3 * a + d];
+        for (int d = 0; d < 3; ++d) {
+            Bb4[3 * k + d] += divb * iv * vk[d];
+            rv4[3 * k + d] += divb * iv * B4[3 * k + d];
         }
-        if (live) rv[i] = make_double2(o[0], o[1]);
+    }
+    if (live) {
+        double2* rr = reinterpret_cast<double2*>(A.cbv + 24 * static_cast<size_t>(e) + 12 * h);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) rr[i] = make_double2(rv4[2 * i], rv4[2 * i + 1]);
     }
     volb -= divb * div * iv;
     double xb[8] = {0}, yb[8] = {0}, zb[8] = {0};
     if (setup) {
-        // den = rho0; vol and B belong to the fixed initial mesh (its
-        // coordinate adjoint is not a gradient of the run)
-        if (live) A.rhob[e] += dnb;
+        // den = rho0; vol and B belong to the fixed initial mesh
+        if (live && h == 0) A.rhob[e] += dnb;
     } else {
-        // den = M / vol
-        if (live) A.Mb[e] += dnb / vol;
+        // den = M / vol; vol = V(x) with dV/dx_a = B_a, on this lane's nodes
+        if (live && h == 0) A.Mb[e] += dnb / vol;
         volb -= dnb * den / vol;
-        // vol = V(x): dV/dx_a = B_a
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            xb[a] = volb * B[3 * a];
-            yb[a] = volb * B[3 * a + 1];
-            zb[a] = volb * B[3 * a + 2];
+        for (int k = 0; k < 4; ++k) {
+            const double bx = volb * B4[3 * k], by = volb * B4[3 * k + 1],
+                         bz = volb * B4[3 * k + 2];
+            xb[k] = pick(h, bx, 0.0);
+            yb[k] = pick(h, by, 0.0);
+            zb[k] = pick(h, bz, 0.0);
+            xb[4 + k] = pick(h, 0.0, bx);
+            yb[4 + k] = pick(h, 0.0, by);
+            zb[4 + k] = pick(h, 0.0, bz);
         }
     }
-    hex_bmatrix_vjp(x, y, z, Bb, xb, yb, zb);
-    double row[24];
+    {
+        // the one B-matrix VJP of the stage on this lane's rows (the other
+        // half zero)
+        double Bbf[24];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        row[3 * a] = xb[a];
-        row[3 * a + 1] = yb[a];
-        row[3 * a + 2] = zb[a];
+        for (int i = 0; i < 12; ++i) {
+            Bbf[i] = pick(h, Bb4[i], 0.0);
+            Bbf[12 + i] = pick(h, 0.0, Bb4[i]);
+        }
+        // the VJP re-forms the coordinate differences hex_bmatrix formed:
+        // opaque copies of the coordinates keep the compiler from carrying
+        // those values from the B matrix to here instead (registers)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) asm volatile("" : "+v"(x[i]), "+v"(y[i]), "+v"(z[i]));
+        hex_bmatrix_vjp(x, y, z, Bbf, xb, yb, zb);
     }
-    wave_store_rows(A.cbx, bi.start, nvalid, row, rbuf);
+    // this lane's nodes: its partial plus the partner's
+    double row[12];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double sx = pick(h, xb[4 + k], xb[k]), sy = pick(h, yb[4 + k], yb[k]),
+                     sz = pick(h, zb[4 + k], zb[k]);
+        row[3 * k] = pick(h, xb[k], xb[4 + k]) + __shfl_xor(sx, 1, 64);
+        row[3 * k + 1] = pick(h, yb[k], yb[4 + k]) + __shfl_xor(sy, 1, 64);
+        row[3 * k + 2] = pick(h, zb[k], zb[4 + k]) + __shfl_xor(sz, 1, 64);
+    }
+    if (nvalid > 0) wave_store_half_rows(A.cbx, e0, nvalid, row, buf);
 }
 
 // corner adjoints -> node adjoints, summed in corner-list order (no
@@ -1563,7 +1540,9 @@ void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, cons
         hipLaunchKernelGGL(k_adj_ext_node, dim3(m.n_nblk), tb, 0, c->stream, m, A);
     });
     timed(c, "adj_derived", [&] {
-        hipLaunchKernelGGL(k_adj_derived, dim3(m.n_eblk), tb, 0, c->stream, m, s_in, A,
+        // two lanes per element of the 64-element work items (small_block_mesh)
+        hipLaunchKernelGGL(k_adj_derived_pair, dim3(m.n_eblk), dim3(128), 0, c->stream, m, s_in,
+                           A,
                            static_cast<const double*>(s_in.vol),
                            static_cast<const double*>(s_in.div),
                            static_cast<const double*>(s_in.sie1), in.e,
