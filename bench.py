@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -338,6 +339,8 @@ def plain(args):
         raise SystemExit(f"bench: {st['cycles']} cycles ran, expected "
                          f"{args.warmup + 2 * args.steps}")
     ie, ke = g.energy(0)
+    if not math.isfinite(ie + ke):
+        raise SystemExit(f"bench: non-finite state after the timed cycles (IE {ie}, KE {ke})")
 
     E, N = g.E, g.N
     kernels, lists = kernel_table(g, prof, tuning)
@@ -394,7 +397,10 @@ def sedov_rz(args):
     from fierro_amd.sgh import SGHSolver
 
     n = args.n if args.n != 256 else 4096
-    spec = P.sedov_rz(n, cycles=10 ** 9)
+    # the reference's dt_min = 1e-8 floor exceeds the deposit's CFL step on
+    # meshes this fine (2.6e-9 at 4096^2) and would clamp the run unstable
+    dt_min = 1.0e-8 if n <= 2048 else 1.0e-12
+    spec = P.sedov_rz(n, cycles=10 ** 9, dt_min=dt_min)
     t0 = time.perf_counter()
     g = SGHSolver(spec, renumber=True)
     log(f"setup rz {n}^2: E={g.E} N={g.N} in {time.perf_counter() - t0:.1f} s")
@@ -411,6 +417,9 @@ def sedov_rz(args):
     g.profile(False)
     st = g.status(0)
     log(f"timed loop: {args.steps} cycles in {wall:.3f} s (graph loop {graph_s:.3f} s); {st}")
+    ie, ke = g.energy(0)
+    if not math.isfinite(ie + ke):
+        raise SystemExit(f"bench: non-finite state after the timed cycles (IE {ie}, KE {ke})")
     E, N = g.E, g.N
     alg = rz_alg_bytes(g)
     kernels = {}
@@ -428,7 +437,7 @@ def sedov_rz(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (generated RZ Sedov mesh and initial state)",
         "config": {"workload": f"sedov_rz{n}", "elements": E, "nodes": N, "rk_stages": 2,
-                   "renumber": "morton", "cfl": 0.4, "q2": 1.3333},
+                   "renumber": "morton", "cfl": 0.4, "q2": 1.3333, "dt_min": dt_min},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["GBs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kernels[dom]["GBs"] / HBM_PEAK_GBS,
