@@ -605,6 +605,8 @@ def ensemble(args, steps=500, warmup=5):
     g.profile(False)
     cyc = [g.status(i)["cycles"] for i in range(len(specs))]
     assert min(cyc) == max(cyc) == args.warmup + steps, (min(cyc), max(cyc))
+    if not all(math.isfinite(sum(g.energy(i))) for i in range(len(specs))):
+        raise SystemExit("bench: non-finite instance state after the timed cycles")
     E, N = g.E, g.N
     kernels, _ = kernel_table(g, prof, tuning)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["calls"])
