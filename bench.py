@@ -149,8 +149,9 @@ TRAFFIC_KERNELS = {
                  "rz_node": "k_rz_node", "rz_energy_state_div": "k_rz_energy_state_div"},
     # the reverse pass of the calibration gradient (every launch covers the batch)
     "calibration": {"adj_energy": "k_adj_energy", "adj_node": "k_adj_node<",
-                    "arg_extremes": "k_arg_extremes", "adj_force": "k_adj_force",
-                    "adj_ext_node": "k_adj_ext_node", "adj_derived": "k_adj_derived",
+                    "arg_extremes": "k_arg_extremes", "adj_ext": "k_adj_force_pair<true",
+                    "adj_ext_node": "k_adj_ext_node",
+                    "adj_force_derived": "k_adj_force_pair<false",
                     "adj_gather": "k_adj_gather<"},
 }
 

@@ -48,7 +48,12 @@ KERNELS = {"div_extremes": div_extremes, "force": force, "vel_bc_pos": vel_bc_po
 
 # ---- reverse pass of one RK stage (fsgh_adjoint.hip), accumulating form ----
 # (the first stage reversed in a cycle writes some accumulators instead of
-# adding to them; counted here as read + write throughout)
+# adding to them; counted here as read + write throughout).  Full work: every
+# element and node as if its adjoint were nonzero.  The kernels skip what the
+# reverse of a zero adjoint leaves at zero (k_adj_energy per element, the
+# force/derived passes per wave, k_adj_node's corner forces per block,
+# k_adj_gather's absent rows), so where much of the mesh is not yet reached
+# by the adjoint they read and write less than these figures.
 def adj_energy(E, N):
     # e-bar 8E + e0-bar 16E (rw) + M 8E + g 8E (w) + conn 32E + F 24C + M-bar 16E (rw)
     # + v0, v1 48N
@@ -69,11 +74,12 @@ def arg_extremes(E, N):
     return 32 * E + 4 * (N + 1) + 8 * E + 48 * N
 
 
-def adj_force(E, N):
+def adj_ext(E, N):
     # conn 32E + x, v 48N + vol, div, den, sspd, pres 40E + extreme pairs and their
-    # elements 48N + g 8E + (h, kv) records 48N + B-bar rows 24C (w) + corner
-    # velocity adjoint rows 24C (w) + element adjoints 48E (w) + chosen extreme 4E (w)
-    return 32 * E + 48 * N + 40 * E + 48 * N + 8 * E + 48 * N + 192 * E + 192 * E + 48 * E + 4 * E
+    # elements 48N + g 8E + (h, kv) records 48N + ext-bar 8E (w) + chosen extreme 4E (w)
+    # (every wave's full recompute; a wave with no element whose limiter
+    # depends on the extreme reads only conn, vol, div and the extremes)
+    return 32 * E + 48 * N + 40 * E + 48 * N + 8 * E + 48 * N + 8 * E + 4 * E
 
 
 def adj_ext_node(E, N):
@@ -81,24 +87,24 @@ def adj_ext_node(E, N):
     return 32 * E + 4 * (N + 1) + 4 * E + 8 * E + 32 * N
 
 
-def adj_derived(E, N):
-    # conn 32E + den, pres, sspd, vol, div adjoints 40E + extreme elements 16N
-    # + per-slot sums 32N + x, v 48N + vol, div 16E + gamma, csmin 16E + e 8E + den 8E
-    # + e-bar 16E (rw) + gamma-bar 16E (rw) + B-bar rows 24C + corner velocity rows
-    # 24C (rw) + M-bar (or rho0-bar) 16E (rw) + corner position rows 24C (w)
-    return (32 * E + 40 * E + 16 * N + 32 * N + 48 * N + 16 * E + 16 * E + 8 * E + 8 * E
-            + 16 * E + 16 * E + 192 * E + 384 * E + 16 * E + 192 * E)
+def adj_force_derived(E, N):
+    # conn 32E + x, v 48N + vol, div, den, sspd, pres 40E + extreme pairs and their
+    # elements 48N + g 8E + (h, kv) records 48N + per-slot sums 32N + gamma, csmin
+    # 16E + e 8E + e-bar 16E (rw) + gamma-bar 16E (rw) + M-bar (or rho0-bar) 16E (rw)
+    # + corner velocity rows 24C (w) + corner position rows 24C (w) + row flag 1E (w)
+    return (32 * E + 48 * N + 40 * E + 48 * N + 8 * E + 48 * N + 32 * N + 16 * E + 8 * E
+            + 16 * E + 16 * E + 16 * E + 192 * E + 192 * E + E)
 
 
 def adj_gather(E, N):
-    # corner CSR starts 4(N+1) + sorted corner list 4C + positions 2C + corner
-    # position and velocity rows 48C + x-bar, v-bar 96N (rw)
-    return 4 * (N + 1) + 32 * E + 16 * E + 384 * E + 96 * N
+    # corner CSR starts 4(N+1) + sorted corner list 4C + positions 2C + row flags
+    # 1C + corner position and velocity rows 48C + x-bar, v-bar 96N (rw)
+    return 4 * (N + 1) + 32 * E + 16 * E + 8 * E + 384 * E + 96 * N
 
 
 ADJ_KERNELS = {"adj_energy": adj_energy, "adj_node": adj_node, "arg_extremes": arg_extremes,
-               "adj_force": adj_force, "adj_ext_node": adj_ext_node,
-               "adj_derived": adj_derived, "adj_gather": adj_gather}
+               "adj_ext": adj_ext, "adj_ext_node": adj_ext_node,
+               "adj_force_derived": adj_force_derived, "adj_gather": adj_gather}
 
 
 def adj_stage(E, N):

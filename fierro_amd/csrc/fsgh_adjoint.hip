@@ -16,13 +16,14 @@
 //                  the node (CSR order)                  -> x0-bar, v0-bar, F-bar, m-bar
 //   k_arg_extremes per node, the two smallest / largest divergences with the
 //                  elements holding them
-//   k_adj_force_pair corner forces (MARS + shock detector) -> per-corner (x-bar, v-bar),
-//                  element (den, pres, sspd, vol, div, ext) adjoints, ext's element
+//   k_adj_force_pair<true>  the shock detector's neighbour-extreme adjoint ext-bar of
+//                  each element and the node slot its extreme came from
 //   k_adj_ext_node per node and extreme slot, the ext-adjoints of the elements
 //                  that chose it (each element then adds the sums of the slots
-//                  it occupies to its divergence adjoint, in k_adj_derived)
-//   k_adj_derived  vol / den / EOS / div of the stage input -> e-bar, M-bar,
-//                  gamma-bar, corner x-bar, v-bar
+//                  it occupies to its divergence adjoint, in k_adj_force_pair<false>)
+//   k_adj_force_pair<false> corner forces (MARS + shock detector), then vol / den /
+//                  EOS / div of the stage input -> e-bar, M-bar, gamma-bar,
+//                  corner x-bar, v-bar
 //   k_adj_gather   corner adjoints -> node adjoints, in corner order (no atomics)
 // Branches (compression test, min/max, clamps, the sound-speed floor) follow
 // the forward values; ties of the neighbour extreme go to the first element
@@ -371,10 +372,8 @@ struct AdjView {
     // corners, element order: position (cbx) and velocity (cbv) adjoints,
     // rows of 24 doubles per element (8 corners x 3)
     double *cbx, *cbv;
-    // element rows of 24: the force reverse's adjoint of the B matrix, which
-    // k_adj_derived adds to its own before the one B-matrix VJP of the stage
-    double* bbf;
-    double* eadj;               // element: den, pres, sspd, vol, div, ext (6 x E)
+    uint8_t* rowset;            // element: 1 if its rows were written this stage, 0 if zero
+    double* extb;               // element: adjoint of its neighbour extreme
     int32_t* argext;            // element: 4 * node + slot of its chosen extreme, or -1
     double* nacc;               // node: per-slot sums of ext-bar (4 x N)
     double2 *lo, *hi;
@@ -394,7 +393,7 @@ __device__ __forceinline__ void wave_sync()
 // (e0-bar, M-bar and the scalar g = -alpha dt / M * e1-bar that the node
 // kernel turns into the corner terms g (v1+v0)/2 and g F_c).
 // assign != 0: the accumulators (e0-bar here, x0-bar / v0-bar in the node
-// kernel, the input adjoints in k_adj_derived / k_adj_gather) are written
+// kernel, the input adjoints in k_adj_force_pair<false> / k_adj_gather) are written
 // instead of added to (the first stage reversed in a cycle)
 __global__ __launch_bounds__(kB) void k_adj_energy(DMesh m, DState s, const double* F,
                                                    AdjView A, const double* dts, double alpha,
@@ -429,7 +428,9 @@ __global__ __launch_bounds__(kB) void k_adj_energy(DMesh m, DState s, const doub
 // corner's F_c-bar = g_e h + kv.  The block's corner forces and their
 // elements' g are staged in LDS in sorted-corner order (coalesced loads, as
 // the forward velocity kernel stages its forces); a block over the capacity
-// gathers from global memory.  Sums in list order either way.
+// gathers from global memory.  Sums in list order either way.  (Every
+// thread reaches the block's barriers: threads past the block's nodes return
+// after them.)
 template <int BS>
 __global__ __launch_bounds__(BS) void k_adj_node(DMesh m, DState s, const double* F, AdjView A,
                                                  const double* dts, double alpha, int assign)
@@ -441,40 +442,64 @@ __global__ __launch_bounds__(BS) void k_adj_node(DMesh m, DState s, const double
     const int qb = m.cin_start[bi.start];
     const int nc = m.cin_start[bi.end] - qb;
     const bool staged = m.cs_cor != nullptr && nc <= CAPC;
+    const int n = bi.start + static_cast<int>(threadIdx.x);
+    const bool live = n < bi.end;
+    // the stage output's adjoints; seeded: the node's velocity adjoint
+    // without the energy term is nonzero
+    const uint8_t mk = live ? s.bcmask[n] : 0;
+    double xo[3] = {0.0, 0.0, 0.0}, vo[3] = {0.0, 0.0, 0.0};
+    bool seeded = false;
+    if (live) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            xo[d] = A.xo[d][n];
+            vo[d] = A.vo[d][n];
+            seeded = seeded || (!(mk & (1u << d)) && vo[d] + 0.5 * k * xo[d] != 0.0);
+        }
+    }
+    // The corner forces enter through g F (energy) and v-bar . sum F (mass):
+    // a staged block whose corners' g and nodes' velocity adjoints are all
+    // zero does not read them (block-uniform).
+    bool needF = true;
     if (staged) {
         constexpr int R = CAPC / BS;
         int cid[R];
+        double ge[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = r * BS + static_cast<int>(threadIdx.x);
             cid[r] = j < nc ? m.cs_cor[qb + j] : -1;
+            ge[r] = cid[r] >= 0 ? A.ge[cid[r] >> 3] : 0.0;
+            seeded = seeded || ge[r] != 0.0;
         }
-        double g[R][4];
+        needF = __syncthreads_or(seeded);
+        if (needF) {
+            double g[R][3];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int c = cid[r] < 0 ? 0 : cid[r];
-            const double* f = F + 3 * static_cast<size_t>(c);
-            g[r][0] = f[0];
-            g[r][1] = f[1];
-            g[r][2] = f[2];
-            g[r][3] = A.ge[c >> 3];
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int j = r * BS + static_cast<int>(threadIdx.x);
-            if (cid[r] >= 0) {
-#pragma unroll
-                for (int d = 0; d < 4; ++d) sF[4 * j + d] = g[r][d];
+            for (int r = 0; r < R; ++r) {
+                const int c = cid[r] < 0 ? 0 : cid[r];
+                const double* f = F + 3 * static_cast<size_t>(c);
+                g[r][0] = f[0];
+                g[r][1] = f[1];
+                g[r][2] = f[2];
             }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int j = r * BS + static_cast<int>(threadIdx.x);
+                if (cid[r] >= 0) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) sF[4 * j + d] = g[r][d];
+                    sF[4 * j + 3] = ge[r];
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
-    const int n = bi.start + static_cast<int>(threadIdx.x);
-    if (n >= bi.end) return;
+    if (!live) return;
     const int q0 = m.cin_start[n];
     const int cnt = m.cin_start[n + 1] - q0;
     double sh[3] = {0.0, 0.0, 0.0}, sf[3] = {0.0, 0.0, 0.0};
-    for (int q = q0; q < q0 + cnt; ++q) {
+    for (int q = q0; needF && q < q0 + cnt; ++q) {
         double f[3], g;
         if (staged) {
             const double* p = sF + 4 * m.cs_pos[q];
@@ -497,16 +522,15 @@ __global__ __launch_bounds__(BS) void k_adj_node(DMesh m, DState s, const double
         }
     }
     const double mass = s.nmass[n];
-    const uint8_t mk = s.bcmask[n];
     const double h[3] = {(s.u1[n] + s.u0[n]) * 0.5, (s.v1[n] + s.v0[n]) * 0.5,
                          (s.w1[n] + s.w0[n]) * 0.5};
     double vm[3];
     double dot = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double xb = A.xo[d][n];
+        const double xb = xo[d];
         A.x0b[d][n] = assign ? xb : A.x0b[d][n] + xb;
-        const double vt = A.vo[d][n] + 0.5 * k * xb + 0.5 * sh[d];
+        const double vt = vo[d] + 0.5 * k * xb + 0.5 * sh[d];
         vm[d] = (mk & (1u << d)) ? 0.0 : vt;
         const double v0b = 0.5 * k * xb + 0.5 * sh[d] + vm[d];
         A.v0b[d][n] = assign ? v0b : A.v0b[d][n] + v0b;
@@ -529,14 +553,13 @@ __global__ __launch_bounds__(BS) void k_adj_node(DMesh m, DState s, const double
 // that slot gets ext-bar(e2) added to its divergence adjoint.  Node pass
 // (here): per node and slot, the sum of ext-bar over the elements (in CSR
 // order) whose choice was that node and slot.  Element pass (start of
-// k_adj_derived): each element adds the sums of the slots it occupies at its
+// k_adj_force_pair<false>): each element adds the sums of the slots it occupies at its
 // 8 nodes.  Every e2 contributes once.
 __global__ __launch_bounds__(kB) void k_adj_ext_node(DMesh m, AdjView A)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
     const int n = bi.start + static_cast<int>(threadIdx.x);
     if (n >= bi.end) return;
-    const size_t E = static_cast<size_t>(m.E);
     const int q0 = m.cin_start[n];
     int c8[8];
     const int cnt = corners8(m, n, q0, c8);
@@ -546,7 +569,7 @@ __global__ __launch_bounds__(kB) void k_adj_ext_node(DMesh m, AdjView A)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         code[j] = A.argext[c8[j] >> 3];
-        xb[j] = A.eadj[5 * E + (c8[j] >> 3)];
+        xb[j] = A.extb[c8[j] >> 3];
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -557,7 +580,7 @@ __global__ __launch_bounds__(kB) void k_adj_ext_node(DMesh m, AdjView A)
     for (int q = q0 + 8; q < q0 + cnt; ++q) {
         const int e2 = m.cin[q] >> 3;
         const int cd = A.argext[e2];
-        if (cd >= 0 && (cd >> 2) == n) acc[cd & 3] += A.eadj[5 * E + e2];
+        if (cd >= 0 && (cd >> 2) == n) acc[cd & 3] += A.extb[e2];
     }
     double2* out = reinterpret_cast<double2*>(A.nacc + 4 * static_cast<size_t>(n));
     out[0] = make_double2(acc[0], acc[1]);
@@ -608,6 +631,25 @@ __device__ __forceinline__ void wave_store_half_rows(double* dst_, int e0, int n
     }
 }
 
+// Whether any of this lane's four corners has a nonzero force adjoint
+// F-bar = g_e h + kv (formed as the reverse forms it).  The node indices go
+// through an opaque copy, so that the reverse reads the records again (cache
+// hits) rather than keeping these loads live across the forward recompute.
+__device__ __forceinline__ bool corners_seeded(const double* hk, double ge, const int32_t n4[4])
+{
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int n = n4[k];
+        asm volatile("" : "+v"(n));
+        const double2* rec = reinterpret_cast<const double2*>(hk + 6 * static_cast<size_t>(n));
+        const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+        any = any || (ge * r0.x + r1.y != 0.0) || (ge * r0.y + r2.x != 0.0) ||
+              (ge * r1.x + r2.y != 0.0);
+    }
+    return any;
+}
+
 // Reverse of force_elem (K/force_sgh.cpp:35-359) at the stage input state,
 // with each element's corners split over a lane pair (two waves of 32
 // elements per 64-element work item): a lane holds the per-corner values of
@@ -617,9 +659,28 @@ __device__ __forceinline__ void wave_store_half_rows(double* dst_, int e0, int n
 // kernel's order); the other sums over corners are formed per half and the
 // halves added.  Lanes past the work item compute on the item's first
 // element and store nothing (every lane takes part in the row transposes).
+//
+// Two passes over the elements use it:
+//   EXT = true  (the "adj_ext" pass): only the shock detector's neighbour-extreme
+//     adjoint ext-bar and the chosen extreme, which k_adj_ext_node sums per
+//     node slot for the element that holds the extreme.  Zero where the
+//     limiter does not depend on the extreme (r_face outside (0, 1]); a wave
+//     with no such element skips the recompute.
+//   EXT = false (the "adj_force_derived" pass): the whole force reverse, then the
+//     reverse of the stage input's derived state (volume, density, EOS,
+//     divergence) with the neighbour sums added to the divergence adjoint,
+//     the stage's one B-matrix VJP, and the element's corner rows (position
+//     and velocity adjoints) written once.  setup != 0: the first stage of
+//     the run, whose density, pressure and sound speed come from the initial
+//     conditions (den = rho0) and whose coordinates are the fixed initial
+//     mesh.  assign != 0: the energy-adjoint accumulator is written, not
+//     added to.
+template <bool EXT>
 __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, AdjView A,
                                                            const InstCtl* ctl, const double* vol_a,
-                                                           const double* div_a)
+                                                           const double* div_a,
+                                                           const double* sie_in, double* sieb_in,
+                                                           int setup, int assign)
 {
     __shared__ double2 hbuf[2 * 64 * 7];
     const BlockItem bi = m.eblk[blockIdx.x];
@@ -630,17 +691,74 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
     const double fuzz = ctl[bi.inst].fuzz, small = ctl[bi.inst].small;
     int32_t nd[8];
     ld_conn(m.conn, e, nd);
-    const double vol = vol_a[e], div = div_a[e], den = s.den[e], sspd = s.sspd[e],
-                 pres = s.pres[e];
+    int32_t n4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) n4[k] = pick(h, nd[k], nd[4 + k]);
+    const double div = div_a[e];
+    const double ge = A.ge[e];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int nvalid = bi.end - bi.start - 32 * wave;   // this wave's elements in the item
+    const int e0 = bi.start + 32 * wave;
+    double2* buf = hbuf + wave * 64 * 7;
+    // The reverse of a zero adjoint is zero.  ext-bar = -phi-bar alpha 0.9 /
+    // (div + small) is zero where every corner's force adjoint is (then
+    // phi-bar is) and where alpha is (div = 0); the element's rows and element
+    // adjoints are zero where, in addition, its neighbour term is.  A wave
+    // with no such element stops here (wave-uniform branches).
+    bool seeded = pair_sum(corners_seeded(A.hk, ge, n4) ? 1.0 : 0.0) != 0.0;
+    double divnb_all = 0.0;
+    if (EXT) {
+        seeded = seeded && div != 0.0;
+        if (!__any(live && seeded)) {
+            if (live && h == 0) {
+                A.extb[e] = 0.0;
+                A.argext[e] = -1;   // no slot to add to
+            }
+            return;
+        }
+    } else {
+        // the neighbour term of the divergence adjoint: the per-slot sums
+        // (k_adj_ext_node) of the slots this element occupies at this lane's
+        // four nodes
+        double divnb = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int2 il = A.alo[n4[k]], ih = A.ahi[n4[k]];
+            const double2* p =
+                reinterpret_cast<const double2*>(A.nacc + 4 * static_cast<size_t>(n4[k]));
+            const double2 s01 = p[0], s23 = p[1];
+            divnb += (il.x == e ? s01.x : 0.0) + (il.y == e ? s01.y : 0.0) +
+                     (ih.x == e ? s23.x : 0.0) + (ih.y == e ? s23.y : 0.0);
+        }
+        divnb_all = pair_sum(divnb);
+        if (!__any(live && (seeded || divnb_all != 0.0))) {
+            // rows absent (k_adj_gather reads them as zero)
+            if (live && h == 0) {
+                if (assign) sieb_in[e] = 0.0;
+                A.rowset[e] = 0;
+            }
+            return;
+        }
+    }
+    const double vol = vol_a[e], den = s.den[e], sspd = s.sspd[e], pres = s.pres[e];
     const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
     const bool low = div + small >= 0.0;
     int arg;
     int code;
     const double ext = neighbour_extreme(nd, e, low, A.lo, A.hi, A.alo, A.ahi, arg, nullptr,
                                          nullptr, nullptr, &code);
-    int32_t n4[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) n4[k] = pick(h, nd[k], nd[4 + k]);
+    const double bden = div + small;
+    const double r = 0.9 * (ext + small) / bden;
+    const double rmin = fmin(r, 1.0);
+    // the limiter depends on the neighbour extreme only for r_face in (0, 1]
+    const bool rdep = rmin > 0.0 && r <= 1.0 && arg >= 0;
+    if (EXT && !__any(live && seeded && rdep)) {
+        if (live && h == 0) {
+            A.extb[e] = 0.0;
+            A.argext[e] = -1;
+        }
+        return;
+    }
     // ---- forward recompute ----
     double B4[12];
     {
@@ -686,9 +804,6 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
     double vs[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) vs[d] = riemann ? S[d] / W : 0.0;
-    const double bden = div + small;
-    const double r = 0.9 * (ext + small) / bden;
-    const double rmin = fmin(r, 1.0);
     const double phi0 = 1.0 - fmax(0.0, rmin);
     const double cl = pow(vol, 1.0 / 3.0);
     const double cd = sspd + fuzz;
@@ -698,7 +813,6 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
     // ---- reverse ----
     double Bb4[12], velb4[3][4], Mb[4];
     double presb = 0.0, phib = 0.0, vsb[3] = {0.0, 0.0, 0.0};
-    const double ge = A.ge[e];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         double g = 0.0;
@@ -719,6 +833,15 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
     }
     presb = pair_sum(presb);
     phib = pair_sum(phib);
+    if (EXT) {
+        double extb = 0.0;
+        if (rmin > 0.0 && r <= 1.0) extb = -(phib * alpha) * 0.9 / bden;
+        if (live && h == 0) {
+            A.extb[e] = (arg >= 0) ? extb : 0.0;
+            A.argext[e] = code;
+        }
+        return;
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) vsb[d] = pair_sum(vsb[d]);
     if (riemann) {
@@ -771,7 +894,7 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
     for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int d = 0; d < 3; ++d) velb4[d][k] += 0.125 * vbarb[d];
-    double divb = 0.0, volb = 0.0, extb = 0.0;
+    double divb = 0.0, volb = 0.0;
     const double alphab = phib * phi0;
     const double phi0b = phib * alpha;
     if (argv < 1.0) {
@@ -781,94 +904,33 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
         if (vol > 0.0) volb += clb * cl / (3.0 * vol);
     }
     if (rmin > 0.0 && r <= 1.0) {
+        // (the neighbour extreme's term, rb 0.9 / bden, is the ext-bar of
+        // the EXT pass)
         const double rb = -phi0b;
-        extb = rb * 0.9 / bden;
         divb -= rb * r / bden;
     }
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int nvalid = bi.end - bi.start - 32 * wave;   // this wave's elements in the item
-    const int e0 = bi.start + 32 * wave;
-    double2* buf = hbuf + wave * 64 * 7;
-    if (nvalid > 0) {   // wave-uniform
-        wave_store_half_rows(A.bbf, e0, nvalid, Bb4, buf);
-        double row[12];
+    // ---- the stage input's derived state (K/geometry.cpp:298-306 volume,
+    // K/properties.cpp:39,99 density and EOS, K/momentum.cpp:265-286 div) ----
+    divb = divb + divnb_all;
+    // divergence: div = (sum_a v_a . B_a) / vol
+    const double iv = 1.0 / vol;
+    {
+        double rv[12];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            row[3 * k] = velb4[0][k];
-            row[3 * k + 1] = velb4[1][k];
-            row[3 * k + 2] = velb4[2][k];
-        }
-        wave_store_half_rows(A.cbv, e0, nvalid, row, buf);
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                Bb4[3 * k + d] += divb * iv * vel4[d][k];
+                rv[3 * k + d] = velb4[d][k] + divb * iv * B4[3 * k + d];
+            }
+        if (nvalid > 0) wave_store_half_rows(A.cbv, e0, nvalid, rv, buf);   // wave-uniform
     }
-    if (!live || h != 0) return;
-    const size_t E = static_cast<size_t>(m.E);
-    A.eadj[0 * E + e] = denb;
-    A.eadj[1 * E + e] = presb;
-    A.eadj[2 * E + e] = sspdb;
-    A.eadj[3 * E + e] = volb;
-    A.eadj[4 * E + e] = divb;
-    A.eadj[5 * E + e] = (arg >= 0) ? extb : 0.0;
-    A.argext[e] = code;
-}
-
-// Neighbour divergence adjoint, then the stage-input state (vol, den, EOS,
-// div).  setup != 0: the first stage of the run, whose density, pressure and
-// sound speed come from the initial conditions (den = rho0) and whose
-// coordinates are the fixed initial mesh.  Adds its velocity terms to the
-// corner rows k_adj_force_pair wrote, adds its B-matrix adjoint to the force
-// reverse's and writes the position part of the corner rows from the one
-// VJP.  Lanes past the work item compute on the item's first element and
-// store nothing.
-// Each element's corners are split over a lane pair, as in k_adj_force_pair:
-// a lane holds its four corners' B-matrix adjoint, velocity terms and corner
-// rows; the B matrix is formed in both lanes and the one B-matrix VJP of the
-// stage runs in each lane on its own rows (the other half zero), the two
-// coordinate-adjoint partials then added across the pair for each lane's
-// four nodes (one element per lane needed 404 registers, one wave per SIMD).
-__global__ __launch_bounds__(128, 2) void k_adj_derived_pair(DMesh m, DState s, AdjView A,
-                                                             const double* vol_a,
-                                                             const double* div_a,
-                                                             const double* sie_in,
-                                                             double* sieb_in, int setup,
-                                                             int assign)
-{
-    __shared__ double2 hbuf[2 * 64 * 7];
-    const BlockItem bi = m.eblk[blockIdx.x];
-    const int h = static_cast<int>(threadIdx.x) & 1;
-    const int el = static_cast<int>(threadIdx.x) >> 1;
-    const bool live = bi.start + el < bi.end;
-    const int e = live ? bi.start + el : bi.start;
-    const size_t E = static_cast<size_t>(m.E);
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int nvalid = bi.end - bi.start - 32 * wave;   // this wave's elements in the item
-    const int e0 = bi.start + 32 * wave;
-    double2* buf = hbuf + wave * 64 * 7;
-    int32_t nd[8];
-    ld_conn(m.conn, e, nd);
-    int32_t n4[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) n4[k] = pick(h, nd[k], nd[4 + k]);
-    // divergence adjoint: own term plus the shock-detector neighbour term, the
-    // per-slot sums (k_adj_ext_node) of the slots this element occupies
-    double divb = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int2 il = A.alo[n4[k]], ih = A.ahi[n4[k]];
-        const double2* p = reinterpret_cast<const double2*>(A.nacc + 4 * static_cast<size_t>(n4[k]));
-        const double2 s01 = p[0], s23 = p[1];
-        divb += (il.x == e ? s01.x : 0.0) + (il.y == e ? s01.y : 0.0) +
-                (ih.x == e ? s23.x : 0.0) + (ih.y == e ? s23.y : 0.0);
-    }
-    divb = A.eadj[4 * E + e] + pair_sum(divb);
-    const double denb = A.eadj[0 * E + e], presb = A.eadj[1 * E + e], sspdb = A.eadj[2 * E + e];
-    double volb = A.eadj[3 * E + e];
-    const double vol = vol_a[e], div = div_a[e];
+    volb -= divb * div * iv;
     const double gam = s.gamma[e], csmin = s.csmin[e];
     const double sie = sie_in[e];
-    const double den = s.den[e];
     // the material's EOS: its partial derivatives in density, energy and gamma
     // by three dual-number evaluations (both lanes; the first stores)
-    const int32_t kind = s.mats[s.nmat > 1 ? s.mat[e] : 0].eos;
+    const int32_t kind = mt.eos;
     Dual pr, cs;
     eos(kind, Dual(gam), csmin, Dual(den, 1.0), Dual(sie), pr, cs);
     const double dnb = denb + presb * pr.d + sspdb * cs.d;
@@ -879,54 +941,12 @@ __global__ __launch_bounds__(128, 2) void k_adj_derived_pair(DMesh m, DState s, 
     if (live && h == 0) {
         sieb_in[e] = assign ? eb : sieb_in[e] + eb;
         A.gb[e] += gb;
+        A.rowset[e] = 1;
     }
-    // this lane's four rows of the B matrix
-    double x[8], y[8], z[8], B4[12];
-    g3(s.x1, s.y1, s.z1, nd, x, y, z);
-    {
-        double B[24];
-        hex_bmatrix(x, y, z, B);
-#pragma unroll
-        for (int i = 0; i < 12; ++i) B4[i] = pick(h, B[i], B[12 + i]);
-    }
-    // this lane's half of the force reverse's B-matrix adjoint row and of the
-    // velocity part of its corner row (per-lane 16-byte pieces)
-    const int e_row = live ? e : bi.start;
-    double Bb4[12], rv4[12];
-    {
-        const double2* rb =
-            reinterpret_cast<const double2*>(A.bbf + 24 * static_cast<size_t>(e_row) + 12 * h);
-        const double2* rr =
-            reinterpret_cast<const double2*>(A.cbv + 24 * static_cast<size_t>(e_row) + 12 * h);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            const double2 u = rb[i], w = rr[i];
-            Bb4[2 * i] = u.x;
-            Bb4[2 * i + 1] = u.y;
-            rv4[2 * i] = w.x;
-            rv4[2 * i + 1] = w.y;
-        }
-    }
-    const double iv = 1.0 / vol;
-    // divergence (K/momentum.cpp:265-286): div = (sum_a v_a . B_a) / vol
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double vk[3] = {s.u1[n4[k]], s.v1[n4[k]], s.w1[n4[k]]};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            Bb4[3 * k + d] += divb * iv * vk[d];
-            rv4[3 * k + d] += divb * iv * B4[3 * k + d];
-        }
-    }
-    if (live) {
-        double2* rr = reinterpret_cast<double2*>(A.cbv + 24 * static_cast<size_t>(e) + 12 * h);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) rr[i] = make_double2(rv4[2 * i], rv4[2 * i + 1]);
-    }
-    volb -= divb * div * iv;
     double xb[8] = {0}, yb[8] = {0}, zb[8] = {0};
     if (setup) {
-        // den = rho0; vol and B belong to the fixed initial mesh
+        // den = rho0; vol and B belong to the fixed initial mesh (its
+        // coordinate adjoint is not a gradient of the run)
         if (live && h == 0) A.rhob[e] += dnb;
     } else {
         // den = M / vol; vol = V(x) with dV/dx_a = B_a, on this lane's nodes
@@ -946,16 +966,21 @@ __global__ __launch_bounds__(128, 2) void k_adj_derived_pair(DMesh m, DState s, 
     }
     {
         // the one B-matrix VJP of the stage on this lane's rows (the other
-        // half zero)
+        // half zero), on coordinates read again (cache hits) behind an
+        // opaque copy, so that none of the B matrix's coordinate differences
+        // stay live across the kernel (registers)
         double Bbf[24];
 #pragma unroll
         for (int i = 0; i < 12; ++i) {
             Bbf[i] = pick(h, Bb4[i], 0.0);
             Bbf[12 + i] = pick(h, 0.0, Bb4[i]);
         }
-        // the VJP re-forms the coordinate differences hex_bmatrix formed:
-        // opaque copies of the coordinates keep the compiler from carrying
-        // those values from the B matrix to here instead (registers)
+        // the element's nodes, too, read again behind an opaque index
+        int ei = e;
+        asm volatile("" : "+v"(ei));
+        ld_conn(m.conn, ei, nd);
+        double x[8], y[8], z[8];
+        g3(s.x1, s.y1, s.z1, nd, x, y, z);
 #pragma unroll
         for (int i = 0; i < 8; ++i) asm volatile("" : "+v"(x[i]), "+v"(y[i]), "+v"(z[i]));
         hex_bmatrix_vjp(x, y, z, Bbf, xb, yb, zb);
@@ -976,12 +1001,14 @@ __global__ __launch_bounds__(128, 2) void k_adj_derived_pair(DMesh m, DState s, 
 // corner adjoints -> node adjoints, summed in corner-list order (no
 // atomics).  The block's corner rows are staged in LDS in sorted-corner order
 // (coalesced loads), the position part, then the velocity part; a block over
-// the capacity gathers from global memory.  assign: written, else added to.
+// the capacity gathers from global memory.  rowset (or null: every row
+// present): per element, whether its rows were written; an absent row is
+// zero and not read.  assign: written, else added to.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_adj_gather(DMesh m, const double* cbx, const double* cbv,
-                                                   double* xa0, double* xa1, double* xa2,
-                                                   double* va0, double* va1, double* va2,
-                                                   int assign)
+                                                   const uint8_t* rowset, double* xa0,
+                                                   double* xa1, double* xa2, double* va0,
+                                                   double* va1, double* va2, int assign)
 {
     const BlockItem bi = m.nblk[blockIdx.x];
     constexpr int CAPC = corner_stage_cap(BS);
@@ -999,6 +1026,10 @@ __global__ __launch_bounds__(BS) void k_adj_gather(DMesh m, const double* cbx, c
             const int j = r * BS + static_cast<int>(threadIdx.x);
             cid[r] = j < nc ? m.cs_cor[qb + j] : -1;
         }
+        // a slot whose row is absent is staged as zero, unread (-2)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (rowset && cid[r] >= 0 && !rowset[cid[r] >> 3]) cid[r] = -2;
     }
     int q0 = 0, cnt = 0;
     if (live) {
@@ -1013,16 +1044,19 @@ __global__ __launch_bounds__(BS) void k_adj_gather(DMesh m, const double* cbx, c
             double g[R][3];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const double* p = cb + 3 * static_cast<size_t>(cid[r] < 0 ? 0 : cid[r]);
-                g[r][0] = p[0];
-                g[r][1] = p[1];
-                g[r][2] = p[2];
+                g[r][0] = g[r][1] = g[r][2] = 0.0;
+                if (cid[r] >= 0) {
+                    const double* p = cb + 3 * static_cast<size_t>(cid[r]);
+                    g[r][0] = p[0];
+                    g[r][1] = p[1];
+                    g[r][2] = p[2];
+                }
             }
             if (part) __syncthreads();     // the position part's readers are done
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int j = r * BS + static_cast<int>(threadIdx.x);
-                if (cid[r] >= 0) {
+                if (cid[r] != -1) {
                     sC[3 * j] = g[r][0];
                     sC[3 * j + 1] = g[r][1];
                     sC[3 * j + 2] = g[r][2];
@@ -1031,10 +1065,17 @@ __global__ __launch_bounds__(BS) void k_adj_gather(DMesh m, const double* cbx, c
             __syncthreads();
         }
         for (int q = q0; q < q0 + cnt; ++q) {
-            const double* p = staged ? sC + 3 * m.cs_pos[q] : cb + 3 * static_cast<size_t>(m.cin[q]);
-            acc[3 * part] += p[0];
-            acc[3 * part + 1] += p[1];
-            acc[3 * part + 2] += p[2];
+            if (staged) {
+                const double* p = sC + 3 * m.cs_pos[q];
+                acc[3 * part] += p[0];
+                acc[3 * part + 1] += p[1];
+                acc[3 * part + 2] += p[2];
+            } else if (!rowset || rowset[m.cin[q] >> 3]) {
+                const double* p = cb + 3 * static_cast<size_t>(m.cin[q]);
+                acc[3 * part] += p[0];
+                acc[3 * part + 1] += p[1];
+                acc[3 * part + 2] += p[2];
+            }
         }
     }
     if (!live) return;
@@ -1043,11 +1084,12 @@ __global__ __launch_bounds__(BS) void k_adj_gather(DMesh m, const double* cbx, c
     for (int k = 0; k < 6; ++k) out[k][n] = assign ? acc[k] : acc[k] + out[k][n];
 }
 
-void launch_adj_gather(const DMesh& mn, const double* cbx, const double* cbv, double* const x[3],
-                       double* const v[3], int assign, hipStream_t st)
+void launch_adj_gather(const DMesh& mn, const double* cbx, const double* cbv,
+                       const uint8_t* rowset, double* const x[3], double* const v[3], int assign,
+                       hipStream_t st)
 {
     hipLaunchKernelGGL(k_adj_gather<kAdjNodeBlock>, dim3(mn.n_nblk), dim3(kAdjNodeBlock), 0, st,
-                       mn, cbx, cbv, x[0], x[1], x[2], v[0], v[1], v[2], assign);
+                       mn, cbx, cbv, rowset, x[0], x[1], x[2], v[0], v[1], v[2], assign);
 }
 
 // Final element density den = M / V(x) as an objective term: its seed sd
@@ -1529,26 +1571,28 @@ void reverse_stage(fsgh_ctx* c, int64_t cyc, int stage, const DState& s_in, cons
                                static_cast<const double*>(nullptr), A.lo, A.hi, A.alo, A.ahi,
                                static_cast<double2*>(nullptr), static_cast<double2*>(nullptr));
         });
-    timed(c, "adj_force", [&] {
-        // two lanes per element of the 64-element work items (small_block_mesh)
-        hipLaunchKernelGGL(k_adj_force_pair, dim3(m.n_eblk), dim3(128), 0, c->stream, m, s_in, A,
-                           static_cast<const InstCtl*>(c->d_ctl),
+    // two lanes per element of the 64-element work items (small_block_mesh)
+    const int setup = (cyc == 0 && stage == 0) ? 1 : 0;
+    timed(c, "adj_ext", [&] {
+        hipLaunchKernelGGL(k_adj_force_pair<true>, dim3(m.n_eblk), dim3(128), 0, c->stream, m,
+                           s_in, A, static_cast<const InstCtl*>(c->d_ctl),
                            static_cast<const double*>(s_in.vol),
-                           static_cast<const double*>(s_in.div));
+                           static_cast<const double*>(s_in.div),
+                           static_cast<const double*>(s_in.sie1), in.e, setup, assign);
     });
     timed(c, "adj_ext_node", [&] {
         hipLaunchKernelGGL(k_adj_ext_node, dim3(m.n_nblk), tb, 0, c->stream, m, A);
     });
-    timed(c, "adj_derived", [&] {
-        // two lanes per element of the 64-element work items (small_block_mesh)
-        hipLaunchKernelGGL(k_adj_derived_pair, dim3(m.n_eblk), dim3(128), 0, c->stream, m, s_in,
-                           A,
+    timed(c, "adj_force_derived", [&] {
+        hipLaunchKernelGGL(k_adj_force_pair<false>, dim3(m.n_eblk), dim3(128), 0, c->stream, m,
+                           s_in, A, static_cast<const InstCtl*>(c->d_ctl),
                            static_cast<const double*>(s_in.vol),
                            static_cast<const double*>(s_in.div),
-                           static_cast<const double*>(s_in.sie1), in.e,
-                           (cyc == 0 && stage == 0) ? 1 : 0, assign);
+                           static_cast<const double*>(s_in.sie1), in.e, setup, assign);
     });
-    timed(c, "adj_gather", [&] { launch_adj_gather(mn, A.cbx, A.cbv, in.x, in.v, assign, c->stream); });
+    timed(c, "adj_gather", [&] {
+        launch_adj_gather(mn, A.cbx, A.cbv, A.rowset, in.x, in.v, assign, c->stream);
+    });
 }
 
 int ensure_extreme_buffers(fsgh_ctx* c)
@@ -1640,8 +1684,8 @@ int ensure_reverse_buffers(fsgh_ctx* c)
     rc |= adj_alloc(c, &D.A.hk, 6 * N);
     rc |= adj_alloc(c, &D.A.cbx, 3 * C);
     rc |= adj_alloc(c, &D.A.cbv, 3 * C);
-    rc |= adj_alloc(c, &D.A.bbf, 3 * C);
-    rc |= adj_alloc(c, &D.A.eadj, 6 * E);
+    rc |= adj_alloc(c, &D.A.extb, E);
+    rc |= adj_alloc_t(c, &D.A.rowset, E);
     rc |= adj_alloc_t(c, &D.A.argext, E);
     rc |= adj_alloc(c, &D.A.nacc, 4 * N);
     if (rc) return rc == FSGH_E_HIP ? FSGH_E_HIP : FSGH_E_NOMEM;
@@ -1877,7 +1921,8 @@ int fsgh_backward(fsgh_ctx* c, const double* seed_x, const double* seed_v, const
         DState sf = make_state(c, 1 - c->cur, c->cur);
         hipLaunchKernelGGL(k_adj_seed_den, dim3(m.n_eblk), dim3(kB), 0, c->stream, m, sf,
                            static_cast<const double*>(D.grho), D.A.Mb, D.A.cbx, D.A.cbv);
-        launch_adj_gather(adjoint_node_mesh(c), D.A.cbx, D.A.cbv, O.x, O.v, 0, c->stream);
+        launch_adj_gather(adjoint_node_mesh(c), D.A.cbx, D.A.cbv, nullptr, O.x, O.v, 0,
+                          c->stream);
     }
     const int64_t K = D.every;
     const int64_t nseg = (D.ncyc + K - 1) / K;
