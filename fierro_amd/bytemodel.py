@@ -51,9 +51,10 @@ KERNELS = {"div_extremes": div_extremes, "force": force, "vel_bc_pos": vel_bc_po
 # adding to them; counted here as read + write throughout).  Full work: every
 # element and node as if its adjoint were nonzero.  The kernels skip what the
 # reverse of a zero adjoint leaves at zero (k_adj_energy per element, the
-# force/derived passes per wave, k_adj_node's corner forces per block,
-# k_adj_gather's absent rows), so where much of the mesh is not yet reached
-# by the adjoint they read and write less than these figures.
+# element passes per wave, k_adj_node's corner forces per block,
+# k_adj_gather's absent rows and row-less blocks), so where much of the mesh
+# is not yet reached by the adjoint they read and write less than these
+# figures.
 def adj_energy(E, N):
     # e-bar 8E + e0-bar 16E (rw) + M 8E + g 8E (w) + conn 32E + F 24C + M-bar 16E (rw)
     # + v0, v1 48N
@@ -64,8 +65,9 @@ def adj_node(E, N):
     # corner CSR starts 4(N+1) + sorted corner list 4C + positions 2C + F 24C + g 8E
     # + m 8N + BC mask 1N + v0, v1 48N + x-bar, v-bar of the stage output 48N
     # + x0-bar, v0-bar 96N (rw) + m-bar 16N (rw) + the (h, kv) records 48N (w)
+    # + kv flags 1N (w)
     return (4 * (N + 1) + 32 * E + 16 * E + 192 * E + 8 * E + 8 * N + N + 48 * N + 48 * N
-            + 96 * N + 16 * N + 48 * N)
+            + 96 * N + 16 * N + 48 * N + N)
 
 
 def arg_extremes(E, N):
@@ -76,24 +78,25 @@ def arg_extremes(E, N):
 
 def adj_ext(E, N):
     # conn 32E + x, v 48N + vol, div, den, sspd, pres 40E + extreme pairs and their
-    # elements 48N + g 8E + (h, kv) records 48N + ext-bar 8E (w) + chosen extreme 4E (w)
-    # (every wave's full recompute; a wave with no element whose limiter
-    # depends on the extreme reads only conn, vol, div and the extremes)
-    return 32 * E + 48 * N + 40 * E + 48 * N + 8 * E + 48 * N + 8 * E + 4 * E
+    # elements 48N + g 8E + (h, kv) records 48N + kv flags 1N + ext-bar 8E (w)
+    # + chosen extreme 4E (w)
+    return 32 * E + 48 * N + 40 * E + 48 * N + 8 * E + 48 * N + N + 8 * E + 4 * E
 
 
 def adj_ext_node(E, N):
     # corner CSR (4C + 4(N+1)) + chosen extreme 4E + ext-bar 8E + per-slot sums 32N (w)
-    return 32 * E + 4 * (N + 1) + 4 * E + 8 * E + 32 * N
+    # + slot-sum flags 1N (w)
+    return 32 * E + 4 * (N + 1) + 4 * E + 8 * E + 32 * N + N
 
 
 def adj_force_derived(E, N):
     # conn 32E + x, v 48N + vol, div, den, sspd, pres 40E + extreme pairs and their
-    # elements 48N + g 8E + (h, kv) records 48N + per-slot sums 32N + gamma, csmin
-    # 16E + e 8E + e-bar 16E (rw) + gamma-bar 16E (rw) + M-bar (or rho0-bar) 16E (rw)
-    # + corner velocity rows 24C (w) + corner position rows 24C (w) + row flag 1E (w)
-    return (32 * E + 48 * N + 40 * E + 48 * N + 8 * E + 48 * N + 32 * N + 16 * E + 8 * E
-            + 16 * E + 16 * E + 16 * E + 192 * E + 192 * E + E)
+    # elements 48N + g 8E + (h, kv) records 48N + per-slot sums 32N + kv and
+    # slot-sum flags 2N + gamma, csmin 16E + e 8E + e-bar 16E (rw) + gamma-bar 16E (rw)
+    # + M-bar (or rho0-bar) 16E (rw) + corner velocity rows 24C (w) + corner position
+    # rows 24C (w) + row flag 1E (w)
+    return (32 * E + 48 * N + 40 * E + 48 * N + 8 * E + 48 * N + 32 * N + 2 * N + 16 * E
+            + 8 * E + 16 * E + 16 * E + 16 * E + 192 * E + 192 * E + E)
 
 
 def adj_gather(E, N):

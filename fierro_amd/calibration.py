@@ -115,7 +115,8 @@ def objective(g, target, cycles, dt_replay=None):
 
 
 def gradient(g, params, target, cycles, checkpoint_every=10, dt_replay=None, layout=None):
-    """One batched forward + reverse pass over every instance of solver g.
+    """One batched forward + reverse pass over every instance of solver g,
+    which must hold its set-up state (a new solver, or one set up again).
     ``layout`` (a ``Layout`` of g, built once) holds the bins and origin
     elements; without it they are computed here.
 

@@ -373,6 +373,8 @@ struct AdjView {
     // rows of 24 doubles per element (8 corners x 3)
     double *cbx, *cbv;
     uint8_t* rowset;            // element: 1 if its rows were written this stage, 0 if zero
+    // node, this stage: kv is nonzero (k_adj_node); a slot sum is nonzero (k_adj_ext_node)
+    uint8_t *kvflag, *accflag;
     double* extb;               // element: adjoint of its neighbour extreme
     int32_t* argext;            // element: 4 * node + slot of its chosen extreme, or -1
     double* nacc;               // node: per-slot sums of ext-bar (4 x N)
@@ -538,10 +540,12 @@ __global__ __launch_bounds__(BS) void k_adj_node(DMesh m, DState s, const double
     }
     A.mb[n] += -k * dot / (mass * mass);
     const double km = k / mass;
+    const double kv[3] = {km * vm[0], km * vm[1], km * vm[2]};
     double2* rec = reinterpret_cast<double2*>(A.hk + 6 * static_cast<size_t>(n));
     rec[0] = make_double2(h[0], h[1]);
-    rec[1] = make_double2(h[2], km * vm[0]);
-    rec[2] = make_double2(km * vm[1], km * vm[2]);
+    rec[1] = make_double2(h[2], kv[0]);
+    rec[2] = make_double2(kv[1], kv[2]);
+    A.kvflag[n] = kv[0] != 0.0 || kv[1] != 0.0 || kv[2] != 0.0;
 }
 
 // Shock-detector neighbour term: add to each element's divergence adjoint
@@ -585,6 +589,7 @@ __global__ __launch_bounds__(kB) void k_adj_ext_node(DMesh m, AdjView A)
     double2* out = reinterpret_cast<double2*>(A.nacc + 4 * static_cast<size_t>(n));
     out[0] = make_double2(acc[0], acc[1]);
     out[1] = make_double2(acc[2], acc[3]);
+    A.accflag[n] = acc[0] != 0.0 || acc[1] != 0.0 || acc[2] != 0.0 || acc[3] != 0.0;
 }
 
 // ---- one element over a pair of lanes ----
@@ -631,23 +636,10 @@ __device__ __forceinline__ void wave_store_half_rows(double* dst_, int e0, int n
     }
 }
 
-// Whether any of this lane's four corners has a nonzero force adjoint
-// F-bar = g_e h + kv (formed as the reverse forms it).  The node indices go
-// through an opaque copy, so that the reverse reads the records again (cache
-// hits) rather than keeping these loads live across the forward recompute.
-__device__ __forceinline__ bool corners_seeded(const double* hk, double ge, const int32_t n4[4])
+// Whether either lane of the pair holds a true flag.
+__device__ __forceinline__ bool pair_or(bool b)
 {
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        int n = n4[k];
-        asm volatile("" : "+v"(n));
-        const double2* rec = reinterpret_cast<const double2*>(hk + 6 * static_cast<size_t>(n));
-        const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-        any = any || (ge * r0.x + r1.y != 0.0) || (ge * r0.y + r2.x != 0.0) ||
-              (ge * r1.x + r2.y != 0.0);
-    }
-    return any;
+    return (static_cast<int>(b) | __shfl_xor(static_cast<int>(b), 1, 64)) != 0;
 }
 
 // Reverse of force_elem (K/force_sgh.cpp:35-359) at the stage input state,
@@ -700,16 +692,24 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
     const int nvalid = bi.end - bi.start - 32 * wave;   // this wave's elements in the item
     const int e0 = bi.start + 32 * wave;
     double2* buf = hbuf + wave * 64 * 7;
-    // The reverse of a zero adjoint is zero.  ext-bar = -phi-bar alpha 0.9 /
-    // (div + small) is zero where every corner's force adjoint is (then
-    // phi-bar is) and where alpha is (div = 0); the element's rows and element
-    // adjoints are zero where, in addition, its neighbour term is.  A wave
-    // with no such element stops here (wave-uniform branches).
-    bool seeded = pair_sum(corners_seeded(A.hk, ge, n4) ? 1.0 : 0.0) != 0.0;
+    // The reverse of a zero adjoint is zero.  A corner's force adjoint
+    // g_e h + kv is zero where g_e (ge) and its node's kv are (kvflag);
+    // ext-bar = -phi-bar alpha 0.9 / (div + small) is zero where every
+    // corner's force adjoint is (then phi-bar is) and where alpha is
+    // (div = 0); the element's rows and element adjoints are zero where, in
+    // addition, its neighbour term is (no nonzero slot sum at its nodes,
+    // accflag).  A wave with no element left stops here (wave-uniform
+    // branches).
+    bool kvs = ge != 0.0, accs = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        kvs = kvs || A.kvflag[n4[k]] != 0;
+        if (!EXT) accs = accs || A.accflag[n4[k]] != 0;
+    }
+    const bool seeded = pair_or(kvs);
     double divnb_all = 0.0;
     if (EXT) {
-        seeded = seeded && div != 0.0;
-        if (!__any(live && seeded)) {
+        if (!__any(live && seeded && div != 0.0)) {
             if (live && h == 0) {
                 A.extb[e] = 0.0;
                 A.argext[e] = -1;   // no slot to add to
@@ -717,6 +717,14 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
             return;
         }
     } else {
+        if (!__any(live && (seeded || pair_or(accs)))) {
+            // rows absent (k_adj_gather reads them as zero)
+            if (live && h == 0) {
+                if (assign) sieb_in[e] = 0.0;
+                A.rowset[e] = 0;
+            }
+            return;
+        }
         // the neighbour term of the divergence adjoint: the per-slot sums
         // (k_adj_ext_node) of the slots this element occupies at this lane's
         // four nodes
@@ -731,14 +739,6 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
                      (ih.x == e ? s23.x : 0.0) + (ih.y == e ? s23.y : 0.0);
         }
         divnb_all = pair_sum(divnb);
-        if (!__any(live && (seeded || divnb_all != 0.0))) {
-            // rows absent (k_adj_gather reads them as zero)
-            if (live && h == 0) {
-                if (assign) sieb_in[e] = 0.0;
-                A.rowset[e] = 0;
-            }
-            return;
-        }
     }
     const double vol = vol_a[e], den = s.den[e], sspd = s.sspd[e], pres = s.pres[e];
     const Material mt = s.mats[s.nmat > 1 ? s.mat[e] : 0];
@@ -1027,9 +1027,21 @@ __global__ __launch_bounds__(BS) void k_adj_gather(DMesh m, const double* cbx, c
             cid[r] = j < nc ? m.cs_cor[qb + j] : -1;
         }
         // a slot whose row is absent is staged as zero, unread (-2)
+        bool any = false;
 #pragma unroll
-        for (int r = 0; r < R; ++r)
+        for (int r = 0; r < R; ++r) {
             if (rowset && cid[r] >= 0 && !rowset[cid[r] >> 3]) cid[r] = -2;
+            any = any || cid[r] >= 0;
+        }
+        // a block with no row present: its nodes' sums are zero (block-uniform)
+        if (!__syncthreads_or(any)) {
+            if (live && assign) {
+                double* out[6] = {xa0, xa1, xa2, va0, va1, va2};
+#pragma unroll
+                for (int k = 0; k < 6; ++k) out[k][n] = 0.0;
+            }
+            return;
+        }
     }
     int q0 = 0, cnt = 0;
     if (live) {
@@ -1686,6 +1698,8 @@ int ensure_reverse_buffers(fsgh_ctx* c)
     rc |= adj_alloc(c, &D.A.cbv, 3 * C);
     rc |= adj_alloc(c, &D.A.extb, E);
     rc |= adj_alloc_t(c, &D.A.rowset, E);
+    rc |= adj_alloc_t(c, &D.A.kvflag, N);
+    rc |= adj_alloc_t(c, &D.A.accflag, N);
     rc |= adj_alloc_t(c, &D.A.argext, E);
     rc |= adj_alloc(c, &D.A.nacc, 4 * N);
     if (rc) return rc == FSGH_E_HIP ? FSGH_E_HIP : FSGH_E_NOMEM;
@@ -1797,6 +1811,12 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
         set_error(c, "fsgh_forward: materials not set");
         return FSGH_E_STATE;
     }
+    if (any_advanced(c)) {
+        set_error(c, "fsgh_forward: the state was advanced (fsgh_run, fsgh_forward or a "
+                     "state-update operator) since fsgh_setup_fills or an upload set it; set "
+                     "the initial state again");
+        return FSGH_E_STATE;
+    }
     free_adjoint(c);
     c->adj = new AdjointData();
     AdjointData& D = *c->adj;
@@ -1858,6 +1878,7 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
             return FSGH_E_INVALID;
         }
     D.done = true;
+    mark_advanced(c);
     return FSGH_OK;
 }
 

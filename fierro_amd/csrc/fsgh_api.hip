@@ -20,6 +20,22 @@ void set_error(fsgh_ctx* ctx, const std::string& msg)
 {
     if (ctx) ctx->err = msg;
 }
+
+void mark_advanced(fsgh_ctx* c) { c->advanced.assign(static_cast<size_t>(c->L.ninst), 1); }
+
+void mark_set(fsgh_ctx* c, int32_t inst)
+{
+    if (c->advanced.size() != static_cast<size_t>(c->L.ninst) || inst < 0)
+        c->advanced.assign(static_cast<size_t>(c->L.ninst), 0);
+    if (inst >= 0) c->advanced[static_cast<size_t>(inst)] = 0;
+}
+
+bool any_advanced(const fsgh_ctx* c)
+{
+    for (uint8_t a : c->advanced)
+        if (a) return true;
+    return false;
+}
 }  // namespace fsgh
 
 
@@ -629,8 +645,22 @@ int fsgh_bc_set_size(fsgh_ctx* c, int32_t inst, int32_t set, int64_t* count)
 }
 
 // K/setup.cpp:106-424 for one instance, evaluated on the host in caller numbering.
+
+static int setup_fills_body(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill* fills,
+                            int32_t nmat, int32_t nsv, const double* state_vars,
+                            const int32_t* mat_nsv);
+
 int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill* fills,
                      int32_t nmat, int32_t nsv, const double* state_vars, const int32_t* mat_nsv)
+{
+    const int rc = setup_fills_body(c, inst, nfills, fills, nmat, nsv, state_vars, mat_nsv);
+    if (rc == FSGH_OK) mark_set(c, inst);
+    return rc;
+}
+
+static int setup_fills_body(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill* fills,
+                            int32_t nmat, int32_t nsv, const double* state_vars,
+                            const int32_t* mat_nsv)
 {
     CHECK_CTX(c);
     if (c->host_only) return FSGH_E_STATE;
@@ -904,7 +934,9 @@ static int field_io(fsgh_ctx* c, int32_t field, void* host, size_t bytes, bool u
 
 int fsgh_upload(fsgh_ctx* c, int32_t field, const void* host, size_t bytes)
 {
-    return field_io(c, field, const_cast<void*>(host), bytes, true);
+    const int rc = field_io(c, field, const_cast<void*>(host), bytes, true);
+    if (rc == FSGH_OK) mark_set(c, -1);
+    return rc;
 }
 
 int fsgh_download(fsgh_ctx* c, int32_t field, void* host, size_t bytes)
@@ -1001,6 +1033,7 @@ int fsgh_get_force_sgh(fsgh_ctx* c, double fuzz, double small, double dt, double
 int fsgh_update_velocity_sgh(fsgh_ctx* c, double rk_alpha, double dt)
 {
     OP_PROLOGUE(c);
+    mark_advanced(c);
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
     timed(c, "update_velocity", [&] {
@@ -1017,6 +1050,7 @@ int fsgh_update_velocity_sgh(fsgh_ctx* c, double rk_alpha, double dt)
 int fsgh_boundary_velocity(fsgh_ctx* c)
 {
     OP_PROLOGUE(c);
+    mark_advanced(c);
     DState s = ref_state(c);
     timed(c, "boundary_velocity",
           [&] { launch_boundary_velocity(c->d_bcnodes, c->d_bcmasks, c->n_bcnodes, s, c->stream); });
@@ -1026,6 +1060,7 @@ int fsgh_boundary_velocity(fsgh_ctx* c)
 int fsgh_update_energy_sgh(fsgh_ctx* c, double rk_alpha, double dt)
 {
     OP_PROLOGUE(c);
+    mark_advanced(c);
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
     timed(c, "update_energy", [&] {
@@ -1041,6 +1076,7 @@ int fsgh_update_energy_sgh(fsgh_ctx* c, double rk_alpha, double dt)
 int fsgh_update_position_sgh(fsgh_ctx* c, double rk_alpha, double dt)
 {
     OP_PROLOGUE(c);
+    mark_advanced(c);
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
     timed(c, "update_position", [&] { launch_update_position(m, s, rk_alpha, dt, c->stream); });
@@ -1062,6 +1098,7 @@ int fsgh_get_vol(fsgh_ctx* c)
 int fsgh_update_state(fsgh_ctx* c, double dt, double rk_alpha)
 {
     OP_PROLOGUE(c);
+    mark_advanced(c);
     DMesh m = make_mesh(c);
     DState s = ref_state(c);
     timed(c, "update_state", [&] {
@@ -1290,6 +1327,7 @@ int fsgh_run(fsgh_ctx* c, int64_t ncycles, int32_t flags)
         return FSGH_E_STATE;
     }
     if (ncycles < 0) return FSGH_E_INVALID;
+    if (ncycles > 0) mark_advanced(c);
     const bool graph = (flags & FSGH_RUN_GRAPH) != 0 && !c->prof;
     const bool stop_on_output = (flags & FSGH_RUN_STOP_ON_OUTPUT) != 0;
     const int64_t poll = stop_on_output ? 1 : 32;

@@ -346,7 +346,9 @@ class SGHSolver:
     def forward(self, ncycles, checkpoint_every=20, dt_replay=None):
         """Runs ``ncycles`` cycles of every instance from the set-up state,
         recording dt and checkpointing the state every ``checkpoint_every``
-        cycles; ``dt_replay`` ([ncycles][instances]) replays given steps."""
+        cycles; ``dt_replay`` ([ncycles][instances]) replays given steps.
+        Raises FsghError if the state was advanced since the set-up or the
+        last uploads (set it again first)."""
         rep = None
         if dt_replay is not None:
             rep = np.ascontiguousarray(dt_replay, np.float64).reshape(-1)

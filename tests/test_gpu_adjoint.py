@@ -109,6 +109,27 @@ def test_forward_is_fused_loop(Solver):
     assert np.array_equal(c.download("node_coords"), a.download("node_coords"))
 
 
+def test_forward_refuses_advanced_state(Solver):
+    """fsgh_forward differentiates from the set-up state: on a state that a
+    run has advanced it fails (FSGH_E_STATE) until the state is set again."""
+    from fierro_amd._abi import FsghError
+
+    spec = P.anvil(4, 4, 12, cycles=10)
+    g = Solver(spec)
+    st = {k: g.download(k) for k in INIT}
+    g.forward(10, checkpoint_every=5)
+    x1 = g.download("node_coords")
+    with pytest.raises(FsghError, match="advanced"):
+        g.forward(10, checkpoint_every=5)
+    load(g, st)
+    g.forward(10, checkpoint_every=5)
+    assert np.array_equal(g.download("node_coords"), x1)
+    h = Solver(spec)
+    h.sgh_solve(cycles=3, use_graph=False)
+    with pytest.raises(FsghError, match="advanced"):
+        h.forward(10, checkpoint_every=5)
+
+
 def test_dt_history_matches_oracle(oracle, Solver):
     spec = P.anvil(4, 4, 12, cycles=30)
     g = Solver(spec, renumber=True)
