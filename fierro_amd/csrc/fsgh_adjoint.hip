@@ -367,7 +367,8 @@ struct AdjView {
     // node: the stage's mean velocity h = (v1 + v0)/2 and kv = alpha dt / m
     // times the masked velocity adjoint, one 48-byte record per node
     // (h0, h1, h2, kv0, kv1, kv2); a corner's force adjoint is
-    // F-bar = g_e h + kv at its node
+    // F-bar = g_e h + kv at its node.  Every h it holds is finite (zeroed at
+    // allocation; a quiet node block of k_adj_node keeps the earlier one)
     double* hk;
     // corners, element order: position (cbx) and velocity (cbv) adjoints,
     // rows of 24 doubles per element (8 corners x 3)
@@ -446,36 +447,39 @@ __global__ __launch_bounds__(BS) void k_adj_node(DMesh m, DState s, const double
     const bool staged = m.cs_cor != nullptr && nc <= CAPC;
     const int n = bi.start + static_cast<int>(threadIdx.x);
     const bool live = n < bi.end;
-    // the stage output's adjoints; seeded: the node's velocity adjoint
-    // without the energy term is nonzero
-    const uint8_t mk = live ? s.bcmask[n] : 0;
+    // the stage output's adjoints
     double xo[3] = {0.0, 0.0, 0.0}, vo[3] = {0.0, 0.0, 0.0};
-    bool seeded = false;
     if (live) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             xo[d] = A.xo[d][n];
             vo[d] = A.vo[d][n];
-            seeded = seeded || (!(mk & (1u << d)) && vo[d] + 0.5 * k * xo[d] != 0.0);
         }
     }
-    // The corner forces enter through g F (energy) and v-bar . sum F (mass):
-    // a staged block whose corners' g and nodes' velocity adjoints are all
-    // zero does not read them (block-uniform).
-    bool needF = true;
+    // A staged block whose corners' g and nodes' stage-output adjoints are
+    // all zero is quiet (block-uniform): its outputs are zero, so it reads
+    // no corner forces or velocities, leaves its accumulators as they are
+    // (writes zeros where it assigns) and writes kv = 0 with its flag; the
+    // records keep their h (finite: zero from the allocation or an earlier
+    // stage's mean velocity), which enters a corner's force adjoint only as
+    // g_e h with g_e = 0 at every corner of these nodes.
+    bool quiet = false;
     if (staged) {
         constexpr int R = CAPC / BS;
         int cid[R];
         double ge[R];
+        bool nz = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) nz = nz || xo[d] != 0.0 || vo[d] != 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = r * BS + static_cast<int>(threadIdx.x);
             cid[r] = j < nc ? m.cs_cor[qb + j] : -1;
             ge[r] = cid[r] >= 0 ? A.ge[cid[r] >> 3] : 0.0;
-            seeded = seeded || ge[r] != 0.0;
+            nz = nz || ge[r] != 0.0;
         }
-        needF = __syncthreads_or(seeded);
-        if (needF) {
+        quiet = !__syncthreads_or(nz);
+        if (!quiet) {
             double g[R][3];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -498,10 +502,25 @@ __global__ __launch_bounds__(BS) void k_adj_node(DMesh m, DState s, const double
         }
     }
     if (!live) return;
+    if (quiet) {
+        if (assign) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                A.x0b[d][n] = 0.0;
+                A.v0b[d][n] = 0.0;
+            }
+        }
+        double* rec = A.hk + 6 * static_cast<size_t>(n);
+        rec[3] = 0.0;
+        reinterpret_cast<double2*>(rec)[2] = make_double2(0.0, 0.0);
+        A.kvflag[n] = 0;
+        return;
+    }
+    if (!live) return;
     const int q0 = m.cin_start[n];
     const int cnt = m.cin_start[n + 1] - q0;
     double sh[3] = {0.0, 0.0, 0.0}, sf[3] = {0.0, 0.0, 0.0};
-    for (int q = q0; needF && q < q0 + cnt; ++q) {
+    for (int q = q0; q < q0 + cnt; ++q) {
         double f[3], g;
         if (staged) {
             const double* p = sF + 4 * m.cs_pos[q];
@@ -524,6 +543,7 @@ __global__ __launch_bounds__(BS) void k_adj_node(DMesh m, DState s, const double
         }
     }
     const double mass = s.nmass[n];
+    const uint8_t mk = s.bcmask[n];
     const double h[3] = {(s.u1[n] + s.u0[n]) * 0.5, (s.v1[n] + s.v0[n]) * 0.5,
                          (s.w1[n] + s.w0[n]) * 0.5};
     double vm[3];
@@ -1707,6 +1727,8 @@ int ensure_reverse_buffers(fsgh_ctx* c)
     D.A.hi = D.hi;
     D.A.alo = D.alo;
     D.A.ahi = D.ahi;
+    // the node records start finite (k_adj_node's quiet blocks keep their h)
+    segcopy(c, {Seg{D.A.hk, nullptr, 6 * N}});
     D.rev_ready = true;
     return FSGH_OK;
 }
