@@ -109,6 +109,36 @@ def test_forward_is_fused_loop(Solver):
     assert np.array_equal(c.download("node_coords"), a.download("node_coords"))
 
 
+def test_dot_product_sparse_seed(Solver):
+    """A seed on a few nodes and elements near the origin: the adjoint
+    spreads about two element layers per reversed stage, so over 3 cycles
+    of a 32^3 mesh most waves and node blocks are never reached and the
+    reverse pass skips them; the tangent model skips nothing, and
+    <Jv,w> = <v,J^T w> holds as with a dense seed."""
+    spec = P.sedov(32, cycles=3)
+    g = Solver(spec, renumber=True)
+    g.forward(3, checkpoint_every=2)
+    rng = np.random.default_rng(11)
+    v = random_dir(rng, g.N, g.E)
+    x1 = g.download("node_coords")[1]
+    near = np.argsort(np.linalg.norm(x1, axis=1))[:6]
+    w = dict(x=np.zeros((g.N, 3)), v=np.zeros((g.N, 3)), e=np.zeros(g.E))
+    w["v"][near] = rng.standard_normal((6, 3))
+    w["x"][near[3:]] = rng.standard_normal((3, 3))
+    elems = np.where(np.isin(g.conn, near).any(axis=1))[0]
+    w["e"][elems] = rng.standard_normal(elems.size)
+    t = g.tangent(v["den0"], v["sie0"], v["gamma"], v["vel0"])
+    grad = g.backward(w["x"], w["v"], w["e"])
+    lhs = sum(float(np.sum(t[k] * w[k])) for k in ("x", "v", "e"))
+    rhs = dot_grad(grad, v)
+    print(f"sparse seed: <Jv,w> = {lhs:.17e}  <v,J^T w> = {rhs:.17e}, "
+          f"{np.count_nonzero(grad['den0'])} of {g.E} elements reached")
+    assert rhs != 0.0
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), abs(rhs))
+    # most of the mesh is not reached (its gradient is zero)
+    assert np.count_nonzero(grad["den0"]) < g.E // 2
+
+
 def test_forward_refuses_advanced_state(Solver):
     """fsgh_forward differentiates from the set-up state: on a state that a
     run has advanced it fails (FSGH_E_STATE) until the state is set again."""
