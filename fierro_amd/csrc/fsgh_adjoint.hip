@@ -1684,12 +1684,14 @@ int ensure_reverse_buffers(fsgh_ctx* c)
     const size_t K = static_cast<size_t>(std::min<int64_t>(D.every, D.ncyc));
     // a second bank (recompute overlapped with the reverse pass) when there
     // is more than one segment and both banks, with the reverse pass's other
-    // buffers (~110 doubles per element), take under three quarters of the
-    // free device memory; with per-kernel timing on (fsgh_profile_enable) the
+    // buffers (three adjoint sets; per element the corner rows, 48 doubles,
+    // and ~8 element arrays; per node ~20 doubles of records, slot sums,
+    // extremes and flags), take under three quarters of the free device
+    // memory; with per-kernel timing on (fsgh_profile_enable) the
     // pass keeps one bank on the context stream, so that each reverse kernel's
     // events time it alone
     const size_t bank_bytes = 8 * ((K + 1) * (6 * N + 6 * E) + K * (18 * N + 54 * E));
-    const size_t other_bytes = 8 * (3 * (6 * N + E) + 6 * N + 90 * E);
+    const size_t other_bytes = 8 * (3 * (6 * N + E) + 20 * N + 56 * E);
     size_t mfree = 0, mtotal = 0;
     const bool two = !c->prof && D.ncyc > D.every &&
                      hipMemGetInfo(&mfree, &mtotal) == hipSuccess &&
