@@ -1837,8 +1837,9 @@ int fsgh_forward(fsgh_ctx* c, const fsgh_run_params* params, int64_t ncycles,
     }
     if (any_advanced(c)) {
         set_error(c, "fsgh_forward: the state was advanced (fsgh_run, fsgh_forward or a "
-                     "state-update operator) since fsgh_setup_fills or an upload set it; set "
-                     "the initial state again");
+                     "state-update operator) and its coordinates, velocities and energies "
+                     "have not all been set again since (uploads, or fsgh_setup_fills for "
+                     "velocities and energies); set the initial state again");
         return FSGH_E_STATE;
     }
     free_adjoint(c);

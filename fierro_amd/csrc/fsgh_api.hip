@@ -21,19 +21,20 @@ void set_error(fsgh_ctx* ctx, const std::string& msg)
     if (ctx) ctx->err = msg;
 }
 
-void mark_advanced(fsgh_ctx* c) { c->advanced.assign(static_cast<size_t>(c->L.ninst), 1); }
+void mark_advanced(fsgh_ctx* c) { c->set_since.assign(static_cast<size_t>(c->L.ninst), 0); }
 
-void mark_set(fsgh_ctx* c, int32_t inst)
+void mark_set(fsgh_ctx* c, int32_t inst, uint8_t parts)
 {
-    if (c->advanced.size() != static_cast<size_t>(c->L.ninst) || inst < 0)
-        c->advanced.assign(static_cast<size_t>(c->L.ninst), 0);
-    if (inst >= 0) c->advanced[static_cast<size_t>(inst)] = 0;
+    if (c->set_since.size() != static_cast<size_t>(c->L.ninst))
+        c->set_since.assign(static_cast<size_t>(c->L.ninst), 7);   // never advanced
+    for (size_t i = 0; i < c->set_since.size(); ++i)
+        if (inst < 0 || static_cast<size_t>(inst) == i) c->set_since[i] |= parts;
 }
 
 bool any_advanced(const fsgh_ctx* c)
 {
-    for (uint8_t a : c->advanced)
-        if (a) return true;
+    for (uint8_t a : c->set_since)
+        if (a != 7) return true;
     return false;
 }
 }  // namespace fsgh
@@ -654,7 +655,7 @@ int fsgh_setup_fills(fsgh_ctx* c, int32_t inst, int32_t nfills, const fsgh_fill*
                      int32_t nmat, int32_t nsv, const double* state_vars, const int32_t* mat_nsv)
 {
     const int rc = setup_fills_body(c, inst, nfills, fills, nmat, nsv, state_vars, mat_nsv);
-    if (rc == FSGH_OK) mark_set(c, inst);
+    if (rc == FSGH_OK) mark_set(c, inst, 2 | 4);   // velocities and energies
     return rc;
 }
 
@@ -935,7 +936,9 @@ static int field_io(fsgh_ctx* c, int32_t field, void* host, size_t bytes, bool u
 int fsgh_upload(fsgh_ctx* c, int32_t field, const void* host, size_t bytes)
 {
     const int rc = field_io(c, field, const_cast<void*>(host), bytes, true);
-    if (rc == FSGH_OK) mark_set(c, -1);
+    if (rc == FSGH_OK)
+        mark_set(c, -1, field == FSGH_NODE_COORDS ? 1 : field == FSGH_NODE_VEL ? 2
+                        : field == FSGH_ELEM_SIE ? 4 : 0);
     return rc;
 }
 

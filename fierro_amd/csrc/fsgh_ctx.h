@@ -92,9 +92,11 @@ struct fsgh_ctx {
     std::vector<int64_t> eoff32, noff32;
     int cur = 1;                            // physical bin holding reference bin 1
     bool run_active = false;
-    // per instance: its state was advanced (fsgh_run, fsgh_forward, the
-    // state-update operators) since fsgh_setup_fills or an upload last set it
-    std::vector<uint8_t> advanced;
+    // per instance: which of coordinates (1), velocities (2) and energies (4)
+    // were set again (uploads; fsgh_setup_fills sets 2 and 4) since the state
+    // was last advanced (fsgh_run, fsgh_forward, the state-update operators);
+    // 7 for an instance never advanced
+    std::vector<uint8_t> set_since;
     bool host_only = false;                 // device < 0: host-side maps only
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     // profiling
@@ -121,10 +123,12 @@ struct fsgh_ctx {
 namespace fsgh {
 // shared by the ABI translation units
 DState make_state(fsgh_ctx* c, int b0, int b1);
-// the per-instance "advanced since set" flags: every instance advanced, one
-// instance set (inst < 0: every instance set), or whether any is advanced
+// the per-instance record of what was set since the state last advanced:
+// every instance advanced; parts (bits of coordinates 1, velocities 2,
+// energies 4) set for one instance (inst < 0: every instance); whether any
+// instance is advanced and not wholly set again
 void mark_advanced(fsgh_ctx* c);
-void mark_set(fsgh_ctx* c, int32_t inst);
+void mark_set(fsgh_ctx* c, int32_t inst, uint8_t parts);
 bool any_advanced(const fsgh_ctx* c);
 DMesh make_mesh(fsgh_ctx* c);
 // the mesh view with the work tables of a tuned kernel's configuration

@@ -364,8 +364,10 @@ int fsgh_get_tuning(fsgh_ctx *ctx, fsgh_tuning *out, int32_t count);
  * initial state); runs `ncycles` cycles of every instance from time 0,
  * recording dt per cycle and instance and storing the state (x, v, e) every
  * `checkpoint_every` cycles.  FSGH_E_STATE if an instance's state was
- * advanced (fsgh_run, fsgh_forward, the state-update operators) since
- * fsgh_setup_fills or an upload last set it.  dt_replay (may be NULL; [ncycles][instances])
+ * advanced (fsgh_run, fsgh_forward, the state-update operators) and its
+ * coordinates, velocities and energies have not all been set again since
+ * (fsgh_upload of FSGH_NODE_COORDS, FSGH_NODE_VEL, FSGH_ELEM_SIE;
+ * fsgh_setup_fills sets velocities and energies).  dt_replay (may be NULL; [ncycles][instances])
  * replays given time steps instead of computing them.  Every instance must
  * run all ncycles (time_final and cycle_stop not reached earlier), otherwise
  * FSGH_E_INVALID.  The state afterwards is the forward result, as after

@@ -151,6 +151,12 @@ def test_forward_refuses_advanced_state(Solver):
     x1 = g.download("node_coords")
     with pytest.raises(FsghError, match="advanced"):
         g.forward(10, checkpoint_every=5)
+    # setting gamma, velocities and energies again is not enough: the
+    # coordinates are still the advanced ones
+    for k in ("elem_gamma", "node_vel", "elem_sie"):
+        g.upload(k, st[k])
+    with pytest.raises(FsghError, match="advanced"):
+        g.forward(10, checkpoint_every=5)
     load(g, st)
     g.forward(10, checkpoint_every=5)
     assert np.array_equal(g.download("node_coords"), x1)
