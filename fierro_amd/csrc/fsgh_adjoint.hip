@@ -378,7 +378,7 @@ struct AdjView {
     uint8_t *kvflag, *accflag;
     double* extb;               // element: adjoint of its neighbour extreme
     int32_t* argext;            // element: 4 * node + slot of its chosen extreme, or -1
-    double* nacc;               // node: per-slot sums of ext-bar (4 x N)
+    double* nacc;               // node: per-slot sums of ext-bar (4 x N), where accflag
     double2 *lo, *hi;
     int2 *alo, *ahi;
 };
@@ -606,10 +606,15 @@ __global__ __launch_bounds__(kB) void k_adj_ext_node(DMesh m, AdjView A)
         const int cd = A.argext[e2];
         if (cd >= 0 && (cd >> 2) == n) acc[cd & 3] += A.extb[e2];
     }
-    double2* out = reinterpret_cast<double2*>(A.nacc + 4 * static_cast<size_t>(n));
-    out[0] = make_double2(acc[0], acc[1]);
-    out[1] = make_double2(acc[2], acc[3]);
-    A.accflag[n] = acc[0] != 0.0 || acc[1] != 0.0 || acc[2] != 0.0 || acc[3] != 0.0;
+    // the sums are written only where one is nonzero (accflag; the element
+    // pass reads them only there)
+    const bool any = acc[0] != 0.0 || acc[1] != 0.0 || acc[2] != 0.0 || acc[3] != 0.0;
+    if (any) {
+        double2* out = reinterpret_cast<double2*>(A.nacc + 4 * static_cast<size_t>(n));
+        out[0] = make_double2(acc[0], acc[1]);
+        out[1] = make_double2(acc[2], acc[3]);
+    }
+    A.accflag[n] = any;
 }
 
 // ---- one element over a pair of lanes ----
@@ -747,16 +752,18 @@ __global__ __launch_bounds__(128, 2) void k_adj_force_pair(DMesh m, DState s, Ad
         }
         // the neighbour term of the divergence adjoint: the per-slot sums
         // (k_adj_ext_node) of the slots this element occupies at this lane's
-        // four nodes
+        // four nodes, read where a node has a nonzero one (accflag)
         double divnb = 0.0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int2 il = A.alo[n4[k]], ih = A.ahi[n4[k]];
-            const double2* p =
-                reinterpret_cast<const double2*>(A.nacc + 4 * static_cast<size_t>(n4[k]));
-            const double2 s01 = p[0], s23 = p[1];
-            divnb += (il.x == e ? s01.x : 0.0) + (il.y == e ? s01.y : 0.0) +
-                     (ih.x == e ? s23.x : 0.0) + (ih.y == e ? s23.y : 0.0);
+            if (A.accflag[n4[k]]) {
+                const int2 il = A.alo[n4[k]], ih = A.ahi[n4[k]];
+                const double2* p =
+                    reinterpret_cast<const double2*>(A.nacc + 4 * static_cast<size_t>(n4[k]));
+                const double2 s01 = p[0], s23 = p[1];
+                divnb += (il.x == e ? s01.x : 0.0) + (il.y == e ? s01.y : 0.0) +
+                         (ih.x == e ? s23.x : 0.0) + (ih.y == e ? s23.y : 0.0);
+            }
         }
         divnb_all = pair_sum(divnb);
     }
